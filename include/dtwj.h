@@ -1,0 +1,173 @@
+/*
+ * dtwj.h -- C ABI of the MI355X (gfx950) top-1 subsequence similarity join under
+ * Sakoe-Chiba-banded DTW.
+ *
+ * Problem (same as the reference join, CPU_version1/new_dtw_motifGUI.cpp:21-586 and
+ * GPU_version1/src/new_dtw_motifGUI_malloc.cu:30-816):
+ *   given a series a[0..n), a window length m and a band radius w, find the pair of
+ *   windows (i, j), 0 <= i, j - i >= m (trivial-match exclusion, minlag = m,
+ *   CPU_version1/new_dtw_motifGUI.cpp:182,364), minimising
+ *       DTW_w(z_i, z_j) = sqrt(D[m-1][m-1]),
+ *       D[r][c] = (z_i[r] - z_j[c])^2 + min(D[r-1][c], D[r][c-1], D[r-1][c-1]), |r - c| <= w,
+ *   where z_i[k] = (a[i+k] - mu_i) / sigma_i with the population standard deviation
+ *   (CPU_version1/allmatrixop/movmean.cpp:9-33, CPU_version1/new_dtw_motifGUI.cpp:41-44).
+ *   Exact ties are resolved towards the smallest i, then the smallest j.
+ *
+ * All indices in this ABI are 0-based.  The reference prints 1-based indices
+ * ("first min is i+1 sec min is j+1", CPU_version1/new_dtw_motifGUI.cpp:541-545,575);
+ * the gpu_dtw command-line tool reproduces that line.
+ *
+ * Ownership: callers own every host buffer passed in; they are only read during the
+ * call.  Device memory is owned by the library (per call, or by a dtwj_context).
+ * Errors are returned as codes, never by exiting; dtwj_last_error() gives a message
+ * (thread-local).
+ */
+#ifndef DTWJ_H
+#define DTWJ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------- */
+#define DTWJ_OK 0
+#define DTWJ_EINVAL 1    /* bad argument: m < 4, w < 0, n < 2m, NaN/Inf, constant window */
+#define DTWJ_ENOMEM 2    /* host or device allocation failed */
+#define DTWJ_EDEVICE 3   /* no usable HIP device, or a HIP call failed */
+#define DTWJ_EIO 4       /* file could not be opened / parsed / written */
+#define DTWJ_EOVERFLOW 5 /* an internal candidate buffer overflowed */
+
+/* ---- options and results --------------------------------------------------------- */
+typedef struct dtwj_options {
+    int32_t device;        /* HIP device ordinal (default 0) */
+    int32_t seed_items;    /* ED-scan seed work items before the join (default 0 = automatic) */
+    int32_t rows_per_item; /* rows per sweep work item, multiple of 128 (default 0 = automatic) */
+    int32_t waves_per_cu;  /* persistent waves per CU (default 0 = automatic) */
+    int32_t verbose;       /* print per-stage counters to stderr */
+    int32_t reserved[7];
+} dtwj_options;
+
+typedef struct dtwj_result {
+    int64_t i, j;        /* 0-based winning pair, i < j, j - i >= m */
+    double dist;         /* FP64-verified banded DTW distance of (i, j) */
+    double dist_fp32;    /* the FP32 distance the GPU found for (i, j) */
+    double seed_dist;    /* best-so-far distance entering the join (from the seed) */
+    /* per-stage counters of the GPU cascade */
+    int64_t pairs_total;     /* candidate pairs in the triangle: K(K+1)/2, K = n-2m+1 */
+    int64_t pairs_swept;     /* pairs evaluated by the diagonal bound sweep */
+    int64_t pass_ball;       /* pairs surviving the two-sided ball bound */
+    int64_t pass_kim;        /* ... and the LB_Kim end-point bound */
+    int64_t pass_keogh;      /* ... and the two-sided LB_Keogh bound (these reach DTW) */
+    int64_t dtw_abandoned;   /* DTW evaluations stopped early by the best-so-far */
+    int64_t dtw_cells;       /* DTW cells evaluated (FP32) */
+    int64_t candidates;      /* FP32 results re-verified in FP64 */
+    /* wall-clock seconds (host steady clock, device synchronised) */
+    double t_total;      /* entry to result, including host<->device copies */
+    double t_prep;       /* statistics, envelopes, bound tables */
+    double t_seed;       /* seed scan */
+    double t_join;       /* fused sweep + cascade + DTW */
+    double t_verify;     /* FP64 re-verification */
+    double join_kernel_ms; /* GPU time of the fused join kernel (HIP events) */
+    int32_t items;       /* sweep work items */
+    int32_t reserved[7];
+} dtwj_result;
+
+/* Fills *opt with defaults.  No reference counterpart (the reference hard-codes
+ * its tuning in macros, GPU_version1/alldef/GPU_parameters.h:8-18). */
+void dtwj_default_options(dtwj_options *opt);
+
+/* Band radius from a ratio, the reference rule w = MAX(round(m*r), 1)
+ * (GPU_version1/src/AAAA.cu:34; CPU_version1/AAAA.cpp:36). */
+int32_t dtwj_band_from_ratio(int32_t m, double r);
+
+/* Top-1 join on a host series.  Replaces the in-process entry point
+ * new_dtw_motifGUI_malloc(a, m, w, mp_ed, mp_ed_index, file)
+ * (GPU_version1/src/new_dtw_motifGUI_malloc.cu:30-31; CPU twin
+ * CPU_version1/new_dtw_motifGUI.cpp:21-22) minus the ED seed arguments, which the
+ * library computes itself on the GPU.  opt may be NULL. */
+int dtwj_top1(const double *series, int64_t n, int32_t m, int32_t w, const dtwj_options *opt,
+              dtwj_result *out);
+
+/* Same, seeded with the reference's ED matrix profile (mp_ed[k] = ED distance of window
+ * k to its nearest non-trivial neighbour, mp_idx_1based[k] its 1-based index), exactly the
+ * extra arguments of new_dtw_motifGUI_malloc (GPU_version1/src/new_dtw_motifGUI_malloc.cu:30-31).
+ * The 100 smallest entries (reference k = 100, GPU_version1/src/new_dtw_motifGUI_malloc.cu:82)
+ * are verified by DTW and seed the best-so-far; the answer does not depend on the seed. */
+int dtwj_top1_seeded(const double *series, int64_t n, int32_t m, int32_t w, const double *mp_ed,
+                     const int32_t *mp_idx_1based, int64_t n_mp, const dtwj_options *opt,
+                     dtwj_result *out);
+
+/* Reusable device context: keeps device buffers between calls (for repeated joins). */
+typedef struct dtwj_context dtwj_context;
+int dtwj_context_create(int32_t device, dtwj_context **ctx);
+void dtwj_context_destroy(dtwj_context *ctx);
+/* Top-1 join on a series already resident in device memory (d_series: n doubles on the
+ * context's device).  Used to time the join with inputs resident in HBM. */
+int dtwj_top1_device(dtwj_context *ctx, const double *d_series, int64_t n, int32_t m, int32_t w,
+                     const dtwj_options *opt, dtwj_result *out);
+/* Copies n doubles to a device buffer owned by the context (valid until the next call of
+ * this function or the context's destruction) and returns its device address. */
+int dtwj_context_upload(dtwj_context *ctx, const double *series, int64_t n, const double **d_series);
+
+/* Whole pipeline of the reference CLI for one (m, w): DTWMotifDiscoveryGUI(TS, m, w, file)
+ * (GPU_version1/src/DTWMotifDiscoveryGUI.cu:12-28): runs the join, prints the reference's
+ * result lines to stdout and appends one "%lf\n" seconds line to time_file
+ * (GPU_version1/src/new_dtw_motifGUI_malloc.cu:744-753).  time_file may be NULL. */
+int dtwj_motif_discovery(const double *series, int64_t n, int32_t m, int32_t w, const char *time_file,
+                         const dtwj_options *opt, dtwj_result *out);
+
+/* Text loader: whitespace-separated decimal numbers -> malloc'ed double array (free with
+ * dtwj_free).  Replaces loadfile (GPU_version1/src/loadfile.cu:14-33), which reads FP32 on the
+ * GPU build and exits with status 0 on an unreadable file; this returns DTWJ_EIO instead. */
+int dtwj_load_text(const char *path, double **out, int64_t *n);
+void dtwj_free(void *p);
+
+/* Message of the last error on this thread ("" if none). */
+const char *dtwj_last_error(void);
+
+/* Number of visible HIP devices (0 when none; never fails). */
+int32_t dtwj_device_count(void);
+
+/* ---- building blocks exposed for tests and tools (host pointers in and out) -------- */
+
+/* Sliding mean and population standard deviation of every length-m window, computed on
+ * the GPU from compensated (double-double) FP64 prefix sums.  Reference: mvmean
+ * (CPU_version1/allmatrixop/movmean.cpp:9-33).  mu, sigma: n-m+1 doubles each. */
+int dtwj_sliding_stats(const double *series, int64_t n, int32_t m, double *mu, double *sigma);
+
+/* Lower/upper envelope of radius w clipped at the series ends, computed on the GPU.
+ * Reference: lower_upper_lemire (CPU_version1/allmatrixop/lower_upper_lemire.cpp:7-43). */
+int dtwj_envelope(const double *series, int64_t n, int32_t w, double *lower, double *upper);
+
+/* Banded DTW (squared cost D[m-1][m-1], not its square root) of z-normalised windows for
+ * an explicit pair list, FP32 cells on the GPU (prec = 32) or FP64 cells (prec = 64).
+ * Reference kernels: DTW_stairs / DTW_stairs_for_block (GPU_version1/src/DTW_stairs.cu:8-599). */
+int dtwj_dtw_pairs(const double *series, int64_t n, int32_t m, int32_t w, const int64_t *pi,
+                   const int64_t *pj, int64_t npairs, int32_t prec, double *cost);
+
+/* ---- synthetic workloads ------------------------------------------------------------ */
+typedef struct dtwj_gen_spec {
+    int64_t n;             /* series length */
+    uint64_t seed;         /* RNG seed */
+    double step_std;       /* random-walk step standard deviation */
+    double step_std_alt;   /* alternate step std for regime shifts (used when regime_len > 0) */
+    int64_t regime_len;    /* samples per regime (0 = no regime shifts) */
+    int32_t n_plants;      /* planted motif pairs */
+    int32_t plant_len;     /* planted window length */
+    double warp;           /* time-warp ratio of the copy (0.05 = +-5 %) */
+    double noise_std;      /* std of the noise added to the copy */
+    const int64_t *plant_src; /* optional explicit source offsets (n_plants), NULL = random */
+    const int64_t *plant_dst; /* optional explicit destination offsets */
+} dtwj_gen_spec;
+
+void dtwj_gen_spec_default(dtwj_gen_spec *spec);
+/* Writes spec->n doubles to out; src_out/dst_out (may be NULL) receive the plant offsets. */
+int dtwj_generate(const dtwj_gen_spec *spec, double *out, int64_t *src_out, int64_t *dst_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DTWJ_H */

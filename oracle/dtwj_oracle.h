@@ -1,0 +1,79 @@
+/*
+ * CPU oracle for the top-1 banded-DTW subsequence join -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this
+ * library, and only as the checker (or the timed CPU baseline).  The product path never
+ * calls it.
+ *
+ * Two independent oracles:
+ *  (1) orc_join / orc_motif_discovery: a plain-C restatement of the reference's FP64 CPU
+ *      join (CPU_version1/new_dtw_motifGUI.cpp:21-586 and the files it calls), in the same
+ *      floating-point operation order, so the seed, the first two pruning stages and the
+ *      answer agree bit for bit with the compiled reference (checked against golden vectors
+ *      from oracle/_ref).  Left out, because they only prune and therefore change neither
+ *      the best-so-far trajectory nor the result: the alpha-tuned local masks
+ *      (CPU_version1/new_dtw_motifGUI.cpp:411-498) and LB_Petitjean
+ *      (CPU_version1/new_dtw_motifGUI.cpp:517-520).
+ *  (2) orc_bruteforce_top1: textbook banded DTW over every admissible pair (tiny n), with
+ *      its own two-pass window statistics; shares no code with (1).
+ */
+#ifndef DTWJ_ORACLE_H
+#define DTWJ_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_join_result {
+    int64_t first_min, sec_min; /* as the reference reports them (1-based when found by the
+                                   sweep, 0-based seed fallback otherwise) */
+    int32_t found_in_sweep;     /* 1 if the sweep updated the answer */
+    int64_t i, j;               /* 0-based winning pair (i < j) */
+    double bsf;                 /* final distance */
+    double seed_bsf;            /* best_so_far after the seed loop ("input of bsf") */
+    double mp_seed;             /* mp_ed[min_index[0]], the seed before DTW refinement */
+    double fast_cnt;            /* pairs pruned by diag_fast */
+    double global_cnt;          /* pairs pruned by diag_mask_global / LB_KIM */
+    int64_t all_cnt;            /* pairs visited */
+    double kk_cnt;              /* pairs reaching LB_KK */
+    double dtw_cnt;             /* pairs reaching MON_dtw (no local masks / LB_Petitjean here) */
+    double fast_stage_rounds;   /* diagonals processed */
+    double global_rounds;       /* diagonals with diag_mask_global (else LB_KIM) */
+} orc_join_result;
+
+/* CPU_version1/allmatrixop/movmean.cpp:9-33 */
+void orc_mvmean(const double *a, int64_t n, int32_t m, double *mu, double *sig);
+/* CPU_version1/allmatrixop/lower_upper_lemire.cpp:7-43 */
+void orc_lemire(const double *a, int64_t n, int32_t w, double *lower, double *upper);
+/* CPU_version1/allmatrixop/moving_mean.cpp:12-50 (compensated) */
+void orc_moving_mean(const double *a, int64_t n, int32_t w, double *res);
+/* CPU_version1/mpx_v2.cpp:13-150: ED matrix profile (mp: distances, mpi: 1-based index) */
+int orc_mpx(const double *a, int64_t n, int32_t minlag, int32_t m, double *mp, int32_t *mpi);
+/* CPU_version1/allmatrixop/min_v.cpp:40-71 (out: k+10 ints) */
+void orc_min_v_k(const double *v, int64_t len, int32_t k, int32_t *out);
+/* CPU_version1/MON_DTW.cpp:20-108 (EAPrunedDTW; squared cost or 1e20) */
+double orc_mon_dtw(const double *lines, const double *cols, const double *cb, int32_t l, int32_t w,
+                   double bsf);
+/* CPU_version1/new_dtw_motifGUI.cpp:21-586 (see header comment) */
+int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_ed, const int32_t *mpi,
+             orc_join_result *out);
+/* CPU_version1/DTWMotifDiscoveryGUI.cpp:12-27: orc_mpx then orc_join */
+int orc_motif_discovery(const double *a, int64_t n, int32_t m, int32_t w, orc_join_result *out);
+
+/* ---- independent oracle ---- */
+/* textbook Sakoe-Chiba DTW: squared cost D[m-1][m-1] */
+double orc_dtw_sq(const double *x, const double *y, int32_t m, int32_t w);
+/* z-normalised window with its own two-pass mean / population std */
+int orc_znorm(const double *a, int64_t n, int64_t i, int32_t m, double *z);
+/* every pair j - i >= m; ties -> smallest i, then smallest j; *d = sqrt(cost) */
+int orc_bruteforce_top1(const double *a, int64_t n, int32_t m, int32_t w, int64_t *bi, int64_t *bj,
+                        double *d);
+/* textbook two-sided-envelope LB_Keogh of z_j against the radius-w envelope of z_i
+   clipped at the window ends (squared) */
+double orc_lb_keogh_sq(const double *zi, const double *zj, int32_t m, int32_t w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

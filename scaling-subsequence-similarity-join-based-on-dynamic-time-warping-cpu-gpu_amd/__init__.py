@@ -1,0 +1,314 @@
+"""Host-side mirror of the reference's top-1 DTW join interface, over the gfx950 C ABI.
+
+The reference exposes the join as two C++ functions and a command line
+(GPU_version1/src/AAAA.cu:17-36, GPU_version1/src/DTWMotifDiscoveryGUI.cu:12-28,
+GPU_version1/src/new_dtw_motifGUI_malloc.cu:30-31):
+
+    DTWMotifDiscoveryGUI(TS, subseqlen, maxwarp, output_time_file_path)
+    new_dtw_motifGUI_malloc(a, subseqLen, maxwarp, mp_ed, mp_ed_index, file)
+    GPU_dtw <data> <time_out> <m> <r>
+
+This module provides the same names with the same argument meaning, backed by
+lib/libdtwj.so (include/dtwj.h).  Differences: results are returned (and also printed
+like the reference), indices are 0-based in the returned object, errors raise DTWJError
+instead of exiting.  There is no CPU fallback: if the HIP library is missing or no GPU is
+visible, every compute call raises.
+"""
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libdtwj.so")
+CLI_PATH = os.path.join(_HERE, "bin", "gpu_dtw")
+
+DTWJ_OK, DTWJ_EINVAL, DTWJ_ENOMEM, DTWJ_EDEVICE, DTWJ_EIO, DTWJ_EOVERFLOW = range(6)
+
+
+class DTWJError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__("dtwj error %d: %s" % (code, message))
+        self.code = code
+        self.message = message
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("seed_items", ctypes.c_int32), ("rows_per_item", ctypes.c_int32),
+                ("waves_per_cu", ctypes.c_int32), ("verbose", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("i", ctypes.c_int64), ("j", ctypes.c_int64), ("dist", ctypes.c_double),
+                ("dist_fp32", ctypes.c_double), ("seed_dist", ctypes.c_double),
+                ("pairs_total", ctypes.c_int64), ("pairs_swept", ctypes.c_int64), ("pass_ball", ctypes.c_int64),
+                ("pass_kim", ctypes.c_int64), ("pass_keogh", ctypes.c_int64), ("dtw_abandoned", ctypes.c_int64),
+                ("dtw_cells", ctypes.c_int64), ("candidates", ctypes.c_int64),
+                ("t_total", ctypes.c_double), ("t_prep", ctypes.c_double), ("t_seed", ctypes.c_double),
+                ("t_join", ctypes.c_double), ("t_verify", ctypes.c_double), ("join_kernel_ms", ctypes.c_double),
+                ("items", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+
+
+class GenSpec(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("seed", ctypes.c_uint64), ("step_std", ctypes.c_double),
+                ("step_std_alt", ctypes.c_double), ("regime_len", ctypes.c_int64), ("n_plants", ctypes.c_int32),
+                ("plant_len", ctypes.c_int32), ("warp", ctypes.c_double), ("noise_std", ctypes.c_double),
+                ("plant_src", ctypes.POINTER(ctypes.c_int64)), ("plant_dst", ctypes.POINTER(ctypes.c_int64))]
+
+
+# Every symbol include/dtwj.h declares (checked by the ABI test).
+EXPORTED = [
+    "dtwj_default_options", "dtwj_band_from_ratio", "dtwj_top1", "dtwj_top1_seeded", "dtwj_context_create",
+    "dtwj_context_destroy", "dtwj_top1_device", "dtwj_context_upload", "dtwj_motif_discovery", "dtwj_load_text",
+    "dtwj_free", "dtwj_last_error", "dtwj_device_count", "dtwj_sliding_stats", "dtwj_envelope", "dtwj_dtw_pairs",
+    "dtwj_gen_spec_default", "dtwj_generate",
+]
+
+_lib = None
+
+
+def lib():
+    """Loads lib/libdtwj.so (built by build.py); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DTWJError(DTWJ_EDEVICE, "HIP library not built: %s (run build.py)" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    d_p, i64_p, i32_p = P(ctypes.c_double), P(ctypes.c_int64), P(ctypes.c_int32)
+    L.dtwj_last_error.restype = ctypes.c_char_p
+    L.dtwj_band_from_ratio.argtypes = [ctypes.c_int32, ctypes.c_double]
+    L.dtwj_band_from_ratio.restype = ctypes.c_int32
+    L.dtwj_device_count.restype = ctypes.c_int32
+    L.dtwj_default_options.argtypes = [P(Options)]
+    L.dtwj_top1.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, P(Options), P(Result)]
+    L.dtwj_top1_seeded.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, d_p, i32_p, ctypes.c_int64,
+                                   P(Options), P(Result)]
+    L.dtwj_motif_discovery.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p,
+                                       P(Options), P(Result)]
+    L.dtwj_context_create.argtypes = [ctypes.c_int32, P(ctypes.c_void_p)]
+    L.dtwj_context_destroy.argtypes = [ctypes.c_void_p]
+    L.dtwj_context_upload.argtypes = [ctypes.c_void_p, d_p, ctypes.c_int64, P(ctypes.c_void_p)]
+    L.dtwj_top1_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                   P(Options), P(Result)]
+    L.dtwj_load_text.argtypes = [ctypes.c_char_p, P(d_p), i64_p]
+    L.dtwj_free.argtypes = [ctypes.c_void_p]
+    L.dtwj_sliding_stats.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, d_p, d_p]
+    L.dtwj_envelope.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, d_p, d_p]
+    L.dtwj_dtw_pairs.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, i64_p, i64_p, ctypes.c_int64,
+                                 ctypes.c_int32, d_p]
+    L.dtwj_gen_spec_default.argtypes = [P(GenSpec)]
+    L.dtwj_generate.argtypes = [P(GenSpec), d_p, i64_p, i64_p]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != DTWJ_OK:
+        raise DTWJError(rc, lib().dtwj_last_error().decode(errors="replace"))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _as_series(x):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    if a.ndim != 1:
+        raise ValueError("series must be one-dimensional")
+    return a
+
+
+@dataclass
+class Top1:
+    """Result of one join; i, j are 0-based (the reference prints i+1, j+1)."""
+    i: int
+    j: int
+    dist: float
+    dist_fp32: float
+    seed_dist: float
+    pairs_total: int
+    pairs_swept: int
+    pass_ball: int
+    pass_kim: int
+    pass_keogh: int
+    dtw_abandoned: int
+    dtw_cells: int
+    candidates: int
+    t_total: float
+    t_prep: float
+    t_seed: float
+    t_join: float
+    t_verify: float
+    join_kernel_ms: float
+    items: int
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def first_min(self):
+        """1-based index of the first window, as the reference prints it."""
+        return self.i + 1
+
+    @property
+    def sec_min(self):
+        return self.j + 1
+
+
+def _result(r):
+    return Top1(**{k: getattr(r, k) for k, _ in Result._fields_ if k != "reserved"})
+
+
+def make_options(device=0, seed_items=0, rows_per_item=0, waves_per_cu=0, verbose=False):
+    o = Options()
+    lib().dtwj_default_options(ctypes.byref(o))
+    o.device, o.seed_items, o.rows_per_item, o.waves_per_cu, o.verbose = (
+        device, seed_items, rows_per_item, waves_per_cu, int(bool(verbose)))
+    return o
+
+
+def device_count():
+    return int(lib().dtwj_device_count())
+
+
+def band_from_ratio(m, r):
+    """w = MAX(round(m*r), 1) as the reference CLI computes it (GPU_version1/src/AAAA.cu:34)."""
+    return int(lib().dtwj_band_from_ratio(int(m), float(r)))
+
+
+def top1(series, m, w, **opts):
+    """Exact top-1 pair (smallest banded-DTW distance, j - i >= m) on the GPU."""
+    a = _as_series(series)
+    r = Result()
+    _check(lib().dtwj_top1(_dptr(a), a.size, int(m), int(w), ctypes.byref(make_options(**opts)), ctypes.byref(r)))
+    return _result(r)
+
+
+def top1_seeded(series, m, w, mp_ed, mp_idx_1based, **opts):
+    a = _as_series(series)
+    mp = np.ascontiguousarray(np.asarray(mp_ed, dtype=np.float64))
+    mi = np.ascontiguousarray(np.asarray(mp_idx_1based, dtype=np.int32))
+    r = Result()
+    _check(lib().dtwj_top1_seeded(_dptr(a), a.size, int(m), int(w), _dptr(mp),
+                                  mi.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), mp.size,
+                                  ctypes.byref(make_options(**opts)), ctypes.byref(r)))
+    return _result(r)
+
+
+def DTWMotifDiscoveryGUI(TS, subseqlen, maxwarp, output_time_file_path, **opts):
+    """Mirror of DTWMotifDiscoveryGUI (GPU_version1/src/DTWMotifDiscoveryGUI.cu:12-28):
+    runs the join for one (m, w), prints the reference's result lines and appends the
+    seconds to output_time_file_path.  Returns the Top1 result."""
+    a = _as_series(TS)
+    r = Result()
+    path = None if output_time_file_path is None else os.fsencode(output_time_file_path)
+    _check(lib().dtwj_motif_discovery(_dptr(a), a.size, int(subseqlen), int(maxwarp), path,
+                                      ctypes.byref(make_options(**opts)), ctypes.byref(r)))
+    return _result(r)
+
+
+def new_dtw_motifGUI_malloc(a, subseqLen, maxwarp, mp_ed, mp_ed_index, file, **opts):
+    """Mirror of new_dtw_motifGUI_malloc (GPU_version1/src/new_dtw_motifGUI_malloc.cu:30-31):
+    the join seeded with the ED matrix profile (mp_ed) and its 1-based neighbour index."""
+    res = top1_seeded(a, subseqLen, maxwarp, mp_ed, mp_ed_index, **opts)
+    print("first min is %d sec min is %d" % (res.first_min, res.sec_min))
+    print("total time is %fs   input of bsf:%5.3f  ,final distance :%f  " % (res.t_total, res.seed_dist, res.dist))
+    if file is not None:
+        with open(file, "a") as f:
+            f.write("%f\n" % res.t_total)
+    return res
+
+
+def loadfile(filename):
+    """Mirror of loadfile (GPU_version1/src/loadfile.cu:14-33); FP64, raises on a bad file."""
+    p = ctypes.POINTER(ctypes.c_double)()
+    n = ctypes.c_int64()
+    _check(lib().dtwj_load_text(os.fsencode(filename), ctypes.byref(p), ctypes.byref(n)))
+    try:
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0)
+    finally:
+        lib().dtwj_free(p)
+
+
+def sliding_stats(series, m):
+    a = _as_series(series)
+    mu = np.empty(a.size - m + 1)
+    sg = np.empty(a.size - m + 1)
+    _check(lib().dtwj_sliding_stats(_dptr(a), a.size, int(m), _dptr(mu), _dptr(sg)))
+    return mu, sg
+
+
+def envelope(series, w):
+    a = _as_series(series)
+    lo = np.empty(a.size)
+    up = np.empty(a.size)
+    _check(lib().dtwj_envelope(_dptr(a), a.size, int(w), _dptr(lo), _dptr(up)))
+    return lo, up
+
+
+def dtw_pairs(series, m, w, pi, pj, prec=32):
+    """Squared banded-DTW cost of z-normalised windows for each (pi[k], pj[k])."""
+    a = _as_series(series)
+    pi = np.ascontiguousarray(np.asarray(pi, dtype=np.int64))
+    pj = np.ascontiguousarray(np.asarray(pj, dtype=np.int64))
+    out = np.empty(pi.size)
+    i64 = ctypes.POINTER(ctypes.c_int64)
+    _check(lib().dtwj_dtw_pairs(_dptr(a), a.size, int(m), int(w), pi.ctypes.data_as(i64), pj.ctypes.data_as(i64),
+                                pi.size, int(prec), _dptr(out)))
+    return out
+
+
+def generate(n, seed, step_std=1.0, step_std_alt=0.0, regime_len=0, n_plants=0, plant_len=128, warp=0.05,
+             noise_std=0.1, plant_src=None, plant_dst=None):
+    """Seeded random walk with planted time-warped motif pairs (see csrc/series_gen.cpp).
+    Returns (series, src_offsets, dst_offsets)."""
+    s = GenSpec()
+    lib().dtwj_gen_spec_default(ctypes.byref(s))
+    s.n, s.seed, s.step_std, s.step_std_alt, s.regime_len = int(n), int(seed), step_std, step_std_alt, int(regime_len)
+    s.n_plants, s.plant_len, s.warp, s.noise_std = int(n_plants), int(plant_len), warp, noise_std
+    keep = []
+    if plant_src is not None:
+        ps = np.ascontiguousarray(np.asarray(plant_src, dtype=np.int64))
+        pd = np.ascontiguousarray(np.asarray(plant_dst, dtype=np.int64))
+        keep += [ps, pd]
+        i64 = ctypes.POINTER(ctypes.c_int64)
+        s.plant_src, s.plant_dst = ps.ctypes.data_as(i64), pd.ctypes.data_as(i64)
+    out = np.empty(int(n))
+    src = np.zeros(max(int(n_plants), 1), dtype=np.int64)
+    dst = np.zeros(max(int(n_plants), 1), dtype=np.int64)
+    i64 = ctypes.POINTER(ctypes.c_int64)
+    _check(lib().dtwj_generate(ctypes.byref(s), _dptr(out), src.ctypes.data_as(i64), dst.ctypes.data_as(i64)))
+    return out, src[:int(n_plants)], dst[:int(n_plants)]
+
+
+class Context:
+    """A device context that keeps buffers between joins (for repeated, device-resident runs)."""
+
+    def __init__(self, device=0):
+        self._c = ctypes.c_void_p()
+        _check(lib().dtwj_context_create(int(device), ctypes.byref(self._c)))
+        self._d = ctypes.c_void_p()
+        self.n = 0
+
+    def upload(self, series):
+        a = _as_series(series)
+        _check(lib().dtwj_context_upload(self._c, _dptr(a), a.size, ctypes.byref(self._d)))
+        self.n = a.size
+
+    def top1(self, m, w, **opts):
+        r = Result()
+        _check(lib().dtwj_top1_device(self._c, self._d, self.n, int(m), int(w), ctypes.byref(make_options(**opts)),
+                                      ctypes.byref(r)))
+        return _result(r)
+
+    def close(self):
+        if self._c:
+            lib().dtwj_context_destroy(self._c)
+            self._c = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,147 @@
+// Shared definitions for the gfx950 join: error plumbing, double-double arithmetic,
+// wave-level helpers and the argument blocks passed from the host orchestration
+// (dtwj_api.cpp) to the kernels (prep.hip, join.hip, dtw.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#define DTWJ_WAVE 64
+
+// ---------------------------------------------------------------- host error plumbing
+namespace dtwj {
+void set_error(const std::string &msg);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+}  // namespace dtwj
+
+#define DTWJ_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) return dtwj::hip_fail(e_, #call, __FILE__, __LINE__);     \
+    } while (0)
+
+// ---------------------------------------------------------------- double-double (FP64x2)
+// An unevaluated sum hi + lo with |lo| <= ulp(hi)/2.  Used for the compensated prefix sums
+// of the sliding statistics (Knuth TwoSum / FMA TwoProd).
+struct dd_t {
+    double hi, lo;
+};
+
+__host__ __device__ inline dd_t dd_two_sum(double a, double b) {
+    double s = a + b;
+    double bb = s - a;
+    double err = (a - (s - bb)) + (b - bb);
+    return {s, err};
+}
+__host__ __device__ inline dd_t dd_quick_two_sum(double a, double b) {
+    double s = a + b;
+    return {s, b - (s - a)};
+}
+__host__ __device__ inline dd_t dd_add(dd_t a, dd_t b) {
+    dd_t s = dd_two_sum(a.hi, b.hi);
+    dd_t t = dd_two_sum(a.lo, b.lo);
+    s.lo += t.hi;
+    s = dd_quick_two_sum(s.hi, s.lo);
+    s.lo += t.lo;
+    return dd_quick_two_sum(s.hi, s.lo);
+}
+__host__ __device__ inline dd_t dd_neg(dd_t a) { return {-a.hi, -a.lo}; }
+__host__ __device__ inline dd_t dd_sub(dd_t a, dd_t b) { return dd_add(a, dd_neg(b)); }
+__host__ __device__ inline dd_t dd_mul_d(dd_t a, double b) {
+    double p = a.hi * b;
+    double e = fma(a.hi, b, -p);
+    e = fma(a.lo, b, e);
+    return dd_quick_two_sum(p, e);
+}
+__host__ __device__ inline dd_t dd_mul(dd_t a, dd_t b) {
+    double p = a.hi * b.hi;
+    double e = fma(a.hi, b.hi, -p);
+    e = fma(a.hi, b.lo, e);
+    e = fma(a.lo, b.hi, e);
+    return dd_quick_two_sum(p, e);
+}
+__host__ __device__ inline dd_t dd_div_d(dd_t a, double b) {
+    double q1 = a.hi / b;
+    dd_t r = dd_sub(a, dd_mul_d(dd_t{q1, 0.0}, b));
+    double q2 = r.hi / b;
+    return dd_quick_two_sum(q1, q2);
+}
+
+// ---------------------------------------------------------------- packed best-so-far
+// 64-bit key = (FP32 bits of the squared DTW cost) << 32 | i.  Non-negative floats order
+// like their bit patterns, so an unsigned 64-bit atomicMin keeps the smallest cost and,
+// among equal costs, the smallest i.
+__host__ __device__ inline unsigned long long pack_bsf(float cost, uint32_t i) {
+    return ((unsigned long long)__builtin_bit_cast(uint32_t, cost) << 32) | (unsigned long long)i;
+}
+__host__ __device__ inline float bsf_cost(unsigned long long key) {
+    return __builtin_bit_cast(float, (uint32_t)(key >> 32));
+}
+
+// ---------------------------------------------------------------- device helpers
+#ifdef __HIP_DEVICE_COMPILE__
+__device__ inline int lane_id() { return (int)__lane_id(); }
+#endif
+
+// Argument blocks -------------------------------------------------------------------------
+// Per-window table, 8 floats (32 bytes) so that a row is one scalar 8-dword load.
+struct WinRow {
+    float dfa, dga;  // MPX update terms of the series
+    float dfv, dgv;  // MPX update terms of the envelope mid-line v = (U+L)/2
+    float isig;      // 1 / sigma
+    float mN;        // m + ||(v_i - mu_i)/sigma_i||^2
+    float R;         // ball radius ||(U_i - L_i)/2|| / sigma_i
+    float sig;       // sigma
+};
+
+// Everything the fused join kernel reads.  Device pointers unless noted.
+struct JoinArgs {
+    // series and windows
+    const double *a;       // n, FP64
+    const double *v;       // n, FP64 envelope mid-line
+    const double *mu;      // nw, FP64 window means of a
+    const double *muv;     // nw, FP64 window means of v
+    const WinRow *win;     // nw
+    const float *ac;       // n, a - c0 (FP32)
+    const float *uc;       // n, U - c0
+    const float *lc;       // n, L - c0
+    const float *mu_c;     // nw, mu - c0
+    const float *blkmax;   // 3 x nblk: max sigma, max sv, max 1/sigma over 256-window blocks
+    int32_t nblk;
+    // problem
+    int64_t n, nw;
+    int32_t m, w;
+    // work decomposition
+    const int64_t *item_start;  // G+1 cumulative item counts per diagonal group
+    int32_t groups;
+    int64_t n_items;
+    int32_t rows_per_item;
+    int32_t *work_counter;
+    // shared best-so-far and results
+    unsigned long long *bsf;      // packed
+    int4 *cands;                  // (i, j, cost bits, 0)
+    int32_t *n_cands;
+    int32_t cand_cap;
+    unsigned long long *counters; // per-stage counters (see dtwj_api.cpp)
+    // error model (host-computed)
+    float amax;        // max |ac|, |uc|, |lc|
+    float eps_z;       // bound on |z32 - z| for z computed from ac/mu_c/isig
+    float rel_dtw;     // relative inflation of the best-so-far cost used as pruning threshold
+};
+
+// seed scan: per-item best ED pair
+struct SeedArgs {
+    const double *a;
+    const double *mu;
+    const WinRow *win;
+    int64_t n, nw;
+    int32_t m;
+    const int64_t *item_start;
+    int32_t groups;
+    int64_t n_items;
+    int32_t rows_per_item;  // rows per join work item (item -> row segment)
+    int32_t rows;           // rows scanned per seed item
+    int32_t n_seed;         // number of seed items (spread over the item space)
+};

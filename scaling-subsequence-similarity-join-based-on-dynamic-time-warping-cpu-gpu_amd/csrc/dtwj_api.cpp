@@ -1,0 +1,669 @@
+// C ABI of the gfx950 top-1 banded-DTW join (include/dtwj.h): argument validation, device
+// workspace, the launch sequence prep -> seed -> fused join -> FP64 re-verification, and the
+// reference-style reporting.  Host code only; kernels live in prep.hip / join.hip / dtw.hip.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dtwj.h"
+#include "internal.hpp"
+
+namespace dtwj {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file,
+                  line);
+    set_error(buf);
+    return e == hipErrorOutOfMemory ? DTWJ_ENOMEM : DTWJ_EDEVICE;
+}
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+inline double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+constexpr int kCandCap = 1 << 20;
+constexpr int kDiagGroup = 256;  // diagonals per work item (join.hip DG)
+constexpr int kNCounters = 8;
+
+int fail(int code, const std::string &msg) {
+    set_error(msg);
+    return code;
+}
+
+// A bump allocator over one device allocation, 256-byte aligned slices.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, off = 0;
+    template <typename T>
+    T *take(size_t count) {
+        size_t bytes = (count * sizeof(T) + 255) & ~(size_t)255;
+        T *p = reinterpret_cast<T *>(base + off);
+        off += bytes;
+        return p;
+    }
+};
+
+size_t bytes_for(int64_t n, int64_t m, int64_t n_items_upper, int64_t groups, int64_t nseed) {
+    const int64_t nw = n - m + 1;
+    const int64_t nb = prep_scan_blocks(n);
+    const int64_t nblk = (nw + 255) / 256;
+    size_t b = 0;
+    auto add = [&](size_t bytes) { b += (bytes + 255) & ~(size_t)255; };
+    add(8 * n);           // a copy
+    add(8 * n * 3);       // U L v
+    add(8 * nw * 3);      // mu sig muv
+    add(8);               // c0
+    add(4 * n * 3);       // ac uc lc
+    add(4 * nw);          // mu_c
+    add(sizeof(WinRow) * nw);
+    add(4 * 3 * nblk);
+    add(4 * 8);
+    add(sizeof(dd_t) * 5 * (n + 1));
+    add(sizeof(dd_t) * 5 * nb);
+    add(8 * 2 * 1024);
+    add(4 * 4);
+    add(8);                          // bsf
+    add(4);                          // work counter
+    add(8 * (groups + 1));           // item_start
+    add(16 * (size_t)kCandCap);      // cands
+    add(4);                          // n_cands
+    add(8 * kNCounters);             // counters
+    add(8 * (nseed + 128));          // seed pairs
+    add(8 * 2 * (size_t)kCandCap);   // verify pair lists
+    add(8 * (size_t)kCandCap);       // verify costs
+    (void)n_items_upper;
+    return b + 4096;
+}
+
+}  // namespace
+
+}  // namespace dtwj
+
+using namespace dtwj;
+
+struct dtwj_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void *mem = nullptr;
+    size_t mem_cap = 0;
+    double *upload = nullptr;
+    int64_t upload_cap = 0;
+    int num_cus = 0;
+};
+
+extern "C" {
+
+void dtwj_default_options(dtwj_options *opt) {
+    if (!opt) return;
+    std::memset(opt, 0, sizeof(*opt));
+    opt->device = 0;
+}
+
+int32_t dtwj_band_from_ratio(int32_t m, double r) {
+    // GPU_version1/src/AAAA.cu:22-34 parses r as float and computes MAX(round(m*w), 1)
+    float prod = (float)m * (float)r;
+    int32_t w = (int32_t)std::lround((double)prod);
+    return w > 1 ? w : 1;
+}
+
+const char *dtwj_last_error(void) { return g_last_error.c_str(); }
+
+int32_t dtwj_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return c;
+}
+
+void dtwj_free(void *p) { std::free(p); }
+
+int dtwj_context_create(int32_t device, dtwj_context **out) {
+    if (!out) return fail(DTWJ_EINVAL, "null context pointer");
+    *out = nullptr;
+    int count = dtwj_device_count();
+    if (count <= 0) return fail(DTWJ_EDEVICE, "no HIP device available");
+    if (device < 0 || device >= count) return fail(DTWJ_EINVAL, "device ordinal out of range");
+    dtwj_context *c = new dtwj_context();
+    c->device = device;
+    DTWJ_HIP(hipSetDevice(device));
+    DTWJ_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    DTWJ_HIP(hipEventCreate(&c->ev0));
+    DTWJ_HIP(hipEventCreate(&c->ev1));
+    hipDeviceProp_t prop;
+    DTWJ_HIP(hipGetDeviceProperties(&prop, device));
+    c->num_cus = prop.multiProcessorCount;
+    *out = c;
+    return DTWJ_OK;
+}
+
+void dtwj_context_destroy(dtwj_context *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->mem) (void)hipFree(c->mem);
+    if (c->upload) (void)hipFree(c->upload);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int dtwj_context_upload(dtwj_context *c, const double *series, int64_t n, const double **d_series) {
+    if (!c || !series || n <= 0 || !d_series) return fail(DTWJ_EINVAL, "bad upload arguments");
+    DTWJ_HIP(hipSetDevice(c->device));
+    if (n > c->upload_cap) {
+        if (c->upload) DTWJ_HIP(hipFree(c->upload));
+        c->upload = nullptr;
+        DTWJ_HIP(hipMalloc(&c->upload, sizeof(double) * n));
+        c->upload_cap = n;
+    }
+    DTWJ_HIP(hipMemcpyAsync(c->upload, series, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    DTWJ_HIP(hipStreamSynchronize(c->stream));
+    *d_series = c->upload;
+    return DTWJ_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int validate(const double *series, int64_t n, int32_t m, int32_t w) {
+    if (!series) return fail(DTWJ_EINVAL, "series is NULL");
+    if (m < 8) return fail(DTWJ_EINVAL, "subsequence length m must be >= 8");
+    if (w < 0) return fail(DTWJ_EINVAL, "band radius w must be >= 0");
+    if (n < 2 * (int64_t)m) return fail(DTWJ_EINVAL, "series too short: need n >= 2m");
+    if (n > ((int64_t)1 << 31) - 1) return fail(DTWJ_EINVAL, "series too long: n must be < 2^31");
+    const int32_t we = std::min(w, m - 1);
+    if (2 * we + 1 > 1024) return fail(DTWJ_EINVAL, "band too wide: 2*min(w, m-1)+1 must be <= 1024");
+    return DTWJ_OK;
+}
+
+struct Seeds {
+    std::vector<int2> pairs;  // explicit seed pairs (mp-seeded entry point)
+    bool gpu_scan = true;
+};
+
+int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w_in, const dtwj_options *opt,
+             const Seeds &seeds, dtwj_result *out, clk::time_point t_entry) {
+    const int32_t w = std::min(w_in, m - 1);
+    const int64_t nw = n - m + 1;
+    const int64_t K = n - 2 * (int64_t)m + 1;  // diagonal offsets m .. n-m
+    const int32_t groups = (int32_t)((K + kDiagGroup - 1) / kDiagGroup);
+    int32_t rows_per_item = (opt && opt->rows_per_item > 0) ? opt->rows_per_item : 2048;
+    rows_per_item = std::max(128, (rows_per_item + 127) / 128 * 128);
+    std::vector<int64_t> item_start(groups + 1, 0);
+    for (int32_t g = 0; g < groups; ++g) {
+        const int64_t rows_g = nw - ((int64_t)m + (int64_t)kDiagGroup * g);
+        item_start[g + 1] = item_start[g] + (rows_g + rows_per_item - 1) / rows_per_item;
+    }
+    const int64_t n_items = item_start[groups];
+    int32_t n_seed = (opt && opt->seed_items > 0) ? opt->seed_items : 2048;
+    if (!seeds.gpu_scan) n_seed = 0;
+    if (n_seed > n_items) n_seed = (int32_t)n_items;
+    const int64_t nseed_pairs = n_seed + (int64_t)seeds.pairs.size() + 1;
+
+    // ---- workspace
+    const size_t need = bytes_for(n, m, n_items, groups, nseed_pairs);
+    if (need > c->mem_cap) {
+        if (c->mem) DTWJ_HIP(hipFree(c->mem));
+        c->mem = nullptr;
+        c->mem_cap = 0;
+        hipError_t e = hipMalloc(&c->mem, need);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(workspace)", __FILE__, __LINE__);
+        c->mem_cap = need;
+    }
+    Arena ar;
+    ar.base = (char *)c->mem;
+    ar.cap = c->mem_cap;
+    Prep P;
+    ar.take<double>(n);  // (slot for a host-input copy; d_a is provided by the caller)
+    P.a = d_a;
+    P.n = n;
+    P.m = m;
+    P.w = w;
+    P.U = ar.take<double>(n);
+    P.L = ar.take<double>(n);
+    P.v = ar.take<double>(n);
+    P.mu = ar.take<double>(nw);
+    P.sig = ar.take<double>(nw);
+    P.muv = ar.take<double>(nw);
+    P.c0 = ar.take<double>(1);
+    P.ac = ar.take<float>(n);
+    P.uc = ar.take<float>(n);
+    P.lc = ar.take<float>(n);
+    P.mu_c = ar.take<float>(nw);
+    P.win = ar.take<WinRow>(nw);
+    P.nblk = (int32_t)((nw + 255) / 256);
+    P.blkmax = ar.take<float>(3 * (size_t)P.nblk);
+    P.gstat = ar.take<float>(8);
+    P.P = ar.take<dd_t>(5 * (size_t)(n + 1));
+    P.bsum = ar.take<dd_t>(5 * (size_t)prep_scan_blocks(n));
+    P.red = ar.take<double>(2 * 1024);
+    P.flags = ar.take<int32_t>(4);
+    unsigned long long *d_bsf = ar.take<unsigned long long>(1);
+    int32_t *d_work = ar.take<int32_t>(1);
+    int64_t *d_item_start = ar.take<int64_t>(groups + 1);
+    int4 *d_cands = ar.take<int4>(kCandCap);
+    int32_t *d_ncands = ar.take<int32_t>(1);
+    unsigned long long *d_counters = ar.take<unsigned long long>(kNCounters);
+    int2 *d_seed_pairs = ar.take<int2>(nseed_pairs + 128);
+    int64_t *d_vi = ar.take<int64_t>(kCandCap);
+    int64_t *d_vj = ar.take<int64_t>(kCandCap);
+    double *d_vcost = ar.take<double>(kCandCap);
+
+    hipStream_t s = c->stream;
+    const auto t0 = clk::now();
+    // ---- preprocessing
+    if (int rc = prep_series(P, s)) return rc;
+    int32_t flags[4];
+    float gstat[8];
+    DTWJ_HIP(hipMemcpyAsync(flags, P.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipMemcpyAsync(gstat, P.gstat, sizeof(gstat), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    if (flags[0]) return fail(DTWJ_EINVAL, "series contains NaN or Inf");
+    if (flags[1]) return fail(DTWJ_EINVAL, "a window of length m is constant (standard deviation 0)");
+    const auto t1 = clk::now();
+
+    // ---- error model: |z32 - z| <= eps_z for every z value the FP32 stages compute
+    JoinArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.a = d_a;
+    A.v = P.v;
+    A.mu = P.mu;
+    A.muv = P.muv;
+    A.win = P.win;
+    A.ac = P.ac;
+    A.uc = P.uc;
+    A.lc = P.lc;
+    A.mu_c = P.mu_c;
+    A.blkmax = P.blkmax;
+    A.nblk = P.nblk;
+    A.n = n;
+    A.nw = nw;
+    A.m = m;
+    A.w = w;
+    A.item_start = d_item_start;
+    A.groups = groups;
+    A.n_items = n_items;
+    A.rows_per_item = rows_per_item;
+    A.work_counter = d_work;
+    A.bsf = d_bsf;
+    A.cands = d_cands;
+    A.n_cands = d_ncands;
+    A.cand_cap = kCandCap;
+    A.counters = d_counters;
+    A.amax = gstat[0];
+    A.eps_z = (float)(std::ldexp(1.0, -22) * (((double)gstat[0] + (double)gstat[1]) * (double)gstat[2] +
+                                             std::sqrt((double)m)));
+    A.rel_dtw = (float)(std::ldexp(1.0, -10) + 4.0 * m * std::ldexp(1.0, -24));
+
+    // ---- reset shared state
+    const unsigned long long init_key = pack_bsf(3.0e38f, 0xffffffffu);
+    DTWJ_HIP(hipMemcpyAsync(d_bsf, &init_key, sizeof(init_key), hipMemcpyHostToDevice, s));
+    DTWJ_HIP(hipMemsetAsync(d_work, 0, sizeof(int32_t), s));
+    DTWJ_HIP(hipMemsetAsync(d_ncands, 0, sizeof(int32_t), s));
+    DTWJ_HIP(hipMemsetAsync(d_counters, 0, sizeof(unsigned long long) * kNCounters, s));
+    DTWJ_HIP(hipMemcpyAsync(d_item_start, item_start.data(), sizeof(int64_t) * (groups + 1), hipMemcpyHostToDevice, s));
+
+    // ---- seed: ED scan over a sample of items, explicit seeds, and the pair (0, m)
+    std::vector<int2> extra = seeds.pairs;
+    extra.push_back(make_int2(0, m));
+    if (n_seed > 0) {
+        SeedArgs S;
+        S.a = d_a;
+        S.mu = P.mu;
+        S.win = P.win;
+        S.n = n;
+        S.nw = nw;
+        S.m = m;
+        S.item_start = d_item_start;
+        S.groups = groups;
+        S.n_items = n_items;
+        S.rows_per_item = rows_per_item;
+        S.rows = std::min(rows_per_item, 512);
+        S.n_seed = n_seed;
+        if (int rc = launch_seed(S, d_seed_pairs, s)) return rc;
+    }
+    DTWJ_HIP(hipMemcpyAsync(d_seed_pairs + n_seed, extra.data(), sizeof(int2) * extra.size(), hipMemcpyHostToDevice,
+                            s));
+    if (int rc = launch_seed_dtw(A, d_seed_pairs, (int32_t)(n_seed + extra.size()), s)) return rc;
+    unsigned long long seed_key = 0;
+    DTWJ_HIP(hipMemcpyAsync(&seed_key, d_bsf, sizeof(seed_key), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    const auto t2 = clk::now();
+
+    // ---- fused join
+    int wpc = (opt && opt->waves_per_cu > 0) ? opt->waves_per_cu : 8;
+    int64_t grid = (int64_t)c->num_cus * wpc;
+    if (grid > n_items) grid = n_items;
+    if (grid < 1) grid = 1;
+    DTWJ_HIP(hipEventRecord(c->ev0, s));
+    if (int rc = launch_join(A, (int32_t)grid, s)) return rc;
+    DTWJ_HIP(hipEventRecord(c->ev1, s));
+    int32_t ncands = 0;
+    unsigned long long key = 0, counters[kNCounters];
+    DTWJ_HIP(hipMemcpyAsync(&ncands, d_ncands, sizeof(ncands), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipMemcpyAsync(&key, d_bsf, sizeof(key), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipMemcpyAsync(counters, d_counters, sizeof(counters), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    float join_ms = 0.f;
+    DTWJ_HIP(hipEventElapsedTime(&join_ms, c->ev0, c->ev1));
+    const auto t3 = clk::now();
+    if (ncands > kCandCap) return fail(DTWJ_EOVERFLOW, "candidate buffer overflow (too many near-ties)");
+    if (ncands <= 0) return fail(DTWJ_EDEVICE, "join produced no candidate");
+
+    // ---- FP64 re-verification of every candidate within the FP32 error of the best
+    std::vector<int4> cands(ncands);
+    DTWJ_HIP(hipMemcpyAsync(cands.data(), d_cands, sizeof(int4) * ncands, hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    const float best32 = bsf_cost(key);
+    auto err = [&](double D) {
+        const double e = A.eps_z;
+        return (double)A.rel_dtw * D + 4.0 * e * std::sqrt(2.0 * m * D) + 8.0 * m * e * e;
+    };
+    const double T2 = (double)best32 + err(best32);
+    const double ub = T2 + err(T2);
+    std::vector<int64_t> vi, vj;
+    std::vector<float> v32;
+    for (const int4 &cd : cands) {
+        const float cost = __builtin_bit_cast(float, cd.z);
+        if ((double)cost <= ub) {
+            vi.push_back(cd.x);
+            vj.push_back(cd.y);
+            v32.push_back(cost);
+        }
+    }
+    // de-duplicate (a pair can be pushed by the seed and by the join)
+    {
+        std::vector<size_t> idx(vi.size());
+        for (size_t k = 0; k < idx.size(); ++k) idx[k] = k;
+        std::sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
+            return vi[x] != vi[y] ? vi[x] < vi[y] : vj[x] < vj[y];
+        });
+        std::vector<int64_t> ui, uj;
+        std::vector<float> u32;
+        for (size_t k = 0; k < idx.size(); ++k) {
+            if (k > 0 && vi[idx[k]] == vi[idx[k - 1]] && vj[idx[k]] == vj[idx[k - 1]]) continue;
+            ui.push_back(vi[idx[k]]);
+            uj.push_back(vj[idx[k]]);
+            u32.push_back(v32[idx[k]]);
+        }
+        vi.swap(ui);
+        vj.swap(uj);
+        v32.swap(u32);
+    }
+    const int64_t nv = (int64_t)vi.size();
+    std::vector<double> vcost(nv);
+    DTWJ_HIP(hipMemcpyAsync(d_vi, vi.data(), sizeof(int64_t) * nv, hipMemcpyHostToDevice, s));
+    DTWJ_HIP(hipMemcpyAsync(d_vj, vj.data(), sizeof(int64_t) * nv, hipMemcpyHostToDevice, s));
+    if (int rc = launch_dtw_pairs(A, P.sig, d_vi, d_vj, nv, 64, d_vcost, s)) return rc;
+    DTWJ_HIP(hipMemcpyAsync(vcost.data(), d_vcost, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    int64_t best = -1;
+    for (int64_t k = 0; k < nv; ++k) {
+        if (best < 0 || vcost[k] < vcost[best] || (vcost[k] == vcost[best] && (vi[k] < vi[best] ||
+                                                                                (vi[k] == vi[best] && vj[k] < vj[best]))))
+            best = k;
+    }
+    const auto t4 = clk::now();
+
+    std::memset(out, 0, sizeof(*out));
+    out->i = vi[best];
+    out->j = vj[best];
+    out->dist = std::sqrt(vcost[best]);
+    out->dist_fp32 = std::sqrt((double)v32[best]);
+    out->seed_dist = std::sqrt((double)bsf_cost(seed_key));
+    out->pairs_total = K * (K + 1) / 2;
+    out->pairs_swept = (int64_t)counters[0];
+    out->pass_ball = (int64_t)counters[1];
+    out->pass_kim = (int64_t)counters[2];
+    out->pass_keogh = (int64_t)counters[3];
+    out->dtw_abandoned = (int64_t)counters[4];
+    out->dtw_cells = (int64_t)counters[5];
+    out->candidates = nv;
+    out->t_prep = secs(t0, t1);
+    out->t_seed = secs(t1, t2);
+    out->t_join = secs(t2, t3);
+    out->t_verify = secs(t3, t4);
+    out->t_total = secs(t_entry, t4);
+    out->join_kernel_ms = join_ms;
+    out->items = (int32_t)n_items;
+    if (opt && opt->verbose) {
+        std::fprintf(stderr,
+                     "[dtwj] n=%lld m=%d w=%d items=%lld swept=%lld ball=%lld kim=%lld keogh=%lld dtw_ab=%lld "
+                     "cells=%lld cands=%lld seed=%.6f best=%.6f join=%.3f ms\n",
+                     (long long)n, m, w, (long long)n_items, (long long)out->pairs_swept, (long long)out->pass_ball,
+                     (long long)out->pass_kim, (long long)out->pass_keogh, (long long)out->dtw_abandoned,
+                     (long long)out->dtw_cells, (long long)nv, out->seed_dist, out->dist, join_ms);
+    }
+    return DTWJ_OK;
+}
+
+int with_context(int32_t device, dtwj_context **c) { return dtwj_context_create(device, c); }
+
+}  // namespace
+
+extern "C" {
+
+int dtwj_top1_device(dtwj_context *c, const double *d_series, int64_t n, int32_t m, int32_t w,
+                     const dtwj_options *opt, dtwj_result *out) {
+    const auto te = clk::now();
+    if (!c || !out) return fail(DTWJ_EINVAL, "null context or result");
+    if (int rc = validate(d_series, n, m, w)) return rc;
+    DTWJ_HIP(hipSetDevice(c->device));
+    Seeds seeds;
+    return run_join(c, d_series, n, m, w, opt, seeds, out, te);
+}
+
+static int top1_host(const double *series, int64_t n, int32_t m, int32_t w, const Seeds &seeds,
+                     const dtwj_options *opt, dtwj_result *out) {
+    const auto te = clk::now();
+    if (!out) return fail(DTWJ_EINVAL, "null result");
+    if (int rc = validate(series, n, m, w)) return rc;
+    for (int64_t t = 0; t < n; ++t)
+        if (!std::isfinite(series[t])) return fail(DTWJ_EINVAL, "series contains NaN or Inf");
+    dtwj_context *c = nullptr;
+    if (int rc = with_context(opt ? opt->device : 0, &c)) return rc;
+    const double *d = nullptr;
+    int rc = dtwj_context_upload(c, series, n, &d);
+    if (rc == DTWJ_OK) rc = run_join(c, d, n, m, w, opt, seeds, out, te);
+    dtwj_context_destroy(c);
+    return rc;
+}
+
+int dtwj_top1(const double *series, int64_t n, int32_t m, int32_t w, const dtwj_options *opt, dtwj_result *out) {
+    Seeds seeds;
+    return top1_host(series, n, m, w, seeds, opt, out);
+}
+
+int dtwj_top1_seeded(const double *series, int64_t n, int32_t m, int32_t w, const double *mp_ed,
+                     const int32_t *mp_idx_1based, int64_t n_mp, const dtwj_options *opt, dtwj_result *out) {
+    if (!mp_ed || !mp_idx_1based) return fail(DTWJ_EINVAL, "mp_ed / mp_idx is NULL");
+    if (n_mp != n - m + 1) return fail(DTWJ_EINVAL, "matrix profile length must be n - m + 1");
+    // the 100 smallest profile entries (reference k = 100) in scan order
+    const int k = 100;
+    std::vector<double> sorted(mp_ed, mp_ed + n_mp);
+    std::vector<double> finite;
+    for (double x : sorted)
+        if (std::isfinite(x)) finite.push_back(x);
+    Seeds seeds;
+    seeds.gpu_scan = false;
+    if (!finite.empty()) {
+        const size_t kk = std::min<size_t>(k, finite.size());
+        std::nth_element(finite.begin(), finite.begin() + (kk - 1), finite.end());
+        const double kth = finite[kk - 1];
+        for (int64_t t = 0; t < n_mp && (int)seeds.pairs.size() < k; ++t) {
+            if (!(mp_ed[t] <= kth)) continue;
+            int64_t a = t, b = (int64_t)mp_idx_1based[t] - 1;
+            if (b < 0 || b >= n_mp) continue;
+            if (a > b) std::swap(a, b);
+            if (b - a < m) continue;
+            seeds.pairs.push_back(make_int2((int)a, (int)b));
+        }
+    }
+    return top1_host(series, n, m, w, seeds, opt, out);
+}
+
+int dtwj_motif_discovery(const double *series, int64_t n, int32_t m, int32_t w, const char *time_file,
+                         const dtwj_options *opt, dtwj_result *out) {
+    dtwj_result local;
+    dtwj_result *r = out ? out : &local;
+    int rc = dtwj_top1(series, n, m, w, opt, r);
+    if (rc != DTWJ_OK) return rc;
+    // reference result lines (CPU_version1/new_dtw_motifGUI.cpp:575-579,
+    // GPU_version1/src/new_dtw_motifGUI_malloc.cu:744-748); indices 1-based there
+    std::printf("dtw cnt = %lld\n", (long long)(r->pass_keogh));
+    std::printf("first min is %lld sec min is %lld\n", (long long)(r->i + 1), (long long)(r->j + 1));
+    std::printf("total time is %fs   input of bsf:%5.3f  ,final distance :%f  \n", r->t_total, r->seed_dist, r->dist);
+    std::printf("pair (0-based) i=%lld j=%lld dist=%.12g\n", (long long)r->i, (long long)r->j, r->dist);
+    std::fflush(stdout);
+    if (time_file) {
+        FILE *op = std::fopen(time_file, "a");
+        if (!op) return fail(DTWJ_EIO, std::string("cannot open time file: ") + time_file);
+        std::fprintf(op, "%lf\n", r->t_total);
+        std::fclose(op);
+    }
+    return DTWJ_OK;
+}
+
+// ---------------------------------------------------------------- building blocks
+static int building_block_prep(const double *series, int64_t n, int32_t m, int32_t w, dtwj_context **cout,
+                               Prep &P, const double **d_a) {
+    if (int rc = with_context(0, cout)) return rc;
+    dtwj_context *c = *cout;
+    if (int rc = dtwj_context_upload(c, series, n, d_a)) return rc;
+    const int64_t nw = n - m + 1;
+    const size_t need = bytes_for(n, m, 1, 1, 1);
+    DTWJ_HIP(hipMalloc(&c->mem, need));
+    c->mem_cap = need;
+    Arena ar;
+    ar.base = (char *)c->mem;
+    ar.cap = need;
+    P.a = *d_a;
+    P.n = n;
+    P.m = m;
+    P.w = w;
+    P.U = ar.take<double>(n);
+    P.L = ar.take<double>(n);
+    P.v = ar.take<double>(n);
+    P.mu = ar.take<double>(nw);
+    P.sig = ar.take<double>(nw);
+    P.muv = ar.take<double>(nw);
+    P.c0 = ar.take<double>(1);
+    P.ac = ar.take<float>(n);
+    P.uc = ar.take<float>(n);
+    P.lc = ar.take<float>(n);
+    P.mu_c = ar.take<float>(nw);
+    P.win = ar.take<WinRow>(nw);
+    P.nblk = (int32_t)((nw + 255) / 256);
+    P.blkmax = ar.take<float>(3 * (size_t)P.nblk);
+    P.gstat = ar.take<float>(8);
+    P.P = ar.take<dd_t>(5 * (size_t)(n + 1));
+    P.bsum = ar.take<dd_t>(5 * (size_t)prep_scan_blocks(n));
+    P.red = ar.take<double>(2 * 1024);
+    P.flags = ar.take<int32_t>(4);
+    if (int rc = prep_series(P, c->stream)) return rc;
+    DTWJ_HIP(hipStreamSynchronize(c->stream));
+    return DTWJ_OK;
+}
+
+int dtwj_sliding_stats(const double *series, int64_t n, int32_t m, double *mu, double *sigma) {
+    if (!series || !mu || !sigma || m < 2 || n < m) return fail(DTWJ_EINVAL, "bad arguments");
+    dtwj_context *c = nullptr;
+    Prep P;
+    const double *d_a = nullptr;
+    int rc = building_block_prep(series, n, m, 0, &c, P, &d_a);
+    if (rc == DTWJ_OK) {
+        const int64_t nw = n - m + 1;
+        hipError_t e1 = hipMemcpy(mu, P.mu, sizeof(double) * nw, hipMemcpyDeviceToHost);
+        hipError_t e2 = hipMemcpy(sigma, P.sig, sizeof(double) * nw, hipMemcpyDeviceToHost);
+        if (e1 != hipSuccess || e2 != hipSuccess) rc = hip_fail(e1 != hipSuccess ? e1 : e2, "hipMemcpy", __FILE__, __LINE__);
+    }
+    if (c) dtwj_context_destroy(c);
+    return rc;
+}
+
+int dtwj_envelope(const double *series, int64_t n, int32_t w, double *lower, double *upper) {
+    if (!series || !lower || !upper || w < 0 || n < 2) return fail(DTWJ_EINVAL, "bad arguments");
+    dtwj_context *c = nullptr;
+    Prep P;
+    const double *d_a = nullptr;
+    const int32_t m = (int32_t)std::min<int64_t>(n, 2);
+    int rc = building_block_prep(series, n, m, w, &c, P, &d_a);
+    if (rc == DTWJ_OK) {
+        hipError_t e1 = hipMemcpy(lower, P.L, sizeof(double) * n, hipMemcpyDeviceToHost);
+        hipError_t e2 = hipMemcpy(upper, P.U, sizeof(double) * n, hipMemcpyDeviceToHost);
+        if (e1 != hipSuccess || e2 != hipSuccess) rc = hip_fail(e1 != hipSuccess ? e1 : e2, "hipMemcpy", __FILE__, __LINE__);
+    }
+    if (c) dtwj_context_destroy(c);
+    return rc;
+}
+
+int dtwj_dtw_pairs(const double *series, int64_t n, int32_t m, int32_t w, const int64_t *pi, const int64_t *pj,
+                   int64_t npairs, int32_t prec, double *cost) {
+    if (int rc = validate(series, n, m, w)) return rc;
+    if (!pi || !pj || !cost || npairs < 0) return fail(DTWJ_EINVAL, "bad pair arguments");
+    if (prec != 32 && prec != 64) return fail(DTWJ_EINVAL, "prec must be 32 or 64");
+    const int64_t nw = n - m + 1;
+    for (int64_t p = 0; p < npairs; ++p)
+        if (pi[p] < 0 || pj[p] < 0 || pi[p] >= nw || pj[p] >= nw) return fail(DTWJ_EINVAL, "pair index out of range");
+    if (npairs == 0) return DTWJ_OK;
+    dtwj_context *c = nullptr;
+    Prep P;
+    const double *d_a = nullptr;
+    const int32_t we = std::min(w, m - 1);
+    int rc = building_block_prep(series, n, m, we, &c, P, &d_a);
+    int64_t *d_pi = nullptr, *d_pj = nullptr;
+    double *d_cost = nullptr;
+    if (rc == DTWJ_OK) {
+        int32_t flags[4];
+        hipMemcpy(flags, P.flags, sizeof(flags), hipMemcpyDeviceToHost);
+        if (flags[1]) rc = fail(DTWJ_EINVAL, "a window of length m is constant (standard deviation 0)");
+    }
+    if (rc == DTWJ_OK) {
+        if (hipMalloc(&d_pi, sizeof(int64_t) * npairs) != hipSuccess || hipMalloc(&d_pj, sizeof(int64_t) * npairs) != hipSuccess ||
+            hipMalloc(&d_cost, sizeof(double) * npairs) != hipSuccess)
+            rc = fail(DTWJ_ENOMEM, "device allocation failed");
+    }
+    if (rc == DTWJ_OK) {
+        hipMemcpy(d_pi, pi, sizeof(int64_t) * npairs, hipMemcpyHostToDevice);
+        hipMemcpy(d_pj, pj, sizeof(int64_t) * npairs, hipMemcpyHostToDevice);
+        JoinArgs A;
+        std::memset(&A, 0, sizeof(A));
+        A.a = d_a;
+        A.mu = P.mu;
+        A.win = P.win;
+        A.ac = P.ac;
+        A.mu_c = P.mu_c;
+        A.n = n;
+        A.nw = nw;
+        A.m = m;
+        A.w = we;
+        rc = launch_dtw_pairs(A, P.sig, d_pi, d_pj, npairs, prec, d_cost, c->stream);
+        if (rc == DTWJ_OK) {
+            hipError_t e = hipStreamSynchronize(c->stream);
+            if (e == hipSuccess) e = hipMemcpy(cost, d_cost, sizeof(double) * npairs, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = hip_fail(e, "dtw pairs", __FILE__, __LINE__);
+        }
+    }
+    if (d_pi) (void)hipFree(d_pi);
+    if (d_pj) (void)hipFree(d_pj);
+    if (d_cost) (void)hipFree(d_cost);
+    if (c) dtwj_context_destroy(c);
+    return rc;
+}
+
+}  // extern "C"

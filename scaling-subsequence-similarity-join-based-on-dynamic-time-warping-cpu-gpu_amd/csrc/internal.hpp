@@ -1,0 +1,50 @@
+// Host-side prototypes shared by dtwj_api.cpp and the .hip translation units.
+#pragma once
+
+#include "common.hpp"
+
+namespace dtwj {
+
+// Outputs and scratch of the per-series preprocessing (prep.hip).  All device pointers.
+struct Prep {
+    // inputs
+    const double *a = nullptr;
+    int64_t n = 0;
+    int32_t m = 0, w = 0;
+    // FP64 outputs
+    double *U = nullptr, *L = nullptr, *v = nullptr;  // n
+    double *mu = nullptr, *sig = nullptr, *muv = nullptr;  // nw
+    double *c0 = nullptr;                             // 1: centring constant (min+max)/2
+    // FP32 outputs
+    float *ac = nullptr, *uc = nullptr, *lc = nullptr;  // n
+    float *mu_c = nullptr;                              // nw
+    WinRow *win = nullptr;                              // nw
+    float *blkmax = nullptr;                            // 3 * nblk
+    int32_t nblk = 0;
+    float *gstat = nullptr;  // 8: max|ac|, max|mu_c|, max isig, max sig, min sig, -, -, -
+    // scratch
+    dd_t *P = nullptr;      // 5 * (n + 1) prefix sums
+    dd_t *bsum = nullptr;   // 5 * nscan_blocks
+    double *red = nullptr;  // 2 * 1024 min/max partials
+    int32_t *flags = nullptr;  // [0] non-finite input, [1] constant window
+};
+
+int64_t prep_scan_blocks(int64_t n);
+// Runs every preprocessing kernel on stream s (no host synchronisation).
+int prep_series(Prep &P, hipStream_t s);
+
+// Seed scan (join.hip): z-normalised ED on a sample of work items; pairs[k] = closest pair
+// of seed item k, or (-1, -1).
+int launch_seed(const SeedArgs &args, int2 *pairs, hipStream_t s);
+// DTW of a pair list (seeds); atomicMin into *bsf and pushes candidates like the join.
+int launch_seed_dtw(const JoinArgs &args, const int2 *pairs, int32_t npairs, hipStream_t s);
+// The fused persistent join kernel.
+int launch_join(const JoinArgs &args, int32_t grid, hipStream_t s);
+size_t join_lds_bytes(int32_t m);
+
+// DTW of an explicit pair list (dtw.hip).  prec 32: FP32 cells from ac/mu_c/isig;
+// prec 64: FP64 cells from a/mu/sig.  cost: npairs doubles (device).
+int launch_dtw_pairs(const JoinArgs &args, const double *sig64, const int64_t *pi, const int64_t *pj,
+                     int64_t npairs, int32_t prec, double *cost, hipStream_t s);
+
+}  // namespace dtwj

@@ -1,0 +1,546 @@
+// The fused top-1 join on gfx950: a persistent diagonal sweep over the pair triangle with
+// an in-wave pruning cascade (reference: the host tile loop of
+// GPU_version1/src/new_dtw_motifGUI_malloc.cu:622-740 driving GLOBAL_DIAG,
+// GPU_version1/lib/GLOBAL_DIAG.cu:29-73, CUB compaction and the per-pair LB_Keogh + DTW kernel,
+// GPU_version1/src/dtw_kernal.cu:174-267).
+//
+// Work item = (diagonal group g, row segment s): 256 consecutive diagonal offsets
+// d = d0 + 4*lane + q (q = 0..3, four diagonals per lane so the per-diagonal column data is
+// reused from registers) over rows_per_item consecutive rows i (pair (i, j = i + d)).
+// Items are claimed from a device counter; the best-so-far is a packed 64-bit word
+// (cost bits | i) read at every LDS chunk and updated with atomicMin, so every wave prunes
+// with the best value found anywhere on the device.  There is no host synchronisation inside
+// the join.
+//
+// Stages per pair (each a valid lower bound of the squared banded DTW cost):
+//  1. two-sided ball bound (all pairs; cf. diag_fast, GPU_version1/lib/diag_fast.cu:13-90):
+//     the envelope box of window i lies in the ball with centre M_i = (U_i+L_i)/2 and radius
+//     R_i = ||U_i-L_i||/2 (normalised), so LB_Keogh(z_j, env_i) >= (||z_j - M_i|| - R_i)^2.
+//     ||z_j - M_i||^2 = m - 2 X/(sigma_i sigma_j) + ||M_i||^2, X = sum (a_j - mu_j)(v_i - muv_i),
+//     updated in O(1) per step with the MPX identity
+//     X(i+1,j+1) = X(i,j) + dfa[j] dgv[i] + dfv[i] dga[j], re-anchored exactly (FP64) at the
+//     start of every item; a per-item rounding margin keeps the FP32 test conservative.
+//  2. LB_Kim over the 3x3 corners at both ends (cf. CPU_version1/lb_kim.cpp:10-73).
+//  3. two-sided LB_Keogh against the raw-series envelope normalised by the other window's
+//     statistics (cf. lb_keogh_stride_with_nomalise, GPU_version1/src/LB_keogh_warp.cu:74-96),
+//     early-abandoned every 16 positions.
+//  4. banded DTW on the wavefront (dtw_device.hpp), early-abandoned against the best-so-far.
+// Survivors move between stages through LDS queues filled with ballot/mbcnt compaction, and
+// each stage runs one pair per lane on full 64-lane batches.
+#include "dtw_device.hpp"
+#include "../../include/dtwj.h"
+#include "internal.hpp"
+
+namespace dtwj {
+
+namespace {
+
+constexpr int KD = 4;                  // diagonals per lane
+constexpr int DG = 64 * KD;            // diagonals per item
+constexpr int RC = 128;                // rows per LDS chunk
+constexpr int JW = RC + DG + 8;        // staged column entries per chunk
+constexpr int JWP = JW / 4;            // entries per phase (phase-split layout)
+constexpr int Q1CAP = 64 + DG + 64;
+constexpr int Q2CAP = 128;
+constexpr int Q3CAP = 128;
+constexpr int NJARR = 6;
+
+enum { C_SWEPT = 0, C_BALL, C_KIM, C_KEOGH, C_DTW_AB, C_CELLS, C_NCOUNTERS };
+
+struct Thr {
+    float T2;      // squared-cost threshold (best-so-far inflated by the FP32 DTW error)
+    float T;       // sqrt(T2), for the ball bound
+    float kim;     // LB_Kim threshold
+    float dtw_ub;  // DTW abandon / candidate threshold
+};
+
+__device__ inline float dtw_err(float D, const JoinArgs &A) {
+    const float e = A.eps_z;
+    return A.rel_dtw * D + 4.0f * e * sqrtf(2.0f * A.m * D) + 8.0f * A.m * e * e;
+}
+
+__device__ inline Thr load_threshold(const JoinArgs &A) {
+    unsigned long long key = __hip_atomic_load(A.bsf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const float cost = bsf_cost(key);
+    Thr t;
+    t.T2 = cost + dtw_err(cost, A);
+    t.T = sqrtf(t.T2) * (1.0f + 0x1p-20f);
+    const float e = A.eps_z;
+    t.kim = t.T2 + 4.0f * e * sqrtf(12.0f * t.T2) + 48.0f * e * e + 0x1p-20f * t.T2;
+    t.dtw_ub = t.T2 + dtw_err(t.T2, A);
+    return t;
+}
+
+__device__ inline float sq(float x) { return x * x; }
+
+__device__ inline void decode_item(const JoinArgs &A, int64_t item, int &g, int64_t &s) {
+    int lo = 0, hi = A.groups - 1;  // find g with item_start[g] <= item < item_start[g+1]
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (A.item_start[mid] <= item)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    g = lo;
+    s = item - A.item_start[g];
+}
+
+__device__ inline unsigned int mbcnt64(unsigned long long msk) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned int)(msk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)msk, 0u));
+}
+
+// Maximum of blkmax[which] over the 256-window blocks covering windows [lo, hi].
+__device__ inline float range_max(const JoinArgs &A, int which, int64_t lo, int64_t hi) {
+    float r = 0.f;
+    for (int64_t b = lo >> 8; b <= (hi >> 8); ++b) r = fmaxf(r, A.blkmax[(int64_t)which * A.nblk + b]);
+    return r;
+}
+
+template <int Q>
+__global__ __launch_bounds__(64) void k_join(JoinArgs A) {
+    extern __shared__ float smem[];
+    float *jarr = smem;                                    // NJARR x [4][JWP]
+    int2 *q1 = reinterpret_cast<int2 *>(jarr + NJARR * JW);
+    int2 *q2 = q1 + Q1CAP;
+    int2 *q3 = q2 + Q2CAP;
+    float *zbuf = reinterpret_cast<float *>(q3 + Q3CAP);  // 2m
+    const int lane = (int)threadIdx.x;
+    const int m = A.m;
+    const int64_t nw = A.nw;
+    const unsigned long long lane_bit = 1ull << lane;
+
+    unsigned long long c_swept = 0, c_ball = 0, c_kim = 0, c_keogh = 0, c_ab = 0, c_cells = 0;
+    int q1n = 0, q2n = 0, q3n = 0;
+    Thr th = load_threshold(A);
+
+    // ---- stage 4: DTW of every queued pair, one pair at a time on the whole wave
+    auto process_q3 = [&]() {
+        for (int e = 0; e < q3n; ++e) {
+            const int2 pr = q3[e];
+            const int i = pr.x, j = pr.y;
+            th = load_threshold(A);
+            const float mi = A.mu_c[i], ii = A.win[i].isig, mj = A.mu_c[j], ij = A.win[j].isig;
+            for (int k = lane; k < m; k += 64) {
+                zbuf[k] = (A.ac[i + k] - mi) * ii;
+                zbuf[m + k] = (A.ac[j + k] - mj) * ij;
+            }
+            __syncthreads();
+            int64_t cells;
+            bool ab;
+            const float cost = dtw_wave<float, Q>(zbuf, zbuf + m, m, A.w, th.dtw_ub, cells, ab);
+            c_cells += (unsigned long long)cells;
+            if (ab) {
+                ++c_ab;
+            } else if (lane == 0) {
+                int slot = atomicAdd(A.n_cands, 1);
+                if (slot < A.cand_cap) A.cands[slot] = make_int4(i, j, __float_as_int(cost), 0);
+                atomicMin(A.bsf, pack_bsf(cost, (uint32_t)i));
+            }
+            __syncthreads();
+        }
+        q3n = 0;
+    };
+
+    // ---- stage 3: two-sided LB_Keogh, one pair per lane
+    auto process_q2 = [&](bool force) {
+        while (q2n >= 64 || (force && q2n > 0)) {
+            __syncthreads();
+            const int batch = q2n < 64 ? q2n : 64;
+            const int base = q2n - batch;
+            const bool act = lane < batch;
+            int i = 0, j = 0;
+            if (act) {
+                int2 pr = q2[base + lane];
+                i = pr.x;
+                j = pr.y;
+            }
+            q2n = base;
+            th = load_threshold(A);
+            bool alive = act;
+            float accA = 0.f, accB = 0.f, thrA = 0.f, thrB = 0.f, rA = 0.f, bA = 0.f, rB = 0.f, bB = 0.f;
+            if (act) {
+                const float si = A.win[i].sig, sj = A.win[j].sig, isi = A.win[i].isig, isj = A.win[j].isig;
+                const float mci = A.mu_c[i], mcj = A.mu_c[j];
+                rA = sj * isi;
+                bA = mcj - mci * rA;
+                rB = si * isj;
+                bB = mci - mcj * rB;
+                const float sm = sqrtf((float)m);
+                const float eA = 0x1p-22f * ((A.amax * (1.f + rA) + fabsf(bA)) * isj + sm);
+                const float eB = 0x1p-22f * ((A.amax * (1.f + rB) + fabsf(bB)) * isi + sm);
+                const float T2 = th.T2;
+                thrA = (T2 + 2.f * eA * sqrtf(m * T2) + m * eA * eA + m * 0x1p-22f * T2) * (sj * sj);
+                thrB = (T2 + 2.f * eB * sqrtf(m * T2) + m * eB * eB + m * 0x1p-22f * T2) * (si * si);
+            }
+            for (int k0 = 0; k0 < m; k0 += 16) {
+                if (__ballot(alive) == 0ull) break;
+                if (alive) {
+                    const int kend = k0 + 16 < m ? k0 + 16 : m;
+                    for (int k = k0; k < kend; ++k) {
+                        const float x = A.ac[j + k];
+                        const float hi = fmaf(A.uc[i + k], rA, bA), lo = fmaf(A.lc[i + k], rA, bA);
+                        const float t = fmaxf(fmaxf(x - hi, lo - x), 0.f);
+                        accA = fmaf(t, t, accA);
+                        const float y = A.ac[i + k];
+                        const float hj = fmaf(A.uc[j + k], rB, bB), lj = fmaf(A.lc[j + k], rB, bB);
+                        const float u = fmaxf(fmaxf(y - hj, lj - y), 0.f);
+                        accB = fmaf(u, u, accB);
+                    }
+                    alive = (accA <= thrA) && (accB <= thrB);
+                }
+            }
+            const unsigned long long msk = __ballot(alive);
+            if (msk) {
+                if (alive) q3[q3n + (int)mbcnt64(msk)] = make_int2(i, j);
+                q3n += __popcll(msk);
+                c_keogh += __popcll(msk);
+                __syncthreads();
+                process_q3();
+            }
+        }
+    };
+
+    // ---- stage 2: LB_Kim on the 3x3 corners of both ends, one pair per lane
+    auto process_q1 = [&](bool force) {
+        while (q1n >= 64 || (force && q1n > 0)) {
+            __syncthreads();
+            const int batch = q1n < 64 ? q1n : 64;
+            const int base = q1n - batch;
+            const bool act = lane < batch;
+            int i = 0, j = 0;
+            if (act) {
+                int2 pr = q1[base + lane];
+                i = pr.x;
+                j = pr.y;
+            }
+            q1n = base;
+            th = load_threshold(A);
+            bool alive = false;
+            if (act) {
+                const float mi = A.mu_c[i], ii = A.win[i].isig, mj = A.mu_c[j], ij = A.win[j].isig;
+                const float *ai = A.ac + i, *aj = A.ac + j;
+                // x: window i (columns), y: window j (rows)
+                const float x0 = (ai[0] - mi) * ii, x1 = (ai[1] - mi) * ii, x2 = (ai[2] - mi) * ii;
+                const float y0 = (aj[0] - mj) * ij, y1 = (aj[1] - mj) * ij, y2 = (aj[2] - mj) * ij;
+                const float xe0 = (ai[m - 1] - mi) * ii, xe1 = (ai[m - 2] - mi) * ii, xe2 = (ai[m - 3] - mi) * ii;
+                const float ye0 = (aj[m - 1] - mj) * ij, ye1 = (aj[m - 2] - mj) * ij, ye2 = (aj[m - 3] - mj) * ij;
+                // Any warping path from (0,0) first leaves {0,1}^2 at (2,0),(2,1),(2,2),(1,2) or
+                // (0,2), entered from an adjacent cell of {0,1}^2.  Same at the far corner.
+                const float c00 = sq(y0 - x0), c10 = sq(y1 - x0), c01 = sq(y0 - x1), c11 = sq(y1 - x1);
+                const float c20 = sq(y2 - x0), c21 = sq(y2 - x1), c22 = sq(y2 - x2), c12 = sq(y1 - x2),
+                            c02 = sq(y0 - x2);
+                float bl = c10 + c20;
+                bl = fminf(bl, fminf(c10, c11) + c21);
+                bl = fminf(bl, c11 + c22);
+                bl = fminf(bl, fminf(c01, c11) + c12);
+                bl = fminf(bl, c01 + c02);
+                const float e00 = sq(ye0 - xe0), e10 = sq(ye1 - xe0), e01 = sq(ye0 - xe1), e11 = sq(ye1 - xe1);
+                const float e20 = sq(ye2 - xe0), e21 = sq(ye2 - xe1), e22 = sq(ye2 - xe2), e12 = sq(ye1 - xe2),
+                            e02 = sq(ye0 - xe2);
+                float br = e10 + e20;
+                br = fminf(br, fminf(e10, e11) + e21);
+                br = fminf(br, e11 + e22);
+                br = fminf(br, fminf(e01, e11) + e12);
+                br = fminf(br, e01 + e02);
+                const float kim = (c00 + bl) + (e00 + br);
+                alive = kim <= th.kim;
+            }
+            const unsigned long long msk = __ballot(alive);
+            if (msk) {
+                if (alive) q2[q2n + (int)mbcnt64(msk)] = make_int2(i, j);
+                q2n += __popcll(msk);
+                c_kim += __popcll(msk);
+            }
+            if (q2n >= 64) process_q2(false);
+        }
+        if (force) process_q2(true);
+    };
+
+    for (;;) {
+        int item_l = 0;
+        if (lane == 0) item_l = atomicAdd(A.work_counter, 1);
+        const int64_t item = (int64_t)__builtin_amdgcn_readfirstlane(item_l);
+        if (item >= A.n_items) break;
+        int g;
+        int64_t sidx;
+        decode_item(A, item, g, sidx);
+        const int64_t d0 = (int64_t)m + (int64_t)DG * g;
+        const int64_t rows_g = nw - d0;  // rows of diagonal d0
+        const int64_t i_beg = sidx * A.rows_per_item;
+        const int64_t i_end = (i_beg + A.rows_per_item < rows_g) ? i_beg + A.rows_per_item : rows_g;
+        const int64_t dq0 = d0 + KD * lane;
+        // last valid row per owned diagonal: i <= nw - 1 - d
+        int64_t lim[KD];
+#pragma unroll
+        for (int q = 0; q < KD; ++q) lim[q] = nw - 1 - (dq0 + q);
+
+        // ---- exact FP64 anchor of X^A, X^B at row i_beg
+        float XA[KD], XB[KD];
+        {
+            double xa[KD], xb[KD];
+#pragma unroll
+            for (int q = 0; q < KD; ++q) xa[q] = xb[q] = 0.0;
+            const double mv0 = A.muv[i_beg], m0 = A.mu[i_beg];
+            const int64_t jb = i_beg + dq0;
+            for (int k = 0; k < m; ++k) {
+                const double cA = A.v[i_beg + k] - mv0;
+                const double cB = A.a[i_beg + k] - m0;
+#pragma unroll
+                for (int q = 0; q < KD; ++q) {
+                    if (i_beg <= lim[q]) {
+                        xa[q] = fma(cA, A.a[jb + q + k], xa[q]);
+                        xb[q] = fma(cB, A.v[jb + q + k], xb[q]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < KD; ++q) {
+                XA[q] = (float)xa[q];
+                XB[q] = (float)xb[q];
+            }
+        }
+        // ---- per-item rounding margins of the FP32 sweep (units of X / sigma)
+        const int64_t jlo = i_beg + d0, jhi = (i_end - 1 + d0 + DG - 1 < nw - 1) ? i_end - 1 + d0 + DG - 1 : nw - 1;
+        const float smaxJ = range_max(A, 0, jlo, jhi), svmaxJ = range_max(A, 1, jlo, jhi),
+                    ismaxJ = range_max(A, 2, jlo, jhi);
+        const float smaxI = range_max(A, 0, i_beg, i_end - 1), svmaxI = range_max(A, 1, i_beg, i_end - 1),
+                    ismaxI = range_max(A, 2, i_beg, i_end - 1);
+        const float steps = (float)(i_end - i_beg + 8);
+        const float EA = 0x1p-20f * steps * (float)m * smaxJ * svmaxI * ismaxJ;
+        const float EB = 0x1p-20f * steps * (float)m * smaxI * svmaxJ * ismaxI;
+
+        for (int64_t c0 = i_beg; c0 < i_end; c0 += RC) {
+            const int64_t c1 = (c0 + RC < i_end) ? c0 + RC : i_end;
+            th = load_threshold(A);
+            const float T = th.T;
+            __syncthreads();
+            // stage column data for j = c0 + d0 + x, x in [0, JW), phase-split by x mod 4
+            for (int x = lane; x < JW; x += 64) {
+                const int64_t j = c0 + d0 + x;
+                float f0 = 0.f, f1 = 0.f, f2 = 0.f, f3 = 0.f, f4 = 0.f, f5 = 0.f;
+                if (j <= nw - 1) {
+                    const WinRow r = A.win[j];
+                    const float tB = T + r.R;
+                    f0 = r.dfa;
+                    f1 = r.dga;
+                    f2 = r.dfv;
+                    f3 = r.dgv;
+                    f4 = r.isig;
+                    f5 = (0.5f * (r.mN - tB * tB) - 0x1p-20f * (r.mN + tB * tB)) * r.sig - EB;
+                }
+                const int o = (x & 3) * JWP + (x >> 2);
+                jarr[0 * JW + o] = f0;
+                jarr[1 * JW + o] = f1;
+                jarr[2 * JW + o] = f2;
+                jarr[3 * JW + o] = f3;
+                jarr[4 * JW + o] = f4;
+                jarr[5 * JW + o] = f5;
+            }
+            __syncthreads();
+            // column window e = 0..6 (e = row-in-group + q)
+            float jv[NJARR][7];
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const int o = (e & 3) * JWP + (e >> 2) + lane;
+#pragma unroll
+                for (int r = 0; r < NJARR; ++r) jv[r][e] = jarr[r * JW + o];
+            }
+            for (int64_t rb = c0; rb < c1; rb += 4) {
+                const int g4 = (int)((rb - c0) >> 2);
+#pragma unroll
+                for (int e = 3; e < 7; ++e) {
+                    const int o = (e & 3) * JWP + g4 + (e >> 2) + lane;
+#pragma unroll
+                    for (int r = 0; r < NJARR; ++r) jv[r][e] = jarr[r * JW + o];
+                }
+                // uniform: every pair of the 4-row group is inside the triangle
+                const bool full = (rb + 3 < c1) && (rb + 3 + d0 + DG - 1 <= nw - 1);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t i = rb + u;
+                    if (i >= c1) break;
+                    const int iu = __builtin_amdgcn_readfirstlane((int)i);
+                    const WinRow ri = A.win[iu];
+                    const float tA = T + ri.R;
+                    const float thA = (0.5f * (ri.mN - tA * tA) - 0x1p-20f * (ri.mN + tA * tA)) * ri.sig - EA;
+#pragma unroll
+                    for (int q = 0; q < KD; ++q) {
+                        const int e = u + q;
+                        const float pA = XA[q] * jv[4][e];
+                        const float pB = XB[q] * ri.isig;
+                        bool ok = (pA >= thA) && (pB >= jv[5][e]);
+                        if (!full) ok = ok && (i <= lim[q]);
+                        const unsigned long long msk = __ballot(ok);
+                        if (msk) {
+                            if (ok) q1[q1n + (int)mbcnt64(msk)] = make_int2((int)i, (int)(i + dq0 + q));
+                            q1n += __popcll(msk);
+                            c_ball += __popcll(msk);
+                        }
+                        if (full)
+                            c_swept += 64;
+                        else
+                            c_swept += __popcll(__ballot(i <= lim[q]));
+                        XA[q] = fmaf(jv[0][e], ri.dgv, XA[q]);
+                        XA[q] = fmaf(ri.dfv, jv[1][e], XA[q]);
+                        XB[q] = fmaf(ri.dfa, jv[3][e], XB[q]);
+                        XB[q] = fmaf(jv[2][e], ri.dga, XB[q]);
+                    }
+                    if (q1n >= 64) process_q1(false);
+                }
+#pragma unroll
+                for (int r = 0; r < NJARR; ++r) {
+                    jv[r][0] = jv[r][4];
+                    jv[r][1] = jv[r][5];
+                    jv[r][2] = jv[r][6];
+                }
+            }
+        }
+        process_q1(true);
+    }
+    if (lane == 0) {
+        atomicAdd(&A.counters[C_SWEPT], c_swept);
+        atomicAdd(&A.counters[C_BALL], c_ball);
+        atomicAdd(&A.counters[C_KIM], c_kim);
+        atomicAdd(&A.counters[C_KEOGH], c_keogh);
+        atomicAdd(&A.counters[C_DTW_AB], c_ab);
+        atomicAdd(&A.counters[C_CELLS], c_cells);
+    }
+}
+
+// ---- seed scan: z-normalised Euclidean distance (an upper bound of banded DTW) over a
+// sample of work items; each wave reports its closest pair.
+__global__ __launch_bounds__(64) void k_seed(SeedArgs S, int2 *pairs) {
+    const int lane = (int)threadIdx.x;
+    const int m = S.m;
+    const int64_t nw = S.nw;
+    const int64_t item = ((int64_t)blockIdx.x * S.n_items) / S.n_seed;
+    int lo = 0, hi = S.groups - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (S.item_start[mid] <= item)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int g = lo;
+    const int64_t d0 = (int64_t)m + (int64_t)DG * g;
+    const int64_t rows_g = nw - d0;
+    const int64_t i_beg = (item - S.item_start[g]) * (int64_t)S.rows_per_item;
+    int64_t i_end = i_beg + S.rows;
+    if (i_end > rows_g) i_end = rows_g;
+    const int64_t dq0 = d0 + KD * lane;
+    float C[KD];
+    {
+        double c[KD] = {0.0, 0.0, 0.0, 0.0};
+        const double m0 = S.mu[i_beg];
+        for (int k = 0; k < m; ++k) {
+            const double cc = S.a[i_beg + k] - m0;
+#pragma unroll
+            for (int q = 0; q < KD; ++q)
+                if (i_beg + dq0 + q <= nw - 1) c[q] = fma(cc, S.a[i_beg + dq0 + q + k], c[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < KD; ++q) C[q] = (float)c[q];
+    }
+    float best = __builtin_huge_valf();
+    int bi = -1, bj = -1;
+    for (int64_t i = i_beg; i < i_end; ++i) {
+        const WinRow ri = S.win[i];
+#pragma unroll
+        for (int q = 0; q < KD; ++q) {
+            const int64_t j = i + dq0 + q;
+            if (j <= nw - 1) {
+                const WinRow rj = S.win[j];
+                const float ed2 = 2.0f * m - 2.0f * C[q] * ri.isig * rj.isig;
+                if (ed2 < best) {
+                    best = ed2;
+                    bi = (int)i;
+                    bj = (int)j;
+                }
+                C[q] = C[q] + rj.dfa * ri.dga + ri.dfa * rj.dga;
+            }
+        }
+    }
+    // wave argmin
+    float v = best;
+    for (int off = 32; off > 0; off >>= 1) {
+        float o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    const unsigned long long msk = __ballot(best == v && bi >= 0);
+    const int src = msk ? (int)__ffsll((long long)msk) - 1 : 0;
+    const int wi = __shfl(bi, src, 64), wj = __shfl(bj, src, 64);
+    if (lane == 0) pairs[blockIdx.x] = msk ? make_int2(wi, wj) : make_int2(-1, -1);
+}
+
+// ---- DTW of a list of candidate pairs (seeds); pushes results like the join does.
+template <int Q>
+__global__ __launch_bounds__(64) void k_seed_dtw(JoinArgs A, const int2 *pairs, int npairs) {
+    extern __shared__ float zs[];
+    const int lane = (int)threadIdx.x;
+    const int m = A.m;
+    const int2 pr = pairs[blockIdx.x];
+    if (blockIdx.x >= (unsigned)npairs || pr.x < 0) return;
+    const int i = pr.x, j = pr.y;
+    const Thr th = load_threshold(A);
+    const float mi = A.mu_c[i], ii = A.win[i].isig, mj = A.mu_c[j], ij = A.win[j].isig;
+    for (int k = lane; k < m; k += 64) {
+        zs[k] = (A.ac[i + k] - mi) * ii;
+        zs[m + k] = (A.ac[j + k] - mj) * ij;
+    }
+    __syncthreads();
+    int64_t cells;
+    bool ab;
+    const float cost = dtw_wave<float, Q>(zs, zs + m, m, A.w, th.dtw_ub, cells, ab);
+    if (!ab && lane == 0) {
+        int slot = atomicAdd(A.n_cands, 1);
+        if (slot < A.cand_cap) A.cands[slot] = make_int4(i, j, __float_as_int(cost), 0);
+        atomicMin(A.bsf, pack_bsf(cost, (uint32_t)i));
+    }
+}
+
+}  // namespace
+
+size_t join_lds_bytes(int32_t m) {
+    return sizeof(float) * NJARR * JW + sizeof(int2) * (Q1CAP + Q2CAP + Q3CAP) + sizeof(float) * 2 * (size_t)m;
+}
+
+int launch_join(const JoinArgs &A, int32_t grid, hipStream_t s) {
+    const int Q = dtw_q_for(A.w);
+    const size_t lds = join_lds_bytes(A.m);
+    switch (Q) {
+        case 1: hipLaunchKernelGGL(k_join<1>, dim3(grid), dim3(64), lds, s, A); break;
+        case 2: hipLaunchKernelGGL(k_join<2>, dim3(grid), dim3(64), lds, s, A); break;
+        case 4: hipLaunchKernelGGL(k_join<4>, dim3(grid), dim3(64), lds, s, A); break;
+        case 8: hipLaunchKernelGGL(k_join<8>, dim3(grid), dim3(64), lds, s, A); break;
+        case 16: hipLaunchKernelGGL(k_join<16>, dim3(grid), dim3(64), lds, s, A); break;
+        default: set_error("band too wide: 2w+1 must be <= 1024"); return DTWJ_EINVAL;
+    }
+    DTWJ_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_seed(const SeedArgs &S, int2 *pairs, hipStream_t s) {
+    hipLaunchKernelGGL(k_seed, dim3(S.n_seed), dim3(64), 0, s, S, pairs);
+    DTWJ_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_seed_dtw(const JoinArgs &A, const int2 *pairs, int32_t npairs, hipStream_t s) {
+    if (npairs <= 0) return 0;
+    const int Q = dtw_q_for(A.w);
+    const size_t lds = sizeof(float) * 2 * (size_t)A.m;
+    switch (Q) {
+        case 1: hipLaunchKernelGGL(k_seed_dtw<1>, dim3(npairs), dim3(64), lds, s, A, pairs, npairs); break;
+        case 2: hipLaunchKernelGGL(k_seed_dtw<2>, dim3(npairs), dim3(64), lds, s, A, pairs, npairs); break;
+        case 4: hipLaunchKernelGGL(k_seed_dtw<4>, dim3(npairs), dim3(64), lds, s, A, pairs, npairs); break;
+        case 8: hipLaunchKernelGGL(k_seed_dtw<8>, dim3(npairs), dim3(64), lds, s, A, pairs, npairs); break;
+        case 16: hipLaunchKernelGGL(k_seed_dtw<16>, dim3(npairs), dim3(64), lds, s, A, pairs, npairs); break;
+        default: set_error("band too wide: 2w+1 must be <= 1024"); return DTWJ_EINVAL;
+    }
+    DTWJ_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace dtwj

@@ -1,0 +1,350 @@
+// Per-series preprocessing on gfx950, all O(n) and HBM-bound:
+//   * centring constant c0 = (min a + max a) / 2 and a non-finite check,
+//   * radius-w lower/upper envelope U, L (reference: lower_upper_lemire,
+//     CPU_version1/allmatrixop/lower_upper_lemire.cpp:7-43, clipped at the series ends),
+//   * compensated FP64 (double-double) prefix sums of a-c0, (a-c0)^2, v-c0, (v-c0)^2, h^2 with
+//     v = (U+L)/2, h = (U-L)/2, giving window means / population standard deviations
+//     (reference: mvmean, CPU_version1/allmatrixop/movmean.cpp:9-33, which uses plain running
+//     sums),
+//   * the per-window table the diagonal sweep reads (WinRow), FP32 centred copies of the
+//     series and envelope, and per-256-window maxima for the sweep's rounding-error margins.
+#include "internal.hpp"
+
+#include <cmath>
+
+namespace dtwj {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanPerThread = 8;
+constexpr int kScanBlock = kScanThreads * kScanPerThread;  // 2048 elements per scan block
+constexpr int kRedBlocks = 1024;
+
+__device__ inline unsigned int fbits(float x) { return __float_as_uint(x); }
+
+__global__ void k_init(int32_t *flags, float *gstat) {
+    int t = threadIdx.x;
+    if (t < 4) flags[t] = 0;
+    if (t < 8) gstat[t] = (t == 4) ? INFINITY : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_minmax_partial(const double *__restrict__ a, int64_t n, double *red,
+                                                        int32_t *flags) {
+    __shared__ double smin[256], smax[256];
+    double lo = INFINITY, hi = -INFINITY;
+    int bad = 0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        double x = a[t];
+        if (!isfinite(x)) bad = 1;
+        lo = fmin(lo, x);
+        hi = fmax(hi, x);
+    }
+    if (bad) atomicOr(&flags[0], 1);
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        red[2 * blockIdx.x] = smin[0];
+        red[2 * blockIdx.x + 1] = smax[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_minmax_final(const double *red, int nparts, double *c0) {
+    __shared__ double smin[256], smax[256];
+    double lo = INFINITY, hi = -INFINITY;
+    for (int p = threadIdx.x; p < nparts; p += 256) {
+        lo = fmin(lo, red[2 * p]);
+        hi = fmax(hi, red[2 * p + 1]);
+    }
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *c0 = 0.5 * smin[0] + 0.5 * smax[0];
+}
+
+// Envelope: U[t] = max a[max(0,t-w) .. min(n-1,t+w)], L likewise; v = (U+L)/2.
+// The values are exact copies of series values (selection only).
+__global__ __launch_bounds__(256) void k_envelope(const double *__restrict__ a, int64_t n, int w, double *U, double *L,
+                                                  double *v) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int64_t lo = t - w < 0 ? 0 : t - w;
+    int64_t hi = t + w > n - 1 ? n - 1 : t + w;
+    double mx = a[lo], mn = a[lo];
+    for (int64_t k = lo + 1; k <= hi; ++k) {
+        double x = a[k];
+        mx = fmax(mx, x);
+        mn = fmin(mn, x);
+    }
+    U[t] = mx;
+    L[t] = mn;
+    v[t] = 0.5 * mx + 0.5 * mn;
+}
+
+// Channel values of the prefix-sum scan.
+__device__ inline void scan_values(const double *__restrict__ a, const double *__restrict__ U,
+                                   const double *__restrict__ L, double c0, int64_t t, double x[5]) {
+    double ad = a[t] - c0;
+    double u = U[t], l = L[t];
+    double vd = (0.5 * u + 0.5 * l) - c0;
+    double h = 0.5 * u - 0.5 * l;
+    x[0] = ad;
+    x[1] = ad * ad;
+    x[2] = vd;
+    x[3] = vd * vd;
+    x[4] = h * h;
+}
+
+// Inclusive block scan of one dd value per thread (Hillis-Steele in LDS).
+__device__ inline dd_t block_scan_dd(dd_t x, dd_t *buf) {
+    int t = threadIdx.x;
+    buf[t] = x;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        dd_t y = buf[t];
+        if (t >= off) y = dd_add(buf[t - off], y);
+        __syncthreads();
+        buf[t] = y;
+        __syncthreads();
+    }
+    dd_t r = buf[t];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_partial(const double *__restrict__ a,
+                                                               const double *__restrict__ U,
+                                                               const double *__restrict__ L, const double *c0p,
+                                                               int64_t n, int64_t nb, dd_t *bsum) {
+    __shared__ dd_t buf[kScanThreads];
+    const double c0 = *c0p;
+    dd_t acc[5];
+    for (int c = 0; c < 5; ++c) acc[c] = {0.0, 0.0};
+    int64_t base = (int64_t)blockIdx.x * kScanBlock + (int64_t)threadIdx.x * kScanPerThread;
+    for (int k = 0; k < kScanPerThread; ++k) {
+        int64_t t = base + k;
+        if (t < n) {
+            double x[5];
+            scan_values(a, U, L, c0, t, x);
+            for (int c = 0; c < 5; ++c) acc[c] = dd_add(acc[c], dd_t{x[c], 0.0});
+        }
+    }
+    for (int c = 0; c < 5; ++c) {
+        dd_t s = block_scan_dd(acc[c], buf);
+        if (threadIdx.x == kScanThreads - 1) bsum[(int64_t)c * nb + blockIdx.x] = s;
+    }
+}
+
+// Exclusive scan of the block sums, one block per channel.
+__global__ __launch_bounds__(kScanThreads) void k_scan_blocks(dd_t *bsum, int64_t nb) {
+    __shared__ dd_t buf[kScanThreads];
+    dd_t *b = bsum + (int64_t)blockIdx.x * nb;
+    int64_t per = (nb + kScanThreads - 1) / kScanThreads;
+    int64_t beg = (int64_t)threadIdx.x * per;
+    int64_t end = beg + per < nb ? beg + per : nb;
+    dd_t s = {0.0, 0.0};
+    for (int64_t k = beg; k < end; ++k) s = dd_add(s, b[k]);
+    dd_t inc = block_scan_dd(s, buf);
+    dd_t run = dd_sub(inc, s);  // exclusive prefix of this thread's chunk
+    for (int64_t k = beg; k < end; ++k) {
+        dd_t x = b[k];
+        b[k] = run;
+        run = dd_add(run, x);
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_final(const double *__restrict__ a, const double *__restrict__ U,
+                                                             const double *__restrict__ L, const double *c0p,
+                                                             int64_t n, int64_t nb, const dd_t *bsum, dd_t *P) {
+    __shared__ dd_t buf[kScanThreads];
+    const double c0 = *c0p;
+    int64_t base = (int64_t)blockIdx.x * kScanBlock + (int64_t)threadIdx.x * kScanPerThread;
+    double xs[kScanPerThread][5];
+    dd_t tot[5];
+    for (int c = 0; c < 5; ++c) tot[c] = {0.0, 0.0};
+    for (int k = 0; k < kScanPerThread; ++k) {
+        int64_t t = base + k;
+        if (t < n) {
+            scan_values(a, U, L, c0, t, xs[k]);
+        } else {
+            for (int c = 0; c < 5; ++c) xs[k][c] = 0.0;
+        }
+        for (int c = 0; c < 5; ++c) tot[c] = dd_add(tot[c], dd_t{xs[k][c], 0.0});
+    }
+    for (int c = 0; c < 5; ++c) {
+        dd_t inc = block_scan_dd(tot[c], buf);
+        dd_t run = dd_add(bsum[(int64_t)c * nb + blockIdx.x], dd_sub(inc, tot[c]));
+        dd_t *Pc = P + (int64_t)c * (n + 1);
+        if (blockIdx.x == 0 && threadIdx.x == 0) Pc[0] = {0.0, 0.0};
+        for (int k = 0; k < kScanPerThread; ++k) {
+            int64_t t = base + k;
+            run = dd_add(run, dd_t{xs[k][c], 0.0});
+            if (t < n) Pc[t + 1] = run;
+        }
+    }
+}
+
+struct WinStats {
+    double mu, sig, muv, sv, N, R;
+    bool degenerate;
+};
+
+__device__ inline WinStats window_stats(const dd_t *P, int64_t n, int m, double c0, int64_t i) {
+    dd_t S[5];
+    for (int c = 0; c < 5; ++c) S[c] = dd_sub(P[(int64_t)c * (n + 1) + i + m], P[(int64_t)c * (n + 1) + i]);
+    WinStats r;
+    dd_t mean_d = dd_div_d(S[0], (double)m);
+    r.mu = c0 + (mean_d.hi + mean_d.lo);
+    // m * var = S1 - S0^2 / m, in double-double
+    dd_t mv = dd_sub(S[1], dd_div_d(dd_mul(S[0], S[0]), (double)m));
+    double var = (mv.hi + mv.lo) / m;
+    double mabs = fabs(mean_d.hi) + 1.0;
+    r.degenerate = !(var > 0.0) || sqrt(var) <= 0x1p-36 * mabs;
+    r.sig = sqrt(var > 0.0 ? var : 0.0);
+    dd_t meanv_d = dd_div_d(S[2], (double)m);
+    r.muv = c0 + (meanv_d.hi + meanv_d.lo);
+    dd_t vv = dd_sub(S[3], dd_div_d(dd_mul(S[2], S[2]), (double)m));  // sum (v - muv)^2
+    double svv = vv.hi + vv.lo;
+    if (svv < 0.0) svv = 0.0;
+    r.sv = sqrt(svv / m);
+    double dmu = (meanv_d.hi + meanv_d.lo) - (mean_d.hi + mean_d.lo);  // muv - mu, from the centred means
+    double var_safe = r.sig > 0.0 ? r.sig * r.sig : 1.0;
+    r.N = (svv + (double)m * dmu * dmu) / var_safe;
+    double h2 = S[4].hi + S[4].lo;
+    r.R = sqrt(h2 > 0.0 ? h2 : 0.0) / (r.sig > 0.0 ? r.sig : 1.0);
+    return r;
+}
+
+__device__ inline void atomic_max_pos(float *p, float x) { atomicMax((unsigned int *)p, fbits(x)); }
+__device__ inline void atomic_min_pos(float *p, float x) { atomicMin((unsigned int *)p, fbits(x)); }
+
+// One thread per window; each block of 256 windows also writes its maxima (sigma, sv, 1/sigma).
+__global__ __launch_bounds__(256) void k_window_stats(const double *__restrict__ a, const double *__restrict__ v,
+                                                      const dd_t *__restrict__ P, const double *c0p, int64_t n,
+                                                      int m, double *mu, double *sig, double *muv, float *mu_c,
+                                                      WinRow *win, float *blkmax, int nblk, float *gstat,
+                                                      int32_t *flags) {
+    __shared__ float r0[256], r1[256], r2[256];
+    const double c0 = *c0p;
+    const int64_t nw = n - m + 1;
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    float fs = 0.f, fsv = 0.f, fis = 0.f;
+    if (i < nw) {
+        WinStats s = window_stats(P, n, m, c0, i);
+        if (s.degenerate) atomicOr(&flags[1], 1);
+        mu[i] = s.mu;
+        sig[i] = s.sig;
+        muv[i] = s.muv;
+        WinRow row;
+        if (i + 1 < nw) {
+            WinStats s1 = window_stats(P, n, m, c0, i + 1);
+            double am = a[i + m], a0 = a[i], vm = v[i + m], v0 = v[i];
+            row.dfa = (float)(0.5 * (am - a0));
+            row.dga = (float)((am - s1.mu) + (a0 - s.mu));
+            row.dfv = (float)(0.5 * (vm - v0));
+            row.dgv = (float)((vm - s1.muv) + (v0 - s.muv));
+        } else {
+            row.dfa = row.dga = row.dfv = row.dgv = 0.f;
+        }
+        double is = s.sig > 0.0 ? 1.0 / s.sig : 0.0;
+        row.isig = (float)is;
+        row.mN = (float)((double)m + s.N);
+        row.R = (float)s.R;
+        row.sig = (float)s.sig;
+        win[i] = row;
+        float mc = (float)(s.mu - c0);
+        mu_c[i] = mc;
+        fs = (float)s.sig;
+        fsv = (float)s.sv;
+        fis = (float)is;
+        atomic_max_pos(&gstat[1], fabsf(mc));
+        atomic_max_pos(&gstat[2], fis);
+        atomic_max_pos(&gstat[3], fs);
+        atomic_min_pos(&gstat[4], fs);
+    }
+    r0[threadIdx.x] = fs;
+    r1[threadIdx.x] = fsv;
+    r2[threadIdx.x] = fis;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            r0[threadIdx.x] = fmaxf(r0[threadIdx.x], r0[threadIdx.x + s]);
+            r1[threadIdx.x] = fmaxf(r1[threadIdx.x], r1[threadIdx.x + s]);
+            r2[threadIdx.x] = fmaxf(r2[threadIdx.x], r2[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && (int)blockIdx.x < nblk) {
+        // round maxima up by one part in 2^20 so the FP32 conversion never underestimates
+        blkmax[0 * nblk + blockIdx.x] = r0[0] * (1.0f + 0x1p-20f);
+        blkmax[1 * nblk + blockIdx.x] = r1[0] * (1.0f + 0x1p-20f);
+        blkmax[2 * nblk + blockIdx.x] = r2[0] * (1.0f + 0x1p-20f);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_positions(const double *__restrict__ a, const double *__restrict__ U,
+                                                   const double *__restrict__ L, const double *c0p, int64_t n,
+                                                   float *ac, float *uc, float *lc, float *gstat) {
+    __shared__ float red[256];
+    const double c0 = *c0p;
+    int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    float mx = 0.f;
+    if (t < n) {
+        float x = (float)(a[t] - c0), u = (float)(U[t] - c0), l = (float)(L[t] - c0);
+        ac[t] = x;
+        uc[t] = u;
+        lc[t] = l;
+        mx = fmaxf(fabsf(x), fmaxf(fabsf(u), fabsf(l)));
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomic_max_pos(&gstat[0], red[0]);
+}
+
+}  // namespace
+
+int64_t prep_scan_blocks(int64_t n) { return (n + kScanBlock - 1) / kScanBlock; }
+
+int prep_series(Prep &P, hipStream_t s) {
+    const int64_t n = P.n;
+    const int m = P.m;
+    const int64_t nw = n - m + 1;
+    const int64_t nb = prep_scan_blocks(n);
+    hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, s, P.flags, P.gstat);
+    int parts = (int)((n + 255) / 256 < kRedBlocks ? (n + 255) / 256 : kRedBlocks);
+    hipLaunchKernelGGL(k_minmax_partial, dim3(parts), dim3(256), 0, s, P.a, n, P.red, P.flags);
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, s, P.red, parts, P.c0);
+    hipLaunchKernelGGL(k_envelope, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P.a, n, P.w, P.U, P.L, P.v);
+    hipLaunchKernelGGL(k_scan_partial, dim3((unsigned)nb), dim3(kScanThreads), 0, s, P.a, P.U, P.L, P.c0, n, nb,
+                       P.bsum);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(5), dim3(kScanThreads), 0, s, P.bsum, nb);
+    hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(kScanThreads), 0, s, P.a, P.U, P.L, P.c0, n, nb, P.bsum,
+                       P.P);
+    hipLaunchKernelGGL(k_window_stats, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, P.a, P.v, P.P, P.c0, n, m,
+                       P.mu, P.sig, P.muv, P.mu_c, P.win, P.blkmax, P.nblk, P.gstat, P.flags);
+    hipLaunchKernelGGL(k_positions, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P.a, P.U, P.L, P.c0, n, P.ac,
+                       P.uc, P.lc, P.gstat);
+    DTWJ_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace dtwj
