@@ -62,6 +62,7 @@ typedef struct dtwj_result {
     int64_t pass_keogh;      /* ... and the two-sided LB_Keogh bound (these reach DTW) */
     int64_t dtw_abandoned;   /* DTW evaluations stopped early by the best-so-far */
     int64_t dtw_cells;       /* DTW cells evaluated (FP32) */
+    int64_t keogh_elems;     /* (pair, position) terms evaluated by LB_Keogh (both directions) */
     int64_t candidates;      /* FP32 results re-verified in FP64 */
     /* wall-clock seconds (host steady clock, device synchronised) */
     double t_total;      /* entry to result, including host<->device copies */

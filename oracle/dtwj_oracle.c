@@ -752,6 +752,7 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
     out->mp_seed = best_so_far;
     double *cb_temp = (double *)xcalloc(subseqLen + maxwarp + 1, sizeof(double));
     int64_t low_index = 0, high_index = 0;
+    int64_t seed_i = min_index[0], seed_j = (int64_t)mp_ed_index[min_index[0]] - 1;
     for (int i = 0; i < k; i++) {
         low_index = min_index[i];
         high_index = (int64_t)mp_ed_index[min_index[i]] - 1;
@@ -759,7 +760,11 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
                         best_so_far * best_so_far, cb_temp)) {
             double dtw_value = sqrt(orc_mon_dtw(SUB(st, low_index), SUB(st, high_index), cb_temp, subseqLen, maxwarp,
                                                 best_so_far * best_so_far));
-            if (dtw_value < best_so_far) best_so_far = dtw_value;
+            if (dtw_value < best_so_far) {
+                best_so_far = dtw_value;
+                seed_i = low_index;
+                seed_j = high_index;
+            }
         }
     }
     double first_min = ORC_MIN(low_index, high_index);
@@ -956,12 +961,14 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
     out->first_min = (int64_t)first_min;
     out->sec_min = (int64_t)sec_min;
     out->found_in_sweep = found;
+    out->seed_i = seed_i < seed_j ? seed_i : seed_j;
+    out->seed_j = seed_i < seed_j ? seed_j : seed_i;
     if (found) {
         out->i = out->first_min - 1;
         out->j = out->sec_min - 1;
     } else {
-        out->i = out->first_min;
-        out->j = out->sec_min;
+        out->i = out->seed_i;
+        out->j = out->seed_j;
     }
     out->bsf = bsf;
     out->fast_cnt = fast_prune_cnt;

@@ -29,7 +29,8 @@ typedef struct orc_join_result {
     int64_t first_min, sec_min; /* as the reference reports them (1-based when found by the
                                    sweep, 0-based seed fallback otherwise) */
     int32_t found_in_sweep;     /* 1 if the sweep updated the answer */
-    int64_t i, j;               /* 0-based winning pair (i < j) */
+    int64_t i, j;               /* 0-based winning pair (i < j): the sweep's pair, or the seed
+                                   pair that set the final distance when the sweep found none */
     double bsf;                 /* final distance */
     double seed_bsf;            /* best_so_far after the seed loop ("input of bsf") */
     double mp_seed;             /* mp_ed[min_index[0]], the seed before DTW refinement */
@@ -40,6 +41,8 @@ typedef struct orc_join_result {
     double dtw_cnt;             /* pairs reaching MON_dtw (no local masks / LB_Petitjean here) */
     double fast_stage_rounds;   /* diagonals processed */
     double global_rounds;       /* diagonals with diag_mask_global (else LB_KIM) */
+    int64_t seed_i, seed_j;     /* 0-based pair that set the seed's best distance (not reported
+                                   by the reference, which prints the last seed visited) */
 } orc_join_result;
 
 /* CPU_version1/allmatrixop/movmean.cpp:9-33 */

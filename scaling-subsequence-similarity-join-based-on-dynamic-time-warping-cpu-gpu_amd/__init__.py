@@ -44,7 +44,7 @@ class Result(ctypes.Structure):
                 ("dist_fp32", ctypes.c_double), ("seed_dist", ctypes.c_double),
                 ("pairs_total", ctypes.c_int64), ("pairs_swept", ctypes.c_int64), ("pass_ball", ctypes.c_int64),
                 ("pass_kim", ctypes.c_int64), ("pass_keogh", ctypes.c_int64), ("dtw_abandoned", ctypes.c_int64),
-                ("dtw_cells", ctypes.c_int64), ("candidates", ctypes.c_int64),
+                ("dtw_cells", ctypes.c_int64), ("keogh_elems", ctypes.c_int64), ("candidates", ctypes.c_int64),
                 ("t_total", ctypes.c_double), ("t_prep", ctypes.c_double), ("t_seed", ctypes.c_double),
                 ("t_join", ctypes.c_double), ("t_verify", ctypes.c_double), ("join_kernel_ms", ctypes.c_double),
                 ("items", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
@@ -136,6 +136,7 @@ class Top1:
     pass_keogh: int
     dtw_abandoned: int
     dtw_cells: int
+    keogh_elems: int
     candidates: int
     t_total: float
     t_prep: float

@@ -29,17 +29,24 @@ struct dd_t {
     double hi, lo;
 };
 
+// Error-free transformations must not be contracted: with -ffp-contract=fast the compiler
+// would fuse a caller's product into TwoSum's addition and silently drop the error term.
+#define DTWJ_NO_CONTRACT _Pragma("clang fp contract(off)")
+
 __host__ __device__ inline dd_t dd_two_sum(double a, double b) {
+    DTWJ_NO_CONTRACT
     double s = a + b;
     double bb = s - a;
     double err = (a - (s - bb)) + (b - bb);
     return {s, err};
 }
 __host__ __device__ inline dd_t dd_quick_two_sum(double a, double b) {
+    DTWJ_NO_CONTRACT
     double s = a + b;
     return {s, b - (s - a)};
 }
 __host__ __device__ inline dd_t dd_add(dd_t a, dd_t b) {
+    DTWJ_NO_CONTRACT
     dd_t s = dd_two_sum(a.hi, b.hi);
     dd_t t = dd_two_sum(a.lo, b.lo);
     s.lo += t.hi;
@@ -50,12 +57,14 @@ __host__ __device__ inline dd_t dd_add(dd_t a, dd_t b) {
 __host__ __device__ inline dd_t dd_neg(dd_t a) { return {-a.hi, -a.lo}; }
 __host__ __device__ inline dd_t dd_sub(dd_t a, dd_t b) { return dd_add(a, dd_neg(b)); }
 __host__ __device__ inline dd_t dd_mul_d(dd_t a, double b) {
+    DTWJ_NO_CONTRACT
     double p = a.hi * b;
     double e = fma(a.hi, b, -p);
     e = fma(a.lo, b, e);
     return dd_quick_two_sum(p, e);
 }
 __host__ __device__ inline dd_t dd_mul(dd_t a, dd_t b) {
+    DTWJ_NO_CONTRACT
     double p = a.hi * b.hi;
     double e = fma(a.hi, b.hi, -p);
     e = fma(a.hi, b.lo, e);
@@ -63,6 +72,7 @@ __host__ __device__ inline dd_t dd_mul(dd_t a, dd_t b) {
     return dd_quick_two_sum(p, e);
 }
 __host__ __device__ inline dd_t dd_div_d(dd_t a, double b) {
+    DTWJ_NO_CONTRACT
     double q1 = a.hi / b;
     dd_t r = dd_sub(a, dd_mul_d(dd_t{q1, 0.0}, b));
     double q2 = r.hi / b;
@@ -86,6 +96,9 @@ __device__ inline int lane_id() { return (int)__lane_id(); }
 #endif
 
 // Argument blocks -------------------------------------------------------------------------
+// Entries of each wave's ball-survivor queue (4 bytes each) in global memory.
+constexpr int kJoinQueueCap = 4096;
+
 // Per-window table, 8 floats (32 bytes) so that a row is one scalar 8-dword load.
 struct WinRow {
     float dfa, dga;  // MPX update terms of the series
@@ -125,6 +138,7 @@ struct JoinArgs {
     int32_t *n_cands;
     int32_t cand_cap;
     unsigned long long *counters; // per-stage counters (see dtwj_api.cpp)
+    uint32_t *queue;              // grid x kJoinQueueCap: per-wave ball-survivor queues
     // error model (host-computed)
     float amax;        // max |ac|, |uc|, |lc|
     float eps_z;       // bound on |z32 - z| for z computed from ac/mu_c/isig

@@ -54,7 +54,7 @@ struct Arena {
     }
 };
 
-size_t bytes_for(int64_t n, int64_t m, int64_t n_items_upper, int64_t groups, int64_t nseed) {
+size_t bytes_for(int64_t n, int64_t m, int64_t grid, int64_t groups, int64_t nseed) {
     const int64_t nw = n - m + 1;
     const int64_t nb = prep_scan_blocks(n);
     const int64_t nblk = (nw + 255) / 256;
@@ -82,7 +82,7 @@ size_t bytes_for(int64_t n, int64_t m, int64_t n_items_upper, int64_t groups, in
     add(8 * (nseed + 128));          // seed pairs
     add(8 * 2 * (size_t)kCandCap);   // verify pair lists
     add(8 * (size_t)kCandCap);       // verify costs
-    (void)n_items_upper;
+    add(4 * (size_t)grid * kJoinQueueCap);  // per-wave survivor queues
     return b + 4096;
 }
 
@@ -210,13 +210,18 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
         item_start[g + 1] = item_start[g] + (rows_g + rows_per_item - 1) / rows_per_item;
     }
     const int64_t n_items = item_start[groups];
-    int32_t n_seed = (opt && opt->seed_items > 0) ? opt->seed_items : 2048;
+    int32_t n_seed = (opt && opt->seed_items > 0) ? opt->seed_items : 4096;
     if (!seeds.gpu_scan) n_seed = 0;
     if (n_seed > n_items) n_seed = (int32_t)n_items;
     const int64_t nseed_pairs = n_seed + (int64_t)seeds.pairs.size() + 1;
+    // persistent grid: one 64-lane wave per workgroup
+    const int wpc = (opt && opt->waves_per_cu > 0) ? opt->waves_per_cu : 8;
+    int64_t grid = (int64_t)c->num_cus * wpc;
+    if (grid > n_items) grid = n_items;
+    if (grid < 1) grid = 1;
 
     // ---- workspace
-    const size_t need = bytes_for(n, m, n_items, groups, nseed_pairs);
+    const size_t need = bytes_for(n, m, grid, groups, nseed_pairs);
     if (need > c->mem_cap) {
         if (c->mem) DTWJ_HIP(hipFree(c->mem));
         c->mem = nullptr;
@@ -263,6 +268,7 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
     int64_t *d_vi = ar.take<int64_t>(kCandCap);
     int64_t *d_vj = ar.take<int64_t>(kCandCap);
     double *d_vcost = ar.take<double>(kCandCap);
+    uint32_t *d_queue = ar.take<uint32_t>((size_t)grid * kJoinQueueCap);
 
     hipStream_t s = c->stream;
     const auto t0 = clk::now();
@@ -305,6 +311,7 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
     A.n_cands = d_ncands;
     A.cand_cap = kCandCap;
     A.counters = d_counters;
+    A.queue = d_queue;
     A.amax = gstat[0];
     A.eps_z = (float)(std::ldexp(1.0, -22) * (((double)gstat[0] + (double)gstat[1]) * (double)gstat[2] +
                                              std::sqrt((double)m)));
@@ -346,10 +353,6 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
     const auto t2 = clk::now();
 
     // ---- fused join
-    int wpc = (opt && opt->waves_per_cu > 0) ? opt->waves_per_cu : 8;
-    int64_t grid = (int64_t)c->num_cus * wpc;
-    if (grid > n_items) grid = n_items;
-    if (grid < 1) grid = 1;
     DTWJ_HIP(hipEventRecord(c->ev0, s));
     if (int rc = launch_join(A, (int32_t)grid, s)) return rc;
     DTWJ_HIP(hipEventRecord(c->ev1, s));
@@ -433,6 +436,7 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
     out->pass_keogh = (int64_t)counters[3];
     out->dtw_abandoned = (int64_t)counters[4];
     out->dtw_cells = (int64_t)counters[5];
+    out->keogh_elems = (int64_t)counters[6];
     out->candidates = nv;
     out->t_prep = secs(t0, t1);
     out->t_seed = secs(t1, t2);

@@ -5,12 +5,11 @@
 // GPU_version1/src/dtw_kernal.cu:174-267).
 //
 // Work item = (diagonal group g, row segment s): 256 consecutive diagonal offsets
-// d = d0 + 4*lane + q (q = 0..3, four diagonals per lane so the per-diagonal column data is
-// reused from registers) over rows_per_item consecutive rows i (pair (i, j = i + d)).
-// Items are claimed from a device counter; the best-so-far is a packed 64-bit word
-// (cost bits | i) read at every LDS chunk and updated with atomicMin, so every wave prunes
-// with the best value found anywhere on the device.  There is no host synchronisation inside
-// the join.
+// d = d0 + 4*lane + q (q = 0..3, four diagonals per lane so the per-column data is reused
+// from registers) over rows_per_item consecutive rows i (pair (i, j = i + d)).  Items are
+// claimed from a device counter; the best-so-far is a packed 64-bit word (cost bits | i) read
+// at every LDS chunk and every cascade batch and updated with atomicMin, so every wave prunes
+// with the best value found anywhere on the device.  No host synchronisation inside the join.
 //
 // Stages per pair (each a valid lower bound of the squared banded DTW cost):
 //  1. two-sided ball bound (all pairs; cf. diag_fast, GPU_version1/lib/diag_fast.cu:13-90):
@@ -20,13 +19,17 @@
 //     updated in O(1) per step with the MPX identity
 //     X(i+1,j+1) = X(i,j) + dfa[j] dgv[i] + dfv[i] dga[j], re-anchored exactly (FP64) at the
 //     start of every item; a per-item rounding margin keeps the FP32 test conservative.
+//     Per pair: 4 FMA + 2 MUL + 2 compares.
 //  2. LB_Kim over the 3x3 corners at both ends (cf. CPU_version1/lb_kim.cpp:10-73).
 //  3. two-sided LB_Keogh against the raw-series envelope normalised by the other window's
 //     statistics (cf. lb_keogh_stride_with_nomalise, GPU_version1/src/LB_keogh_warp.cu:74-96),
 //     early-abandoned every 16 positions.
 //  4. banded DTW on the wavefront (dtw_device.hpp), early-abandoned against the best-so-far.
-// Survivors move between stages through LDS queues filled with ballot/mbcnt compaction, and
-// each stage runs one pair per lane on full 64-lane batches.
+// Ball survivors are appended (ballot/mbcnt compaction, 4-byte packed entries) to a per-wave
+// queue in global memory.  The queue is drained completely at the end of every 64-row chunk
+// (earlier only if it could overflow): the chunk's series / envelope windows and window
+// statistics are then staged in LDS over the sweep's column tables, so stages 2-4 read LDS,
+// not scattered global memory, one pair per lane on full 64-lane batches.
 #include "dtw_device.hpp"
 #include "../../include/dtwj.h"
 #include "internal.hpp"
@@ -37,15 +40,19 @@ namespace {
 
 constexpr int KD = 4;                  // diagonals per lane
 constexpr int DG = 64 * KD;            // diagonals per item
-constexpr int RC = 128;                // rows per LDS chunk
+constexpr int RC = 64;                 // rows per LDS chunk
 constexpr int JW = RC + DG + 8;        // staged column entries per chunk
-constexpr int JWP = JW / 4;            // entries per phase (phase-split layout)
-constexpr int Q1CAP = 64 + DG + 64;
+constexpr int NJARR = 6;               // column arrays: dfa dga dfv dgv 1/sigma thetaB
 constexpr int Q2CAP = 128;
 constexpr int Q3CAP = 128;
-constexpr int NJARR = 6;
+constexpr int QLIM = kJoinQueueCap - 4 * DG;  // drain before a 4-row group could overflow
 
-enum { C_SWEPT = 0, C_BALL, C_KIM, C_KEOGH, C_DTW_AB, C_CELLS, C_NCOUNTERS };
+enum { C_SWEPT = 0, C_BALL, C_KIM, C_KEOGH, C_DTW_AB, C_CELLS, C_KEOGH_ELEMS, C_NCOUNTERS };
+
+// LDS floats of the cascade staging for window length m (overlays the column tables).
+__host__ __device__ constexpr int cascade_floats(int m) {
+    return 3 * (RC + m) + 3 * (RC + DG + m) + 3 * RC + 3 * (RC + DG);
+}
 
 struct Thr {
     float T2;      // squared-cost threshold (best-so-far inflated by the FP32 DTW error)
@@ -73,17 +80,17 @@ __device__ inline Thr load_threshold(const JoinArgs &A) {
 
 __device__ inline float sq(float x) { return x * x; }
 
-__device__ inline void decode_item(const JoinArgs &A, int64_t item, int &g, int64_t &s) {
-    int lo = 0, hi = A.groups - 1;  // find g with item_start[g] <= item < item_start[g+1]
+__device__ inline void decode_item(const int64_t *item_start, int groups, int64_t item, int &g, int64_t &s) {
+    int lo = 0, hi = groups - 1;  // g with item_start[g] <= item < item_start[g+1]
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (A.item_start[mid] <= item)
+        if (item_start[mid] <= item)
             lo = mid;
         else
             hi = mid - 1;
     }
     g = lo;
-    s = item - A.item_start[g];
+    s = item - item_start[g];
 }
 
 __device__ inline unsigned int mbcnt64(unsigned long long msk) {
@@ -100,30 +107,40 @@ __device__ inline float range_max(const JoinArgs &A, int which, int64_t lo, int6
 template <int Q>
 __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
     extern __shared__ float smem[];
-    float *jarr = smem;                                    // NJARR x [4][JWP]
-    int2 *q1 = reinterpret_cast<int2 *>(jarr + NJARR * JW);
-    int2 *q2 = q1 + Q1CAP;
-    int2 *q3 = q2 + Q2CAP;
-    float *zbuf = reinterpret_cast<float *>(q3 + Q3CAP);  // 2m
     const int lane = (int)threadIdx.x;
     const int m = A.m;
     const int64_t nw = A.nw;
-    const unsigned long long lane_bit = 1ull << lane;
+    const int64_t n = A.n;
+    // LDS: [union: column tables | cascade staging] [q2] [q3] [zbuf 2m]
+    const int uni = cascade_floats(m) > NJARR * JW ? cascade_floats(m) : NJARR * JW;
+    float *jarr = smem;  // NJARR x JW column tables of the sweep
+    // cascade staging (valid only while draining): series / envelope windows of the chunk's
+    // rows (positions c0 ..) and columns (positions c0+d0 ..), and window statistics
+    const int CI = RC + m, CJ = RC + DG + m;
+    float *sai = smem, *sui = sai + CI, *sli = sui + CI;
+    float *saj = sli + CI, *suj = saj + CJ, *slj = suj + CJ;
+    float *rmu = slj + CJ, *ris = rmu + RC, *rsg = ris + RC;
+    float *cmu = rsg + RC, *cis = cmu + (RC + DG), *csg = cis + (RC + DG);
+    int2 *q2 = reinterpret_cast<int2 *>(smem + uni);
+    int2 *q3 = q2 + Q2CAP;
+    float *zbuf = reinterpret_cast<float *>(q3 + Q3CAP);  // 2m
+    uint32_t *q1 = A.queue + (size_t)blockIdx.x * kJoinQueueCap;  // this wave's survivor queue
 
     unsigned long long c_swept = 0, c_ball = 0, c_kim = 0, c_keogh = 0, c_ab = 0, c_cells = 0;
+    unsigned long long c_kelem = 0;  // per lane: LB_Keogh positions evaluated
     int q1n = 0, q2n = 0, q3n = 0;
+    int64_t i_beg = 0, d0 = 0, c0 = 0;
     Thr th = load_threshold(A);
 
     // ---- stage 4: DTW of every queued pair, one pair at a time on the whole wave
     auto process_q3 = [&]() {
         for (int e = 0; e < q3n; ++e) {
-            const int2 pr = q3[e];
-            const int i = pr.x, j = pr.y;
+            const int2 pr = q3[e];  // local row / column offsets
             th = load_threshold(A);
-            const float mi = A.mu_c[i], ii = A.win[i].isig, mj = A.mu_c[j], ij = A.win[j].isig;
+            const float mi = rmu[pr.x], ii = ris[pr.x], mj = cmu[pr.y], ij = cis[pr.y];
             for (int k = lane; k < m; k += 64) {
-                zbuf[k] = (A.ac[i + k] - mi) * ii;
-                zbuf[m + k] = (A.ac[j + k] - mj) * ij;
+                zbuf[k] = (sai[pr.x + k] - mi) * ii;
+                zbuf[m + k] = (saj[pr.y + k] - mj) * ij;
             }
             __syncthreads();
             int64_t cells;
@@ -133,9 +150,10 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
             if (ab) {
                 ++c_ab;
             } else if (lane == 0) {
+                const int gi = (int)(c0 + pr.x), gj = (int)(c0 + d0 + pr.y);
                 int slot = atomicAdd(A.n_cands, 1);
-                if (slot < A.cand_cap) A.cands[slot] = make_int4(i, j, __float_as_int(cost), 0);
-                atomicMin(A.bsf, pack_bsf(cost, (uint32_t)i));
+                if (slot < A.cand_cap) A.cands[slot] = make_int4(gi, gj, __float_as_int(cost), 0);
+                atomicMin(A.bsf, pack_bsf(cost, (uint32_t)gi));
             }
             __syncthreads();
         }
@@ -149,19 +167,19 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
             const int batch = q2n < 64 ? q2n : 64;
             const int base = q2n - batch;
             const bool act = lane < batch;
-            int i = 0, j = 0;
+            int li = 0, lj = 0;  // local row / column offsets
             if (act) {
                 int2 pr = q2[base + lane];
-                i = pr.x;
-                j = pr.y;
+                li = pr.x;
+                lj = pr.y;
             }
             q2n = base;
             th = load_threshold(A);
             bool alive = act;
             float accA = 0.f, accB = 0.f, thrA = 0.f, thrB = 0.f, rA = 0.f, bA = 0.f, rB = 0.f, bB = 0.f;
             if (act) {
-                const float si = A.win[i].sig, sj = A.win[j].sig, isi = A.win[i].isig, isj = A.win[j].isig;
-                const float mci = A.mu_c[i], mcj = A.mu_c[j];
+                const float si = rsg[li], sj = csg[lj], isi = ris[li], isj = cis[lj];
+                const float mci = rmu[li], mcj = cmu[lj];
                 rA = sj * isi;
                 bA = mcj - mci * rA;
                 rB = si * isj;
@@ -177,14 +195,15 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
                 if (__ballot(alive) == 0ull) break;
                 if (alive) {
                     const int kend = k0 + 16 < m ? k0 + 16 : m;
+                    c_kelem += (unsigned long long)(kend - k0);
                     for (int k = k0; k < kend; ++k) {
-                        const float x = A.ac[j + k];
-                        const float hi = fmaf(A.uc[i + k], rA, bA), lo = fmaf(A.lc[i + k], rA, bA);
+                        const float x = saj[lj + k];
+                        const float hi = fmaf(sui[li + k], rA, bA), lo = fmaf(sli[li + k], rA, bA);
                         const float t = fmaxf(fmaxf(x - hi, lo - x), 0.f);
                         accA = fmaf(t, t, accA);
-                        const float y = A.ac[i + k];
-                        const float hj = fmaf(A.uc[j + k], rB, bB), lj = fmaf(A.lc[j + k], rB, bB);
-                        const float u = fmaxf(fmaxf(y - hj, lj - y), 0.f);
+                        const float y = sai[li + k];
+                        const float hj = fmaf(suj[lj + k], rB, bB), lj_ = fmaf(slj[lj + k], rB, bB);
+                        const float u = fmaxf(fmaxf(y - hj, lj_ - y), 0.f);
                         accB = fmaf(u, u, accB);
                     }
                     alive = (accA <= thrA) && (accB <= thrB);
@@ -192,7 +211,7 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
             }
             const unsigned long long msk = __ballot(alive);
             if (msk) {
-                if (alive) q3[q3n + (int)mbcnt64(msk)] = make_int2(i, j);
+                if (alive) q3[q3n + (int)mbcnt64(msk)] = make_int2(li, lj);
                 q3n += __popcll(msk);
                 c_keogh += __popcll(msk);
                 __syncthreads();
@@ -201,60 +220,91 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
         }
     };
 
-    // ---- stage 2: LB_Kim on the 3x3 corners of both ends, one pair per lane
-    auto process_q1 = [&](bool force) {
-        while (q1n >= 64 || (force && q1n > 0)) {
-            __syncthreads();
-            const int batch = q1n < 64 ? q1n : 64;
-            const int base = q1n - batch;
-            const bool act = lane < batch;
-            int i = 0, j = 0;
-            if (act) {
-                int2 pr = q1[base + lane];
-                i = pr.x;
-                j = pr.y;
-            }
-            q1n = base;
-            th = load_threshold(A);
-            bool alive = false;
-            if (act) {
-                const float mi = A.mu_c[i], ii = A.win[i].isig, mj = A.mu_c[j], ij = A.win[j].isig;
-                const float *ai = A.ac + i, *aj = A.ac + j;
-                // x: window i (columns), y: window j (rows)
-                const float x0 = (ai[0] - mi) * ii, x1 = (ai[1] - mi) * ii, x2 = (ai[2] - mi) * ii;
-                const float y0 = (aj[0] - mj) * ij, y1 = (aj[1] - mj) * ij, y2 = (aj[2] - mj) * ij;
-                const float xe0 = (ai[m - 1] - mi) * ii, xe1 = (ai[m - 2] - mi) * ii, xe2 = (ai[m - 3] - mi) * ii;
-                const float ye0 = (aj[m - 1] - mj) * ij, ye1 = (aj[m - 2] - mj) * ij, ye2 = (aj[m - 3] - mj) * ij;
-                // Any warping path from (0,0) first leaves {0,1}^2 at (2,0),(2,1),(2,2),(1,2) or
-                // (0,2), entered from an adjacent cell of {0,1}^2.  Same at the far corner.
-                const float c00 = sq(y0 - x0), c10 = sq(y1 - x0), c01 = sq(y0 - x1), c11 = sq(y1 - x1);
-                const float c20 = sq(y2 - x0), c21 = sq(y2 - x1), c22 = sq(y2 - x2), c12 = sq(y1 - x2),
-                            c02 = sq(y0 - x2);
-                float bl = c10 + c20;
-                bl = fminf(bl, fminf(c10, c11) + c21);
-                bl = fminf(bl, c11 + c22);
-                bl = fminf(bl, fminf(c01, c11) + c12);
-                bl = fminf(bl, c01 + c02);
-                const float e00 = sq(ye0 - xe0), e10 = sq(ye1 - xe0), e01 = sq(ye0 - xe1), e11 = sq(ye1 - xe1);
-                const float e20 = sq(ye2 - xe0), e21 = sq(ye2 - xe1), e22 = sq(ye2 - xe2), e12 = sq(ye1 - xe2),
-                            e02 = sq(ye0 - xe2);
-                float br = e10 + e20;
-                br = fminf(br, fminf(e10, e11) + e21);
-                br = fminf(br, e11 + e22);
-                br = fminf(br, fminf(e01, e11) + e12);
-                br = fminf(br, e01 + e02);
-                const float kim = (c00 + bl) + (e00 + br);
-                alive = kim <= th.kim;
-            }
-            const unsigned long long msk = __ballot(alive);
-            if (msk) {
-                if (alive) q2[q2n + (int)mbcnt64(msk)] = make_int2(i, j);
-                q2n += __popcll(msk);
-                c_kim += __popcll(msk);
-            }
-            if (q2n >= 64) process_q2(false);
+    // ---- drain: stage the chunk in LDS, then LB_Kim (stage 2) on 64-lane batches of the
+    // sweep's queue, feeding LB_Keogh and DTW; everything is flushed.
+    auto drain = [&]() {
+        __syncthreads();  // queue entries were stored by other lanes; column tables retire
+        for (int x = lane; x < CI; x += 64) {
+            const int64_t p = c0 + x;
+            const bool ok = p < n;
+            sai[x] = ok ? A.ac[p] : 0.f;
+            sui[x] = ok ? A.uc[p] : 0.f;
+            sli[x] = ok ? A.lc[p] : 0.f;
         }
-        if (force) process_q2(true);
+        for (int x = lane; x < CJ; x += 64) {
+            const int64_t p = c0 + d0 + x;
+            const bool ok = p < n;
+            saj[x] = ok ? A.ac[p] : 0.f;
+            suj[x] = ok ? A.uc[p] : 0.f;
+            slj[x] = ok ? A.lc[p] : 0.f;
+        }
+        for (int x = lane; x < RC + DG; x += 64) {
+            if (x < RC) {
+                const int64_t r = c0 + x;
+                const bool ok = r < nw;
+                rmu[x] = ok ? A.mu_c[r] : 0.f;
+                ris[x] = ok ? A.win[r].isig : 0.f;
+                rsg[x] = ok ? A.win[r].sig : 0.f;
+            }
+            const int64_t cc = c0 + d0 + x;
+            const bool okc = cc < nw;
+            cmu[x] = okc ? A.mu_c[cc] : 0.f;
+            cis[x] = okc ? A.win[cc].isig : 0.f;
+            csg[x] = okc ? A.win[cc].sig : 0.f;
+        }
+        __syncthreads();
+        for (int h = 0; h < q1n || q2n > 0; h += 64) {
+            const int batch = (q1n - h) < 64 ? (q1n - h) : 64;
+            if (batch > 0) {
+                const bool act = lane < batch;
+                int li = 0, lj = 0;
+                if (act) {
+                    const uint32_t e = q1[h + lane];
+                    li = (int)((int64_t)(e >> 8) - (c0 - i_beg));  // row offset in the chunk
+                    lj = li + (int)(e & 255u);                       // column offset
+                }
+                th = load_threshold(A);
+                bool alive = false;
+                if (act) {
+                    const float mi = rmu[li], ii = ris[li], mj = cmu[lj], ij = cis[lj];
+                    const float *ai = sai + li, *aj = saj + lj;
+                    // x: window i (columns), y: window j (rows)
+                    const float x0 = (ai[0] - mi) * ii, x1 = (ai[1] - mi) * ii, x2 = (ai[2] - mi) * ii;
+                    const float y0 = (aj[0] - mj) * ij, y1 = (aj[1] - mj) * ij, y2 = (aj[2] - mj) * ij;
+                    const float xe0 = (ai[m - 1] - mi) * ii, xe1 = (ai[m - 2] - mi) * ii, xe2 = (ai[m - 3] - mi) * ii;
+                    const float ye0 = (aj[m - 1] - mj) * ij, ye1 = (aj[m - 2] - mj) * ij, ye2 = (aj[m - 3] - mj) * ij;
+                    // Any warping path from (0,0) first leaves {0,1}^2 at (2,0),(2,1),(2,2),(1,2) or
+                    // (0,2), entered from an adjacent cell of {0,1}^2.  Same at the far corner.
+                    const float c00 = sq(y0 - x0), c10 = sq(y1 - x0), c01 = sq(y0 - x1), c11 = sq(y1 - x1);
+                    const float c20 = sq(y2 - x0), c21 = sq(y2 - x1), c22 = sq(y2 - x2), c12 = sq(y1 - x2),
+                                c02 = sq(y0 - x2);
+                    float bl = c10 + c20;
+                    bl = fminf(bl, fminf(c10, c11) + c21);
+                    bl = fminf(bl, c11 + c22);
+                    bl = fminf(bl, fminf(c01, c11) + c12);
+                    bl = fminf(bl, c01 + c02);
+                    const float e00 = sq(ye0 - xe0), e10 = sq(ye1 - xe0), e01 = sq(ye0 - xe1), e11 = sq(ye1 - xe1);
+                    const float e20 = sq(ye2 - xe0), e21 = sq(ye2 - xe1), e22 = sq(ye2 - xe2), e12 = sq(ye1 - xe2),
+                                e02 = sq(ye0 - xe2);
+                    float br = e10 + e20;
+                    br = fminf(br, fminf(e10, e11) + e21);
+                    br = fminf(br, e11 + e22);
+                    br = fminf(br, fminf(e01, e11) + e12);
+                    br = fminf(br, e01 + e02);
+                    const float kim = (c00 + bl) + (e00 + br);
+                    alive = kim <= th.kim;
+                }
+                const unsigned long long msk = __ballot(alive);
+                if (msk) {
+                    if (alive) q2[q2n + (int)mbcnt64(msk)] = make_int2(li, lj);
+                    q2n += __popcll(msk);
+                    c_kim += __popcll(msk);
+                }
+            }
+            process_q2(h + 64 >= q1n);
+        }
+        q1n = 0;
+        __syncthreads();
     };
 
     for (;;) {
@@ -264,10 +314,10 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
         if (item >= A.n_items) break;
         int g;
         int64_t sidx;
-        decode_item(A, item, g, sidx);
-        const int64_t d0 = (int64_t)m + (int64_t)DG * g;
+        decode_item(A.item_start, A.groups, item, g, sidx);
+        d0 = (int64_t)m + (int64_t)DG * g;
         const int64_t rows_g = nw - d0;  // rows of diagonal d0
-        const int64_t i_beg = sidx * A.rows_per_item;
+        i_beg = sidx * A.rows_per_item;
         const int64_t i_end = (i_beg + A.rows_per_item < rows_g) ? i_beg + A.rows_per_item : rows_g;
         const int64_t dq0 = d0 + KD * lane;
         // last valid row per owned diagonal: i <= nw - 1 - d
@@ -275,7 +325,8 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
 #pragma unroll
         for (int q = 0; q < KD; ++q) lim[q] = nw - 1 - (dq0 + q);
 
-        // ---- exact FP64 anchor of X^A, X^B at row i_beg
+        // ---- exact FP64 anchor of X^A, X^B at row i_beg (sliding window over the lane's
+        // four consecutive columns)
         float XA[KD], XB[KD];
         {
             double xa[KD], xb[KD];
@@ -283,15 +334,30 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
             for (int q = 0; q < KD; ++q) xa[q] = xb[q] = 0.0;
             const double mv0 = A.muv[i_beg], m0 = A.mu[i_beg];
             const int64_t jb = i_beg + dq0;
+            const bool ok0 = i_beg <= lim[0];
+            // lanes past the triangle read the last window instead (results unused)
+            const int64_t jr = ok0 ? jb : nw - 1 - (KD - 1);
+            double av[KD], vv[KD];
+#pragma unroll
+            for (int q = 0; q < KD - 1; ++q) {
+                av[q + 1] = A.a[jr + q];
+                vv[q + 1] = A.v[jr + q];
+            }
             for (int k = 0; k < m; ++k) {
                 const double cA = A.v[i_beg + k] - mv0;
                 const double cB = A.a[i_beg + k] - m0;
 #pragma unroll
+                for (int q = 0; q < KD - 1; ++q) {
+                    av[q] = av[q + 1];
+                    vv[q] = vv[q + 1];
+                }
+                const int64_t t = jr + k + KD - 1;
+                av[KD - 1] = (t <= n - 1) ? A.a[t] : 0.0;
+                vv[KD - 1] = (t <= n - 1) ? A.v[t] : 0.0;
+#pragma unroll
                 for (int q = 0; q < KD; ++q) {
-                    if (i_beg <= lim[q]) {
-                        xa[q] = fma(cA, A.a[jb + q + k], xa[q]);
-                        xb[q] = fma(cB, A.v[jb + q + k], xb[q]);
-                    }
+                    xa[q] = fma(cA, av[q], xa[q]);
+                    xb[q] = fma(cB, vv[q], xb[q]);
                 }
             }
 #pragma unroll
@@ -300,103 +366,118 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
                 XB[q] = (float)xb[q];
             }
         }
-        // ---- per-item rounding margins of the FP32 sweep (units of X / sigma)
+        // ---- per-item rounding margins of the FP32 sweep, in the units of X / sigma_j
+        // (resp. X / sigma_i): each step rounds two FMAs on values bounded by
+        // |X| <= m sigma_j sv_i, and the FP32 inputs carry 2^-24 relative error.
         const int64_t jlo = i_beg + d0, jhi = (i_end - 1 + d0 + DG - 1 < nw - 1) ? i_end - 1 + d0 + DG - 1 : nw - 1;
         const float smaxJ = range_max(A, 0, jlo, jhi), svmaxJ = range_max(A, 1, jlo, jhi),
                     ismaxJ = range_max(A, 2, jlo, jhi);
         const float smaxI = range_max(A, 0, i_beg, i_end - 1), svmaxI = range_max(A, 1, i_beg, i_end - 1),
                     ismaxI = range_max(A, 2, i_beg, i_end - 1);
         const float steps = (float)(i_end - i_beg + 8);
-        const float EA = 0x1p-20f * steps * (float)m * smaxJ * svmaxI * ismaxJ;
-        const float EB = 0x1p-20f * steps * (float)m * smaxI * svmaxJ * ismaxI;
+        const float EA = 0x1p-22f * steps * (float)m * smaxJ * svmaxI * ismaxJ;
+        const float EB = 0x1p-22f * steps * (float)m * smaxI * svmaxJ * ismaxI;
 
-        for (int64_t c0 = i_beg; c0 < i_end; c0 += RC) {
+        for (c0 = i_beg; c0 < i_end; c0 += RC) {
             const int64_t c1 = (c0 + RC < i_end) ? c0 + RC : i_end;
-            th = load_threshold(A);
-            const float T = th.T;
-            __syncthreads();
-            // stage column data for j = c0 + d0 + x, x in [0, JW), phase-split by x mod 4
-            for (int x = lane; x < JW; x += 64) {
-                const int64_t j = c0 + d0 + x;
-                float f0 = 0.f, f1 = 0.f, f2 = 0.f, f3 = 0.f, f4 = 0.f, f5 = 0.f;
-                if (j <= nw - 1) {
-                    const WinRow r = A.win[j];
-                    const float tB = T + r.R;
-                    f0 = r.dfa;
-                    f1 = r.dga;
-                    f2 = r.dfv;
-                    f3 = r.dgv;
-                    f4 = r.isig;
-                    f5 = (0.5f * (r.mN - tB * tB) - 0x1p-20f * (r.mN + tB * tB)) * r.sig - EB;
-                }
-                const int o = (x & 3) * JWP + (x >> 2);
-                jarr[0 * JW + o] = f0;
-                jarr[1 * JW + o] = f1;
-                jarr[2 * JW + o] = f2;
-                jarr[3 * JW + o] = f3;
-                jarr[4 * JW + o] = f4;
-                jarr[5 * JW + o] = f5;
+            // pairs of this chunk inside the triangle: row i has min(256, nw - i - d0) columns
+            for (int64_t i = c0; i < c1; ++i) {
+                const int64_t cols = nw - i - d0;
+                c_swept += (unsigned long long)(cols < DG ? (cols > 0 ? cols : 0) : DG);
             }
-            __syncthreads();
-            // column window e = 0..6 (e = row-in-group + q)
-            float jv[NJARR][7];
-#pragma unroll
-            for (int e = 0; e < 3; ++e) {
-                const int o = (e & 3) * JWP + (e >> 2) + lane;
-#pragma unroll
-                for (int r = 0; r < NJARR; ++r) jv[r][e] = jarr[r * JW + o];
-            }
-            for (int64_t rb = c0; rb < c1; rb += 4) {
-                const int g4 = (int)((rb - c0) >> 2);
-#pragma unroll
-                for (int e = 3; e < 7; ++e) {
-                    const int o = (e & 3) * JWP + g4 + (e >> 2) + lane;
-#pragma unroll
-                    for (int r = 0; r < NJARR; ++r) jv[r][e] = jarr[r * JW + o];
-                }
-                // uniform: every pair of the 4-row group is inside the triangle
-                const bool full = (rb + 3 < c1) && (rb + 3 + d0 + DG - 1 <= nw - 1);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int64_t i = rb + u;
-                    if (i >= c1) break;
-                    const int iu = __builtin_amdgcn_readfirstlane((int)i);
-                    const WinRow ri = A.win[iu];
-                    const float tA = T + ri.R;
-                    const float thA = (0.5f * (ri.mN - tA * tA) - 0x1p-20f * (ri.mN + tA * tA)) * ri.sig - EA;
-#pragma unroll
-                    for (int q = 0; q < KD; ++q) {
-                        const int e = u + q;
-                        const float pA = XA[q] * jv[4][e];
-                        const float pB = XB[q] * ri.isig;
-                        bool ok = (pA >= thA) && (pB >= jv[5][e]);
-                        if (!full) ok = ok && (i <= lim[q]);
-                        const unsigned long long msk = __ballot(ok);
-                        if (msk) {
-                            if (ok) q1[q1n + (int)mbcnt64(msk)] = make_int2((int)i, (int)(i + dq0 + q));
-                            q1n += __popcll(msk);
-                            c_ball += __popcll(msk);
-                        }
-                        if (full)
-                            c_swept += 64;
-                        else
-                            c_swept += __popcll(__ballot(i <= lim[q]));
-                        XA[q] = fmaf(jv[0][e], ri.dgv, XA[q]);
-                        XA[q] = fmaf(ri.dfv, jv[1][e], XA[q]);
-                        XB[q] = fmaf(ri.dfa, jv[3][e], XB[q]);
-                        XB[q] = fmaf(jv[2][e], ri.dga, XB[q]);
+            int64_t rb = c0;
+            while (rb < c1) {
+                th = load_threshold(A);
+                const float T = th.T;
+                // stage column data for j = c0 + d0 + x, x in [0, JW)
+                for (int x = lane; x < JW; x += 64) {
+                    const int64_t j = c0 + d0 + x;
+                    float f0 = 0.f, f1 = 0.f, f2 = 0.f, f3 = 0.f, f4 = 0.f, f5 = 0.f;
+                    if (j <= nw - 1) {
+                        const WinRow r = A.win[j];
+                        const float tB = T + r.R;
+                        f0 = r.dfa;
+                        f1 = r.dga;
+                        f2 = r.dfv;
+                        f3 = r.dgv;
+                        f4 = r.isig;
+                        f5 = (0.5f * (r.mN - tB * tB) - 0x1p-20f * (r.mN + tB * tB)) * r.sig - EB;
                     }
-                    if (q1n >= 64) process_q1(false);
+                    jarr[0 * JW + x] = f0;
+                    jarr[1 * JW + x] = f1;
+                    jarr[2 * JW + x] = f2;
+                    jarr[3 * JW + x] = f3;
+                    jarr[4 * JW + x] = f4;
+                    jarr[5 * JW + x] = f5;
                 }
+                __syncthreads();
+                for (; rb < c1; rb += 4) {
+                    // columns e = 0..7 of this lane for the 4-row group: x = (rb-c0) + 4 lane + e
+                    const int base = (int)(rb - c0) + 4 * lane;
+                    float jv[NJARR][8];
 #pragma unroll
-                for (int r = 0; r < NJARR; ++r) {
-                    jv[r][0] = jv[r][4];
-                    jv[r][1] = jv[r][5];
-                    jv[r][2] = jv[r][6];
+                    for (int r = 0; r < NJARR; ++r) {
+                        const float4 lo4 = *reinterpret_cast<const float4 *>(&jarr[r * JW + base]);
+                        const float4 hi4 = *reinterpret_cast<const float4 *>(&jarr[r * JW + base + 4]);
+                        jv[r][0] = lo4.x;
+                        jv[r][1] = lo4.y;
+                        jv[r][2] = lo4.z;
+                        jv[r][3] = lo4.w;
+                        jv[r][4] = hi4.x;
+                        jv[r][5] = hi4.y;
+                        jv[r][6] = hi4.z;
+                        jv[r][7] = hi4.w;
+                    }
+                    // uniform: every pair of the 4-row group is inside the triangle
+                    const bool full = (rb + 3 < c1) && (rb + 3 + d0 + DG - 1 <= nw - 1);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int64_t i = rb + u;
+                        if (i >= c1) break;
+                        const int iu = __builtin_amdgcn_readfirstlane((int)i);
+                        const WinRow ri = A.win[iu];
+                        const float tA = T + ri.R;
+                        const float thA = (0.5f * (ri.mN - tA * tA) - 0x1p-20f * (ri.mN + tA * tA)) * ri.sig - EA;
+                        bool ok[KD];
+#pragma unroll
+                        for (int q = 0; q < KD; ++q) {
+                            const int e = u + q;
+                            ok[q] = (XA[q] * jv[4][e] >= thA) && (XB[q] * ri.isig >= jv[5][e]);
+                            if (!full) ok[q] = ok[q] && (i <= lim[q]);
+                        }
+                        const unsigned long long m0 = __ballot(ok[0]), m1 = __ballot(ok[1]), m2 = __ballot(ok[2]),
+                                                 m3 = __ballot(ok[3]);
+                        if ((m0 | m1 | m2 | m3) != 0ull) {
+                            const uint32_t rowcode = (uint32_t)(i - i_beg) << 8;
+                            const unsigned long long mk[KD] = {m0, m1, m2, m3};
+#pragma unroll
+                            for (int q = 0; q < KD; ++q) {
+                                if (mk[q]) {
+                                    if (ok[q]) q1[q1n + (int)mbcnt64(mk[q])] = rowcode | (uint32_t)(KD * lane + q);
+                                    q1n += __popcll(mk[q]);
+                                    c_ball += __popcll(mk[q]);
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int q = 0; q < KD; ++q) {
+                            const int e = u + q;
+                            XA[q] = fmaf(jv[0][e], ri.dgv, XA[q]);
+                            XA[q] = fmaf(ri.dfv, jv[1][e], XA[q]);
+                            XB[q] = fmaf(ri.dfa, jv[3][e], XB[q]);
+                            XB[q] = fmaf(jv[2][e], ri.dga, XB[q]);
+                        }
+                    }
+                    if (q1n > QLIM) {
+                        rb += 4;
+                        break;
+                    }
                 }
+                // single drain point (end of chunk, or before the queue could overflow)
+                if (q1n > 0) drain();
+                __syncthreads();
             }
         }
-        process_q1(true);
     }
     if (lane == 0) {
         atomicAdd(&A.counters[C_SWEPT], c_swept);
@@ -406,6 +487,7 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
         atomicAdd(&A.counters[C_DTW_AB], c_ab);
         atomicAdd(&A.counters[C_CELLS], c_cells);
     }
+    if (c_kelem) atomicAdd(&A.counters[C_KEOGH_ELEMS], c_kelem);
 }
 
 // ---- seed scan: z-normalised Euclidean distance (an upper bound of banded DTW) over a
@@ -415,18 +497,12 @@ __global__ __launch_bounds__(64) void k_seed(SeedArgs S, int2 *pairs) {
     const int m = S.m;
     const int64_t nw = S.nw;
     const int64_t item = ((int64_t)blockIdx.x * S.n_items) / S.n_seed;
-    int lo = 0, hi = S.groups - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (S.item_start[mid] <= item)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const int g = lo;
+    int g;
+    int64_t sidx;
+    decode_item(S.item_start, S.groups, item, g, sidx);
     const int64_t d0 = (int64_t)m + (int64_t)DG * g;
     const int64_t rows_g = nw - d0;
-    const int64_t i_beg = (item - S.item_start[g]) * (int64_t)S.rows_per_item;
+    const int64_t i_beg = sidx * (int64_t)S.rows_per_item;
     int64_t i_end = i_beg + S.rows;
     if (i_end > rows_g) i_end = rows_g;
     const int64_t dq0 = d0 + KD * lane;
@@ -503,12 +579,17 @@ __global__ __launch_bounds__(64) void k_seed_dtw(JoinArgs A, const int2 *pairs, 
 }  // namespace
 
 size_t join_lds_bytes(int32_t m) {
-    return sizeof(float) * NJARR * JW + sizeof(int2) * (Q1CAP + Q2CAP + Q3CAP) + sizeof(float) * 2 * (size_t)m;
+    const int uni = cascade_floats(m) > NJARR * JW ? cascade_floats(m) : NJARR * JW;
+    return sizeof(float) * (size_t)uni + sizeof(int2) * (Q2CAP + Q3CAP) + sizeof(float) * 2 * (size_t)m;
 }
 
 int launch_join(const JoinArgs &A, int32_t grid, hipStream_t s) {
     const int Q = dtw_q_for(A.w);
     const size_t lds = join_lds_bytes(A.m);
+    if (lds > 160 * 1024) {
+        set_error("window too long for the join's LDS staging");
+        return DTWJ_EINVAL;
+    }
     switch (Q) {
         case 1: hipLaunchKernelGGL(k_join<1>, dim3(grid), dim3(64), lds, s, A); break;
         case 2: hipLaunchKernelGGL(k_join<2>, dim3(grid), dim3(64), lds, s, A); break;

@@ -96,18 +96,22 @@ __global__ __launch_bounds__(256) void k_envelope(const double *__restrict__ a, 
     v[t] = 0.5 * mx + 0.5 * mn;
 }
 
-// Channel values of the prefix-sum scan.
+// Channel values of the prefix-sum scan, squares as exact double-double products (TwoProd).
+__device__ inline dd_t exact_sq(double x) {
+    const double p = x * x;
+    return {p, fma(x, x, -p)};
+}
 __device__ inline void scan_values(const double *__restrict__ a, const double *__restrict__ U,
-                                   const double *__restrict__ L, double c0, int64_t t, double x[5]) {
+                                   const double *__restrict__ L, double c0, int64_t t, dd_t x[5]) {
     double ad = a[t] - c0;
     double u = U[t], l = L[t];
     double vd = (0.5 * u + 0.5 * l) - c0;
     double h = 0.5 * u - 0.5 * l;
-    x[0] = ad;
-    x[1] = ad * ad;
-    x[2] = vd;
-    x[3] = vd * vd;
-    x[4] = h * h;
+    x[0] = {ad, 0.0};
+    x[1] = exact_sq(ad);
+    x[2] = {vd, 0.0};
+    x[3] = exact_sq(vd);
+    x[4] = exact_sq(h);
 }
 
 // Inclusive block scan of one dd value per thread (Hillis-Steele in LDS).
@@ -139,9 +143,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partial(const double *__r
     for (int k = 0; k < kScanPerThread; ++k) {
         int64_t t = base + k;
         if (t < n) {
-            double x[5];
+            dd_t x[5];
             scan_values(a, U, L, c0, t, x);
-            for (int c = 0; c < 5; ++c) acc[c] = dd_add(acc[c], dd_t{x[c], 0.0});
+            for (int c = 0; c < 5; ++c) acc[c] = dd_add(acc[c], x[c]);
         }
     }
     for (int c = 0; c < 5; ++c) {
@@ -174,7 +178,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const double *__res
     __shared__ dd_t buf[kScanThreads];
     const double c0 = *c0p;
     int64_t base = (int64_t)blockIdx.x * kScanBlock + (int64_t)threadIdx.x * kScanPerThread;
-    double xs[kScanPerThread][5];
+    dd_t xs[kScanPerThread][5];
     dd_t tot[5];
     for (int c = 0; c < 5; ++c) tot[c] = {0.0, 0.0};
     for (int k = 0; k < kScanPerThread; ++k) {
@@ -182,9 +186,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const double *__res
         if (t < n) {
             scan_values(a, U, L, c0, t, xs[k]);
         } else {
-            for (int c = 0; c < 5; ++c) xs[k][c] = 0.0;
+            for (int c = 0; c < 5; ++c) xs[k][c] = {0.0, 0.0};
         }
-        for (int c = 0; c < 5; ++c) tot[c] = dd_add(tot[c], dd_t{xs[k][c], 0.0});
+        for (int c = 0; c < 5; ++c) tot[c] = dd_add(tot[c], xs[k][c]);
     }
     for (int c = 0; c < 5; ++c) {
         dd_t inc = block_scan_dd(tot[c], buf);
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const double *__res
         if (blockIdx.x == 0 && threadIdx.x == 0) Pc[0] = {0.0, 0.0};
         for (int k = 0; k < kScanPerThread; ++k) {
             int64_t t = base + k;
-            run = dd_add(run, dd_t{xs[k][c], 0.0});
+            run = dd_add(run, xs[k][c]);
             if (t < n) Pc[t + 1] = run;
         }
     }

@@ -1,0 +1,39 @@
+"""The CPU oracle against golden vectors written by the compiled reference CPU path.
+
+tests/golden/golden.json holds, per case, what CPU_version1 printed for the stored series:
+1-based first/second index (CPU_version1/new_dtw_motifGUI.cpp:575), final distance and seed
+distance (:578, printed with %f / %5.3f) and the diag_fast / global-stage prune fractions
+(:569-570, printed with 6 significant digits).
+"""
+import math
+
+import pytest
+
+from conftest import golden_cases, golden_series
+
+CASES = golden_cases()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_oracle_matches_reference(name, oracle):
+    c = CASES[name]
+    a = golden_series(name)
+    r = oracle.motif_discovery(a, c["m"], c["w"])
+    ref = c["ref"]
+    # indices exactly as the reference reports them (1-based if found by the sweep,
+    # 0-based seed fallback otherwise)
+    assert r.first_min == int(ref["first_min"]) and r.sec_min == int(ref["sec_min"])
+    assert abs(r.bsf - ref["final_distance"]) <= 5e-7
+    assert abs(r.seed_bsf - ref["input_bsf"]) <= 5e-4
+    all_cnt = r.all_cnt
+    for key, val in (("fast_prunes", r.fast_cnt), ("global_prunes", r.global_cnt)):
+        got = val / all_cnt
+        want = ref[key]
+        assert math.isclose(got, want, rel_tol=6e-6, abs_tol=1e-12), (key, got, want)
+
+
+def test_golden_pairs_are_admissible():
+    for name, c in CASES.items():
+        ref = c["ref"]
+        i, j = int(ref["first_min"]), int(ref["sec_min"])
+        assert j > i and j - i >= c["m"]
