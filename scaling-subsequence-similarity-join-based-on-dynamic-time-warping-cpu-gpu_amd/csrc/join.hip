@@ -30,6 +30,8 @@
 // (earlier only if it could overflow): the chunk's series / envelope windows and window
 // statistics are then staged in LDS over the sweep's column tables, so stages 2-4 read LDS,
 // not scattered global memory, one pair per lane on full 64-lane batches.
+#include <type_traits>
+
 #include "dtw_device.hpp"
 #include "../../include/dtwj.h"
 #include "internal.hpp"
@@ -43,6 +45,7 @@ constexpr int DG = 64 * KD;            // diagonals per item
 constexpr int RC = 64;                 // rows per LDS chunk
 constexpr int JW = RC + DG + 8;        // staged column entries per chunk
 constexpr int NJARR = 6;               // column arrays: dfa dga dfv dgv 1/sigma thetaB
+constexpr int kSweepFloats = NJARR * JW + NJARR * RC;  // column + row tables
 constexpr int Q2CAP = 128;
 constexpr int Q3CAP = 128;
 constexpr int QLIM = kJoinQueueCap - 4 * DG;  // drain before a 4-row group could overflow
@@ -112,8 +115,9 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
     const int64_t nw = A.nw;
     const int64_t n = A.n;
     // LDS: [union: column tables | cascade staging] [q2] [q3] [zbuf 2m]
-    const int uni = cascade_floats(m) > NJARR * JW ? cascade_floats(m) : NJARR * JW;
-    float *jarr = smem;  // NJARR x JW column tables of the sweep
+    const int uni = cascade_floats(m) > kSweepFloats ? cascade_floats(m) : kSweepFloats;
+    float *jarr = smem;              // NJARR x JW column tables of the sweep
+    float *rarr = jarr + NJARR * JW; // NJARR x RC row tables of the sweep
     // cascade staging (valid only while draining): series / envelope windows of the chunk's
     // rows (positions c0 ..) and columns (positions c0+d0 ..), and window statistics
     const int CI = RC + m, CJ = RC + DG + m;
@@ -379,16 +383,47 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
         const float EB = 0x1p-22f * steps * (float)m * smaxI * svmaxJ * ismaxI;
 
         for (c0 = i_beg; c0 < i_end; c0 += RC) {
-            const int64_t c1 = (c0 + RC < i_end) ? c0 + RC : i_end;
+            const int nrows = (int)(((c0 + RC < i_end) ? c0 + RC : i_end) - c0);
             // pairs of this chunk inside the triangle: row i has min(256, nw - i - d0) columns
-            for (int64_t i = c0; i < c1; ++i) {
-                const int64_t cols = nw - i - d0;
+            for (int r = 0; r < nrows; ++r) {
+                const int64_t cols = nw - (c0 + r) - d0;
                 c_swept += (unsigned long long)(cols < DG ? (cols > 0 ? cols : 0) : DG);
             }
-            int64_t rb = c0;
-            while (rb < c1) {
+            // rows of the chunk whose pair with column q is inside the triangle: r <= limr[q]
+            int limr[KD];
+#pragma unroll
+            for (int q = 0; q < KD; ++q) {
+                const int64_t l = lim[q] - c0;
+                limr[q] = l < -1 ? -1 : (l > RC ? RC : (int)l);
+            }
+            // the whole chunk is inside the triangle for every lane
+            const bool chunk_full = c0 + nrows - 1 + d0 + DG - 1 <= nw - 1;
+            const uint32_t code0 = (uint32_t)(c0 - i_beg) << 8;
+            int rl = 0;  // next local row (multiple of 4)
+            while (rl < nrows) {
                 th = load_threshold(A);
                 const float T = th.T;
+                // stage row data for the chunk's rows: dfa dga dfv dgv 1/sigma thetaA
+                for (int x = lane; x < RC; x += 64) {
+                    const int64_t i = c0 + x;
+                    float f0 = 0.f, f1 = 0.f, f2 = 0.f, f3 = 0.f, f4 = 0.f, f5 = 0.f;
+                    if (x < nrows) {
+                        const WinRow r = A.win[i];
+                        const float tA = T + r.R;
+                        f0 = r.dfa;
+                        f1 = r.dga;
+                        f2 = r.dfv;
+                        f3 = r.dgv;
+                        f4 = r.isig;
+                        f5 = (0.5f * (r.mN - tA * tA) - 0x1p-20f * (r.mN + tA * tA)) * r.sig - EA;
+                    }
+                    rarr[0 * RC + x] = f0;
+                    rarr[1 * RC + x] = f1;
+                    rarr[2 * RC + x] = f2;
+                    rarr[3 * RC + x] = f3;
+                    rarr[4 * RC + x] = f4;
+                    rarr[5 * RC + x] = f5;
+                }
                 // stage column data for j = c0 + d0 + x, x in [0, JW)
                 for (int x = lane; x < JW; x += 64) {
                     const int64_t j = c0 + d0 + x;
@@ -411,9 +446,11 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
                     jarr[5 * JW + x] = f5;
                 }
                 __syncthreads();
-                for (; rb < c1; rb += 4) {
-                    // columns e = 0..7 of this lane for the 4-row group: x = (rb-c0) + 4 lane + e
-                    const int base = (int)(rb - c0) + 4 * lane;
+                // One 4-row group: rows rl..rl+3 of the chunk against this lane's columns
+                // x = rl + 4 lane + e (e = u + q).  FULL: every pair is inside the triangle.
+                auto group = [&](auto full_tag) {
+                    constexpr bool FULL = decltype(full_tag)::value;
+                    const int base = rl + 4 * lane;
                     float jv[NJARR][8];
 #pragma unroll
                     for (int r = 0; r < NJARR; ++r) {
@@ -428,32 +465,34 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
                         jv[r][6] = hi4.z;
                         jv[r][7] = hi4.w;
                     }
-                    // uniform: every pair of the 4-row group is inside the triangle
-                    const bool full = (rb + 3 < c1) && (rb + 3 + d0 + DG - 1 <= nw - 1);
+                    float rv[NJARR][4];  // row data of the 4 rows (same address on every lane)
+#pragma unroll
+                    for (int r = 0; r < NJARR; ++r) {
+                        const float4 v4 = *reinterpret_cast<const float4 *>(&rarr[r * RC + rl]);
+                        rv[r][0] = v4.x;
+                        rv[r][1] = v4.y;
+                        rv[r][2] = v4.z;
+                        rv[r][3] = v4.w;
+                    }
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        const int64_t i = rb + u;
-                        if (i >= c1) break;
-                        const int iu = __builtin_amdgcn_readfirstlane((int)i);
-                        const WinRow ri = A.win[iu];
-                        const float tA = T + ri.R;
-                        const float thA = (0.5f * (ri.mN - tA * tA) - 0x1p-20f * (ri.mN + tA * tA)) * ri.sig - EA;
-                        bool ok[KD];
+                        if (!FULL && rl + u >= nrows) break;
+                        unsigned long long mk[KD];
 #pragma unroll
                         for (int q = 0; q < KD; ++q) {
                             const int e = u + q;
-                            ok[q] = (XA[q] * jv[4][e] >= thA) && (XB[q] * ri.isig >= jv[5][e]);
-                            if (!full) ok[q] = ok[q] && (i <= lim[q]);
+                            // one lane mask per test, combined on the scalar unit
+                            mk[q] = __builtin_amdgcn_ballot_w64(XA[q] * jv[4][e] >= rv[5][u]) &
+                                    __builtin_amdgcn_ballot_w64(XB[q] * rv[4][u] >= jv[5][e]);
+                            if (!FULL) mk[q] &= __builtin_amdgcn_ballot_w64(rl + u <= limr[q]);
                         }
-                        const unsigned long long m0 = __ballot(ok[0]), m1 = __ballot(ok[1]), m2 = __ballot(ok[2]),
-                                                 m3 = __ballot(ok[3]);
-                        if ((m0 | m1 | m2 | m3) != 0ull) {
-                            const uint32_t rowcode = (uint32_t)(i - i_beg) << 8;
-                            const unsigned long long mk[KD] = {m0, m1, m2, m3};
+                        if ((mk[0] | mk[1] | mk[2] | mk[3]) != 0ull) {
+                            const uint32_t rowcode = code0 + ((uint32_t)(rl + u) << 8);
 #pragma unroll
                             for (int q = 0; q < KD; ++q) {
                                 if (mk[q]) {
-                                    if (ok[q]) q1[q1n + (int)mbcnt64(mk[q])] = rowcode | (uint32_t)(KD * lane + q);
+                                    if ((mk[q] >> lane) & 1ull)
+                                        q1[q1n + (int)mbcnt64(mk[q])] = rowcode | (uint32_t)(KD * lane + q);
                                     q1n += __popcll(mk[q]);
                                     c_ball += __popcll(mk[q]);
                                 }
@@ -462,14 +501,20 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
 #pragma unroll
                         for (int q = 0; q < KD; ++q) {
                             const int e = u + q;
-                            XA[q] = fmaf(jv[0][e], ri.dgv, XA[q]);
-                            XA[q] = fmaf(ri.dfv, jv[1][e], XA[q]);
-                            XB[q] = fmaf(ri.dfa, jv[3][e], XB[q]);
-                            XB[q] = fmaf(jv[2][e], ri.dga, XB[q]);
+                            XA[q] = fmaf(jv[0][e], rv[3][u], XA[q]);
+                            XA[q] = fmaf(rv[2][u], jv[1][e], XA[q]);
+                            XB[q] = fmaf(rv[0][u], jv[3][e], XB[q]);
+                            XB[q] = fmaf(jv[2][e], rv[1][u], XB[q]);
                         }
                     }
+                };
+                for (; rl < nrows; rl += 4) {
+                    if (chunk_full && rl + 4 <= nrows)
+                        group(std::integral_constant<bool, true>());
+                    else
+                        group(std::integral_constant<bool, false>());
                     if (q1n > QLIM) {
-                        rb += 4;
+                        rl += 4;
                         break;
                     }
                 }
@@ -579,7 +624,7 @@ __global__ __launch_bounds__(64) void k_seed_dtw(JoinArgs A, const int2 *pairs, 
 }  // namespace
 
 size_t join_lds_bytes(int32_t m) {
-    const int uni = cascade_floats(m) > NJARR * JW ? cascade_floats(m) : NJARR * JW;
+    const int uni = cascade_floats(m) > kSweepFloats ? cascade_floats(m) : kSweepFloats;
     return sizeof(float) * (size_t)uni + sizeof(int2) * (Q2CAP + Q3CAP) + sizeof(float) * 2 * (size_t)m;
 }
 
