@@ -30,12 +30,12 @@ LIB_SOURCES = ["dtwj_api.cpp", "series_gen.cpp", "loader.cpp", "prep.hip", "join
 CLI_SOURCES = ["gpu_dtw_cli.cpp"]
 
 
-def _compile(src):
+def _compile(src, force=False):
     path = os.path.join(CSRC, src)
     out = os.path.join(OBJ, src + ".o")
     deps = [path] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith((".h", ".hpp", ".inc"))]
     deps.append(os.path.join(ROOT, "include", "dtwj.h"))
-    if os.path.exists(out) and all(os.path.getmtime(out) >= os.path.getmtime(d) for d in deps):
+    if not force and os.path.exists(out) and all(os.path.getmtime(out) >= os.path.getmtime(d) for d in deps):
         return out, None
     cmd = [HIPCC] + COMMON
     if src.endswith(".hip"):
@@ -47,12 +47,12 @@ def _compile(src):
     return out, None
 
 
-def build(jobs=8, verbose=False):
+def build(jobs=8, verbose=False, force=False):
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB, exist_ok=True)
     os.makedirs(BIN, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(_compile, LIB_SOURCES + CLI_SOURCES))
+        results = list(ex.map(lambda src: _compile(src, force), LIB_SOURCES + CLI_SOURCES))
     errs = [e for _, e in results if e]
     if errs:
         raise RuntimeError("compilation failed:\n" + "\n".join(errs))
@@ -77,4 +77,4 @@ if __name__ == "__main__":
     j = 8
     if "-j" in sys.argv:
         j = int(sys.argv[sys.argv.index("-j") + 1])
-    build(jobs=j, verbose=True)
+    build(jobs=j, verbose=True, force="--force" in sys.argv)
