@@ -3,6 +3,7 @@
 Everything is built in-tree with hipcc (no JIT cache):
     <package>/lib/libdtwj.so   shared library exporting include/dtwj.h
     <package>/bin/gpu_dtw      drop-in for the reference's GPU_dtw <data> <time_out> <m> <r>
+    <package>/bin/copy_calib   plain 1 GiB copy kernel for calibrating HBM traffic counters
 
 Usage: python build.py [-j N]
 """
@@ -28,6 +29,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", o
 
 LIB_SOURCES = ["dtwj_api.cpp", "series_gen.cpp", "loader.cpp", "prep.hip", "join.hip", "dtw.hip"]
 CLI_SOURCES = ["gpu_dtw_cli.cpp"]
+TOOL_SOURCES = ["copy_calib.hip"]   # standalone executables (traffic-counter calibration)
 
 
 def _compile(src, force=False):
@@ -52,7 +54,7 @@ def build(jobs=8, verbose=False, force=False):
     os.makedirs(LIB, exist_ok=True)
     os.makedirs(BIN, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(lambda src: _compile(src, force), LIB_SOURCES + CLI_SOURCES))
+        results = list(ex.map(lambda src: _compile(src, force), LIB_SOURCES + CLI_SOURCES + TOOL_SOURCES))
     errs = [e for _, e in results if e]
     if errs:
         raise RuntimeError("compilation failed:\n" + "\n".join(errs))
@@ -68,6 +70,11 @@ def build(jobs=8, verbose=False, force=False):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("CLI link failed:\n%s\n%s" % (r.stdout, r.stderr))
+    for src in TOOL_SOURCES:
+        tool = os.path.join(BIN, src.rsplit(".", 1)[0])
+        r = subprocess.run([HIPCC, "--offload-arch=" + ARCH, "-o", tool, objs[src]], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("%s link failed:\n%s\n%s" % (src, r.stdout, r.stderr))
     if verbose:
         print("built", lib, exe)
     return lib

@@ -19,7 +19,8 @@
 //     updated in O(1) per step with the MPX identity
 //     X(i+1,j+1) = X(i,j) + dfa[j] dgv[i] + dfv[i] dga[j], re-anchored exactly (FP64) at the
 //     start of every item; a per-item rounding margin keeps the FP32 test conservative.
-//     Per pair: 4 FMA + 2 MUL + 2 compares.
+//     Per pair: 4 FMA updates + 2 FMA tests + 1 min; the column data of a lane's four
+//     diagonals rotates through registers (one DPP wave shift per table and row).
 //  2. LB_Kim over the 3x3 corners at both ends (cf. CPU_version1/lb_kim.cpp:10-73).
 //  3. two-sided LB_Keogh against the raw-series envelope normalised by the other window's
 //     statistics (cf. lb_keogh_stride_with_nomalise, GPU_version1/src/LB_keogh_warp.cu:74-96),
@@ -48,7 +49,7 @@ constexpr int NJARR = 6;               // column arrays: dfa dga dfv dgv 1/sigma
 constexpr int kSweepFloats = NJARR * JW + NJARR * RC;  // column + row tables
 constexpr int Q2CAP = 128;
 constexpr int Q3CAP = 128;
-constexpr int QLIM = kJoinQueueCap - 4 * DG;  // drain before a 4-row group could overflow
+constexpr int QLIM = kJoinQueueCap - 8 * DG;  // drain before an 8-row group could overflow
 
 enum { C_SWEPT = 0, C_BALL, C_KIM, C_KEOGH, C_DTW_AB, C_CELLS, C_KEOGH_ELEMS, C_NCOUNTERS };
 
@@ -96,6 +97,27 @@ __device__ inline void decode_item(const int64_t *item_start, int groups, int64_
     s = item - item_start[g];
 }
 
+// Lane L receives src from lane L + 1; lane 63 receives `last` (DPP wave_shl:1 with the
+// out-of-range lane left at the old value).
+__device__ inline float wave_shift_down1(float last, float src) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, last),
+                                                                 __builtin_bit_cast(int, src), 0x130, 0xf, 0xf, false));
+}
+
+// Copies a uniform pointer into vector registers.  The join keeps many uniform values alive;
+// pointers used only in a short loop are moved out of the scalar register file so that the
+// compiler does not spill (and reload) scalar register tuples around that loop.
+template <typename T>
+using global_ptr = const __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ inline global_ptr<T> in_vgprs(const T *p) {
+    const uint64_t u = (uint64_t)p;
+    uint32_t lo = (uint32_t)u, hi = (uint32_t)(u >> 32);
+    asm volatile("v_mov_b32 %0, %1" : "=v"(lo) : "s"(lo));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(hi) : "s"(hi));
+    return (global_ptr<T>)(((uint64_t)hi << 32) | lo);
+}
+
 __device__ inline unsigned int mbcnt64(unsigned long long msk) {
     return __builtin_amdgcn_mbcnt_hi((unsigned int)(msk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)msk, 0u));
 }
@@ -109,7 +131,7 @@ __device__ inline float range_max(const JoinArgs &A, int which, int64_t lo, int6
 
 template <int Q>
 __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
-    extern __shared__ float smem[];
+    extern __shared__ __align__(16) float smem[];  // 16-byte aligned: the sweep reads float4 rows
     const int lane = (int)threadIdx.x;
     const int m = A.m;
     const int64_t nw = A.nw;
@@ -130,7 +152,13 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
     float *zbuf = reinterpret_cast<float *>(q3 + Q3CAP);  // 2m
     uint32_t *q1 = A.queue + (size_t)blockIdx.x * kJoinQueueCap;  // this wave's survivor queue
 
-    unsigned long long c_swept = 0, c_ball = 0, c_kim = 0, c_keogh = 0, c_ab = 0, c_cells = 0;
+    // Stage counters live in LDS (updated by lane 0 on the rare paths) so that they occupy no
+    // scalar registers in the sweep; LB_Keogh positions are counted per lane.
+    __shared__ __align__(16) unsigned long long wcnt[8];  // 64 bytes keep the dynamic area aligned
+    if (lane < C_NCOUNTERS) wcnt[lane] = 0ull;
+    auto count = [&](int which, unsigned long long v) {
+        if (lane == 0) wcnt[which] += v;
+    };
     unsigned long long c_kelem = 0;  // per lane: LB_Keogh positions evaluated
     int q1n = 0, q2n = 0, q3n = 0;
     int64_t i_beg = 0, d0 = 0, c0 = 0;
@@ -150,9 +178,9 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
             int64_t cells;
             bool ab;
             const float cost = dtw_wave<float, Q>(zbuf, zbuf + m, m, A.w, th.dtw_ub, cells, ab);
-            c_cells += (unsigned long long)cells;
+            count(C_CELLS, (unsigned long long)cells);
             if (ab) {
-                ++c_ab;
+                count(C_DTW_AB, 1ull);
             } else if (lane == 0) {
                 const int gi = (int)(c0 + pr.x), gj = (int)(c0 + d0 + pr.y);
                 int slot = atomicAdd(A.n_cands, 1);
@@ -217,7 +245,7 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
             if (msk) {
                 if (alive) q3[q3n + (int)mbcnt64(msk)] = make_int2(li, lj);
                 q3n += __popcll(msk);
-                c_keogh += __popcll(msk);
+                count(C_KEOGH, __popcll(msk));
                 __syncthreads();
                 process_q3();
             }
@@ -302,7 +330,7 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
                 if (msk) {
                     if (alive) q2[q2n + (int)mbcnt64(msk)] = make_int2(li, lj);
                     q2n += __popcll(msk);
-                    c_kim += __popcll(msk);
+                    count(C_KIM, __popcll(msk));
                 }
             }
             process_q2(h + 64 >= q1n);
@@ -329,35 +357,39 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
 #pragma unroll
         for (int q = 0; q < KD; ++q) lim[q] = nw - 1 - (dq0 + q);
 
-        // ---- exact FP64 anchor of X^A, X^B at row i_beg (sliding window over the lane's
-        // four consecutive columns)
+        // ---- exact FP64 anchor of X^A, X^B at row i_beg: for the lane's four consecutive
+        // columns j = i_beg + dq0 + q, X^A_q = sum_k (v_{i+k} - muv_i) a_{j+k} and
+        // X^B_q = sum_k (a_{i+k} - mu_i) v_{j+k}.  A column past the triangle (j > nw - 1)
+        // reads clamped positions; its values are never used.  n < 2^31 (checked by the host).
         float XA[KD], XB[KD];
         {
             double xa[KD], xb[KD];
 #pragma unroll
             for (int q = 0; q < KD; ++q) xa[q] = xb[q] = 0.0;
             const double mv0 = A.muv[i_beg], m0 = A.mu[i_beg];
-            const int64_t jb = i_beg + dq0;
-            const bool ok0 = i_beg <= lim[0];
-            // lanes past the triangle read the last window instead (results unused)
-            const int64_t jr = ok0 ? jb : nw - 1 - (KD - 1);
+            const int ib = (int)i_beg, last = (int)n - 1;
+            const global_ptr<double> pa = in_vgprs(A.a), pv = in_vgprs(A.v);
+            const int jb = (int)(i_beg + dq0);
+            // sliding window over positions jb + k .. jb + k + 3
             double av[KD], vv[KD];
 #pragma unroll
             for (int q = 0; q < KD - 1; ++q) {
-                av[q + 1] = A.a[jr + q];
-                vv[q + 1] = A.v[jr + q];
+                const int t = min(jb + q, last);
+                av[q + 1] = pa[t];
+                vv[q + 1] = pv[t];
             }
+#pragma unroll 4
             for (int k = 0; k < m; ++k) {
-                const double cA = A.v[i_beg + k] - mv0;
-                const double cB = A.a[i_beg + k] - m0;
+                const double cA = pv[ib + k] - mv0;
+                const double cB = pa[ib + k] - m0;
 #pragma unroll
                 for (int q = 0; q < KD - 1; ++q) {
                     av[q] = av[q + 1];
                     vv[q] = vv[q + 1];
                 }
-                const int64_t t = jr + k + KD - 1;
-                av[KD - 1] = (t <= n - 1) ? A.a[t] : 0.0;
-                vv[KD - 1] = (t <= n - 1) ? A.v[t] : 0.0;
+                const int t = min(jb + k + KD - 1, last);
+                av[KD - 1] = pa[t];
+                vv[KD - 1] = pv[t];
 #pragma unroll
                 for (int q = 0; q < KD; ++q) {
                     xa[q] = fma(cA, av[q], xa[q]);
@@ -369,6 +401,15 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
                 XA[q] = (float)xa[q];
                 XB[q] = (float)xb[q];
             }
+        }
+        // pairs of the item inside the triangle: row i has min(DG, nw - d0 - i) >= 1 columns
+        {
+            const int64_t F = nw - d0;
+            const int64_t full_end = (F - DG + 1 < i_end) ? F - DG + 1 : i_end;  // rows with DG columns
+            int64_t cnt = full_end > i_beg ? (full_end - i_beg) * DG : 0;
+            const int64_t a0 = (i_beg > F - DG + 1) ? i_beg : F - DG + 1, b0 = i_end - 1;
+            if (a0 <= b0) cnt += (b0 - a0 + 1) * (2 * F - a0 - b0) / 2;
+            count(C_SWEPT, (unsigned long long)cnt);
         }
         // ---- per-item rounding margins of the FP32 sweep, in the units of X / sigma_j
         // (resp. X / sigma_i): each step rounds two FMAs on values bounded by
@@ -384,11 +425,6 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
 
         for (c0 = i_beg; c0 < i_end; c0 += RC) {
             const int nrows = (int)(((c0 + RC < i_end) ? c0 + RC : i_end) - c0);
-            // pairs of this chunk inside the triangle: row i has min(256, nw - i - d0) columns
-            for (int r = 0; r < nrows; ++r) {
-                const int64_t cols = nw - (c0 + r) - d0;
-                c_swept += (unsigned long long)(cols < DG ? (cols > 0 ? cols : 0) : DG);
-            }
             // rows of the chunk whose pair with column q is inside the triangle: r <= limr[q]
             int limr[KD];
 #pragma unroll
@@ -446,75 +482,97 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
                     jarr[5 * JW + x] = f5;
                 }
                 __syncthreads();
-                // One 4-row group: rows rl..rl+3 of the chunk against this lane's columns
-                // x = rl + 4 lane + e (e = u + q).  FULL: every pair is inside the triangle.
+                // Column slots of this lane: slot q holds the staged column x = r + 4 lane + q
+                // for the current local row r.  Moving to row r + 1 shifts every slot by one
+                // column: slots 1..3 move down, slot 3 takes lane + 1's slot 0 (a DPP wave shift)
+                // and lane 63 takes the new column x = r + 256 (a broadcast LDS read).
+                float cs[NJARR][KD];
+#pragma unroll
+                for (int r = 0; r < NJARR; ++r) {
+                    const float4 v4 = *reinterpret_cast<const float4 *>(&jarr[r * JW + rl + 4 * lane]);
+                    cs[r][0] = v4.x;
+                    cs[r][1] = v4.y;
+                    cs[r][2] = v4.z;
+                    cs[r][3] = v4.w;
+                }
+                // One 8-row group: rows rl..rl+7 of the chunk, in two halves of 4 rows whose
+                // entering columns are read into separate registers, so that the rotating
+                // column slots end every group in the registers they started in.  FULL: every
+                // pair is inside the triangle.  Test per pair: X^A / sigma_j >= thetaA_i and
+                // X^B / sigma_i >= thetaB_j as signs of single-rounding FMAs; a lane-wide
+                // min/max reduces the four pairs to one compare per row, and lane masks are only
+                // formed on the rare rows with a survivor.
                 auto group = [&](auto full_tag) {
                     constexpr bool FULL = decltype(full_tag)::value;
-                    const int base = rl + 4 * lane;
-                    float jv[NJARR][8];
 #pragma unroll
-                    for (int r = 0; r < NJARR; ++r) {
-                        const float4 lo4 = *reinterpret_cast<const float4 *>(&jarr[r * JW + base]);
-                        const float4 hi4 = *reinterpret_cast<const float4 *>(&jarr[r * JW + base + 4]);
-                        jv[r][0] = lo4.x;
-                        jv[r][1] = lo4.y;
-                        jv[r][2] = lo4.z;
-                        jv[r][3] = lo4.w;
-                        jv[r][4] = hi4.x;
-                        jv[r][5] = hi4.y;
-                        jv[r][6] = hi4.z;
-                        jv[r][7] = hi4.w;
-                    }
-                    float rv[NJARR][4];  // row data of the 4 rows (same address on every lane)
+                    for (int h = 0; h < 2; ++h) {
+                        const int r0 = rl + 4 * h;
+                        if (!FULL && r0 >= nrows) break;
+                        float rv[NJARR][4];  // row data of the 4 rows (same address on every lane)
+                        float nv[NJARR][4];  // columns entering at lane 63: x = r0 + 256 + u
 #pragma unroll
-                    for (int r = 0; r < NJARR; ++r) {
-                        const float4 v4 = *reinterpret_cast<const float4 *>(&rarr[r * RC + rl]);
-                        rv[r][0] = v4.x;
-                        rv[r][1] = v4.y;
-                        rv[r][2] = v4.z;
-                        rv[r][3] = v4.w;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        if (!FULL && rl + u >= nrows) break;
-                        unsigned long long mk[KD];
-#pragma unroll
-                        for (int q = 0; q < KD; ++q) {
-                            const int e = u + q;
-                            // one lane mask per test, combined on the scalar unit
-                            mk[q] = __builtin_amdgcn_ballot_w64(XA[q] * jv[4][e] >= rv[5][u]) &
-                                    __builtin_amdgcn_ballot_w64(XB[q] * rv[4][u] >= jv[5][e]);
-                            if (!FULL) mk[q] &= __builtin_amdgcn_ballot_w64(rl + u <= limr[q]);
+                        for (int r = 0; r < NJARR; ++r) {
+                            const float4 v4 = *reinterpret_cast<const float4 *>(&rarr[r * RC + r0]);
+                            rv[r][0] = v4.x;
+                            rv[r][1] = v4.y;
+                            rv[r][2] = v4.z;
+                            rv[r][3] = v4.w;
+                            const float4 n4 = *reinterpret_cast<const float4 *>(&jarr[r * JW + r0 + DG]);
+                            nv[r][0] = n4.x;
+                            nv[r][1] = n4.y;
+                            nv[r][2] = n4.z;
+                            nv[r][3] = n4.w;
                         }
-                        if ((mk[0] | mk[1] | mk[2] | mk[3]) != 0ull) {
-                            const uint32_t rowcode = code0 + ((uint32_t)(rl + u) << 8);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            if (!FULL && r0 + u >= nrows) break;
+                            float t[KD];
 #pragma unroll
                             for (int q = 0; q < KD; ++q) {
-                                if (mk[q]) {
-                                    if ((mk[q] >> lane) & 1ull)
-                                        q1[q1n + (int)mbcnt64(mk[q])] = rowcode | (uint32_t)(KD * lane + q);
-                                    q1n += __popcll(mk[q]);
-                                    c_ball += __popcll(mk[q]);
+                                const float sA = fmaf(XA[q], cs[4][q], -rv[5][u]);
+                                const float sB = fmaf(XB[q], rv[4][u], -cs[5][q]);
+                                t[q] = fminf(sA, sB);
+                                if (!FULL && r0 + u > limr[q]) t[q] = -1.0f;
+                            }
+                            const float tmax = fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3]));
+                            if (__builtin_amdgcn_ballot_w64(tmax >= 0.0f) != 0ull) {
+                                const uint32_t rowcode = code0 + ((uint32_t)(r0 + u) << 8);
+#pragma unroll
+                                for (int q = 0; q < KD; ++q) {
+                                    const bool hit = t[q] >= 0.0f;
+                                    const unsigned long long mk = __builtin_amdgcn_ballot_w64(hit);
+                                    if (mk) {
+                                        if (hit) q1[q1n + (int)mbcnt64(mk)] = rowcode | (uint32_t)(KD * lane + q);
+                                        q1n += __popcll(mk);
+                                        count(C_BALL, __popcll(mk));
+                                    }
                                 }
                             }
-                        }
 #pragma unroll
-                        for (int q = 0; q < KD; ++q) {
-                            const int e = u + q;
-                            XA[q] = fmaf(jv[0][e], rv[3][u], XA[q]);
-                            XA[q] = fmaf(rv[2][u], jv[1][e], XA[q]);
-                            XB[q] = fmaf(rv[0][u], jv[3][e], XB[q]);
-                            XB[q] = fmaf(jv[2][e], rv[1][u], XB[q]);
+                            for (int q = 0; q < KD; ++q) {
+                                XA[q] = fmaf(cs[0][q], rv[3][u], XA[q]);
+                                XA[q] = fmaf(rv[2][u], cs[1][q], XA[q]);
+                                XB[q] = fmaf(rv[0][u], cs[3][q], XB[q]);
+                                XB[q] = fmaf(cs[2][q], rv[1][u], XB[q]);
+                            }
+#pragma unroll
+                            for (int r = 0; r < NJARR; ++r) {
+                                const float in = wave_shift_down1(nv[r][u], cs[r][0]);
+                                cs[r][0] = cs[r][1];
+                                cs[r][1] = cs[r][2];
+                                cs[r][2] = cs[r][3];
+                                cs[r][3] = in;
+                            }
                         }
                     }
                 };
-                for (; rl < nrows; rl += 4) {
-                    if (chunk_full && rl + 4 <= nrows)
+                for (; rl < nrows; rl += 8) {
+                    if (chunk_full && rl + 8 <= nrows)
                         group(std::integral_constant<bool, true>());
                     else
                         group(std::integral_constant<bool, false>());
                     if (q1n > QLIM) {
-                        rl += 4;
+                        rl += 8;
                         break;
                     }
                 }
@@ -525,12 +583,7 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
         }
     }
     if (lane == 0) {
-        atomicAdd(&A.counters[C_SWEPT], c_swept);
-        atomicAdd(&A.counters[C_BALL], c_ball);
-        atomicAdd(&A.counters[C_KIM], c_kim);
-        atomicAdd(&A.counters[C_KEOGH], c_keogh);
-        atomicAdd(&A.counters[C_DTW_AB], c_ab);
-        atomicAdd(&A.counters[C_CELLS], c_cells);
+        for (int c = 0; c < C_KEOGH_ELEMS; ++c) atomicAdd(&A.counters[c], wcnt[c]);
     }
     if (c_kelem) atomicAdd(&A.counters[C_KEOGH_ELEMS], c_kelem);
 }
@@ -631,7 +684,7 @@ size_t join_lds_bytes(int32_t m) {
 int launch_join(const JoinArgs &A, int32_t grid, hipStream_t s) {
     const int Q = dtw_q_for(A.w);
     const size_t lds = join_lds_bytes(A.m);
-    if (lds > 160 * 1024) {
+    if (lds + 64 > 160 * 1024) {  // + the kernel's static stage counters
         set_error("window too long for the join's LDS staging");
         return DTWJ_EINVAL;
     }
