@@ -8,7 +8,11 @@ the reference's rounding rule gives w = 3, available with --w 3).
 
 Prints one JSON line.  value = mean time-to-exact-top-1 in seconds (lower is better).
 roofline: the fused join kernel is FP32-VALU-bound; achieved = algorithmic FP32 operations
-of the launch / its HIP-event time (see DESIGN.md for the per-unit counts), peak 157.3 TFLOP/s.
+of the launch (pairs swept x 12, ball survivors x 84, LB_Keogh positions x 20, DTW cells x 5)
+/ its HIP-event time, peak 157.3 TFLOP/s.  traffic = HBM bytes per launch from the rocprofv3
+FETCH_SIZE / WRITE_SIZE summary under profiles/ for the same workload (null otherwise).
+A second workload (band ratio 0.05: seed 3, 5 plants, w = 13) is timed beside the headline one
+and reported under "wide_band".
 cpu_baseline: the repository's FP64 CPU restatement of the reference join (oracle/, the
 "port"), single-threaded, on a bounded prefix of the same series, extrapolated by pair count.
 """
@@ -30,6 +34,39 @@ OPS_KIM_PAIR = 84        # 12 z values (FMA) + 18 squared differences + 10 min/a
 OPS_KEOGH_ELEM = 20      # per position, both directions: 2 FMA envelope, 2 sub, 2 max, 1 FMA
 OPS_DTW_CELL = 5         # sub, mul, add, 2-way min of three
 PEAK_FP32 = 157.3e12     # MI355X FP32, dense
+
+
+def load_traffic(cfg):
+    """HBM bytes per k_join launch measured by rocprofv3 for this exact workload, or None."""
+    path = os.path.join(ROOT, "profiles", "k_join_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = t.get("workload", {})
+    if all(w.get(k) == cfg.get(k) for k in ("n", "m", "w", "seed", "planted_pairs")):
+        return t.get("bytes_per_launch")
+    return None
+
+
+def run_workload(d, n, m, w, seed, plants, steps, warmup, opts):
+    series, src, dst = d.generate(n, seed, n_plants=plants, plant_len=m)
+    ctx = d.Context()
+    ctx.upload(series)
+    for _ in range(warmup):
+        ctx.top1(m, w, **opts)
+    results, walls = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        r = ctx.top1(m, w, **opts)   # returns after the device work has completed
+        walls.append(time.perf_counter() - t0)
+        results.append(r)
+    ctx.close()
+    r = results[-1]
+    assert all((x.i, x.j) == (r.i, r.j) for x in results), "non-deterministic answer"
+    planted = d.dtw_pairs(series, m, w, src, dst, 64) if plants else np.array([np.inf])
+    return series, results, walls, planted
 
 
 def load_pkg():
@@ -77,6 +114,7 @@ def main():
     ap.add_argument("--plants", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-wide-band", action="store_true")
     ap.add_argument("--rows-per-item", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     args = ap.parse_args()
@@ -85,27 +123,18 @@ def main():
         sys.exit(2)
 
     d = load_pkg()
-    series, src, dst = d.generate(args.n, args.seed, n_plants=args.plants, plant_len=args.m)
-    ctx = d.Context()
-    ctx.upload(series)
     opts = dict(rows_per_item=args.rows_per_item, waves_per_cu=args.waves_per_cu)
-    for _ in range(args.warmup):
-        ctx.top1(args.m, args.w, **opts)
-    results, walls = [], []
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        r = ctx.top1(args.m, args.w, **opts)   # returns after the device work has completed
-        walls.append(time.perf_counter() - t0)
-        results.append(r)
-    ctx.close()
+    series, results, walls, planted = run_workload(d, args.n, args.m, args.w, args.seed, args.plants,
+                                                   args.steps, args.warmup, opts)
     r = results[-1]
-    assert all((x.i, x.j) == (r.i, r.j) for x in results), "non-deterministic answer"
     sec = float(np.mean(walls))
     kern_ms = float(np.mean([x.join_kernel_ms for x in results]))
     ops = (r.pairs_swept * OPS_SWEPT_PAIR + r.pass_ball * OPS_KIM_PAIR + r.keogh_elems *
            OPS_KEOGH_ELEM + r.dtw_cells * OPS_DTW_CELL)
     achieved = ops / (kern_ms * 1e-3)
-    planted = d.dtw_pairs(series, args.m, args.w, src, dst, 64) if args.plants else np.array([np.inf])
+    config = {"workload": "top1_dtw_join_plain", "n": args.n, "m": args.m, "w": args.w, "seed": args.seed,
+              "planted_pairs": args.plants, "parallelism": "single GPU, persistent grid"}
+    traffic = load_traffic(config)
     line = {
         "metric": "time_to_exact_top1_s",
         "value": sec,
@@ -117,10 +146,10 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (cells and bounds; FP64 statistics and re-verification)",
+        "dtype": "f32",
+        "precision": "FP32 bounds and DTW cells; FP64 statistics and FP64 re-verification of the winner",
         "data": "synthetic: seeded FP64 random walk with planted time-warped motif pairs (package generator)",
-        "config": {"workload": "top1_dtw_join_plain", "n": args.n, "m": args.m, "w": args.w, "seed": args.seed,
-                   "planted_pairs": args.plants, "parallelism": "single GPU, persistent grid"},
+        "config": config,
         "top1": {"i": r.i, "j": r.j, "dist": r.dist, "dist_fp32": r.dist_fp32,
                  "fp32_rel_err": abs(r.dist_fp32 - r.dist) / max(r.dist, 1e-300),
                  "best_planted_dist": float(np.sqrt(planted.min()))},
@@ -133,9 +162,24 @@ def main():
         "seconds": {"prep": r.t_prep, "seed": r.t_seed, "join": r.t_join, "verify": r.t_verify,
                     "join_kernel": kern_ms * 1e-3},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": None, "kernel": "k_join",
+                     "frac": achieved / PEAK_FP32, "traffic": traffic, "kernel": "k_join",
                      "kernel_ms": kern_ms},
     }
+    if not args.no_wide_band:
+        _, res5, walls5, planted5 = run_workload(d, args.n, args.m, 13, 3, 5, args.steps, args.warmup, opts)
+        r5 = res5[-1]
+        k5 = float(np.mean([x.join_kernel_ms for x in res5]))
+        line["wide_band"] = {
+            "config": {"workload": "top1_dtw_join_plain", "n": args.n, "m": args.m, "w": 13, "seed": 3,
+                       "planted_pairs": 5},
+            "time_to_exact_top1_s": float(np.mean(walls5)), "join_kernel_s": k5 * 1e-3,
+            "top1": {"i": r5.i, "j": r5.j, "dist": r5.dist, "dist_fp32": r5.dist_fp32,
+                     "fp32_rel_err": abs(r5.dist_fp32 - r5.dist) / max(r5.dist, 1e-300),
+                     "best_planted_dist": float(np.sqrt(planted5.min()))},
+            "dtw_cells": r5.dtw_cells, "dtw_cells_per_s": r5.dtw_cells / (k5 * 1e-3),
+            "pass_rates": {"ball": r5.pass_ball / r5.pairs_swept, "kim": r5.pass_kim / r5.pairs_swept,
+                           "keogh_to_dtw": r5.pass_keogh / r5.pairs_swept},
+        }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(series, args.m, args.w, args.cpu_budget)
     print(json.dumps(line), flush=True)

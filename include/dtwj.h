@@ -33,7 +33,7 @@ extern "C" {
 
 /* ---- status codes ---------------------------------------------------------------- */
 #define DTWJ_OK 0
-#define DTWJ_EINVAL 1    /* bad argument: m < 4, w < 0, n < 2m, NaN/Inf, constant window */
+#define DTWJ_EINVAL 1    /* bad argument: m < 8, w < 0, n < 2m, NaN/Inf, constant window */
 #define DTWJ_ENOMEM 2    /* host or device allocation failed */
 #define DTWJ_EDEVICE 3   /* no usable HIP device, or a HIP call failed */
 #define DTWJ_EIO 4       /* file could not be opened / parsed / written */
