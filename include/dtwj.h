@@ -46,7 +46,11 @@ typedef struct dtwj_options {
     int32_t rows_per_item; /* rows per sweep work item, multiple of 128 (default 0 = automatic) */
     int32_t waves_per_cu;  /* persistent waves per CU (default 0 = automatic) */
     int32_t verbose;       /* print per-stage counters to stderr */
-    int32_t reserved[7];
+    int32_t graph;         /* batch: capture prep + seed + join in one HIP graph (default 0 = on,
+                              -1 = launch kernels directly) */
+    int32_t autotune;      /* batch: 1 = pick rows_per_item / waves_per_cu by timing probe joins
+                              on a sample of the batch (0 = use the fields above or defaults) */
+    int32_t reserved[5];
 } dtwj_options;
 
 typedef struct dtwj_result {
@@ -72,7 +76,9 @@ typedef struct dtwj_result {
     double t_verify;     /* FP64 re-verification */
     double join_kernel_ms; /* GPU time of the fused join kernel (HIP events) */
     int32_t items;       /* sweep work items */
-    int32_t reserved[7];
+    int32_t rows_per_item; /* sweep tile height used */
+    int32_t waves_per_cu;  /* persistent waves per CU used */
+    int32_t reserved[5];
 } dtwj_result;
 
 /* Fills *opt with defaults.  No reference counterpart (the reference hard-codes
@@ -100,6 +106,24 @@ int dtwj_top1_seeded(const double *series, int64_t n, int32_t m, int32_t w, cons
                      const int32_t *mp_idx_1based, int64_t n_mp, const dtwj_options *opt,
                      dtwj_result *out);
 
+/* One series of a batch (host memory, read during the call). */
+typedef struct dtwj_series {
+    const double *data;
+    int64_t n;
+    int32_t m;  /* window length */
+    int32_t w;  /* band radius */
+} dtwj_series;
+
+/* Batched form: independent top-1 joins of `count` series of uneven lengths with mixed (m, w),
+ * run as one persistent join launch fed from one device work queue (all series' work items
+ * in one list), with the preprocessing, seeding and the join captured in one HIP graph.
+ * out[k] / status[k] receive series k's result and status code; the call returns DTWJ_OK
+ * when the batch ran (individual series may still have failed: see status), or an error that
+ * concerns the whole batch.  No reference counterpart: the reference joins one series per
+ * process (GPU_version1/src/AAAA.cu:17-36). */
+int dtwj_top1_batch(const dtwj_series *series, int32_t count, const dtwj_options *opt, dtwj_result *out,
+                    int32_t *status);
+
 /* Reusable device context: keeps device buffers between calls (for repeated joins). */
 typedef struct dtwj_context dtwj_context;
 int dtwj_context_create(int32_t device, dtwj_context **ctx);
@@ -111,6 +135,10 @@ int dtwj_top1_device(dtwj_context *ctx, const double *d_series, int64_t n, int32
 /* Copies n doubles to a device buffer owned by the context (valid until the next call of
  * this function or the context's destruction) and returns its device address. */
 int dtwj_context_upload(dtwj_context *ctx, const double *series, int64_t n, const double **d_series);
+/* dtwj_top1_batch on a context: device memory and the autotuner's choices are kept between
+ * calls (no reference counterpart). */
+int dtwj_context_top1_batch(dtwj_context *ctx, const dtwj_series *series, int32_t count, const dtwj_options *opt,
+                            dtwj_result *out, int32_t *status);
 
 /* Whole pipeline of the reference CLI for one (m, w): DTWMotifDiscoveryGUI(TS, m, w, file)
  * (GPU_version1/src/DTWMotifDiscoveryGUI.cu:12-28): runs the join, prints the reference's

@@ -36,7 +36,8 @@ class DTWJError(RuntimeError):
 
 class Options(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("seed_items", ctypes.c_int32), ("rows_per_item", ctypes.c_int32),
-                ("waves_per_cu", ctypes.c_int32), ("verbose", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+                ("waves_per_cu", ctypes.c_int32), ("verbose", ctypes.c_int32), ("graph", ctypes.c_int32),
+                ("autotune", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
 
 
 class Result(ctypes.Structure):
@@ -47,7 +48,13 @@ class Result(ctypes.Structure):
                 ("dtw_cells", ctypes.c_int64), ("keogh_elems", ctypes.c_int64), ("candidates", ctypes.c_int64),
                 ("t_total", ctypes.c_double), ("t_prep", ctypes.c_double), ("t_seed", ctypes.c_double),
                 ("t_join", ctypes.c_double), ("t_verify", ctypes.c_double), ("join_kernel_ms", ctypes.c_double),
-                ("items", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+                ("items", ctypes.c_int32), ("rows_per_item", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 5)]
+
+
+class Series(ctypes.Structure):
+    _fields_ = [("data", ctypes.POINTER(ctypes.c_double)), ("n", ctypes.c_int64), ("m", ctypes.c_int32),
+                ("w", ctypes.c_int32)]
 
 
 class GenSpec(ctypes.Structure):
@@ -62,7 +69,8 @@ EXPORTED = [
     "dtwj_default_options", "dtwj_band_from_ratio", "dtwj_top1", "dtwj_top1_seeded", "dtwj_context_create",
     "dtwj_context_destroy", "dtwj_top1_device", "dtwj_context_upload", "dtwj_motif_discovery", "dtwj_load_text",
     "dtwj_free", "dtwj_last_error", "dtwj_device_count", "dtwj_sliding_stats", "dtwj_envelope", "dtwj_dtw_pairs",
-    "dtwj_gen_spec_default", "dtwj_generate",
+    "dtwj_gen_spec_default", "dtwj_generate", "dtwj_top1_batch",
+    "dtwj_context_top1_batch",
 ]
 
 _lib = None
@@ -99,6 +107,8 @@ def lib():
     L.dtwj_envelope.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, d_p, d_p]
     L.dtwj_dtw_pairs.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, i64_p, i64_p, ctypes.c_int64,
                                  ctypes.c_int32, d_p]
+    L.dtwj_top1_batch.argtypes = [P(Series), ctypes.c_int32, P(Options), P(Result), i32_p]
+    L.dtwj_context_top1_batch.argtypes = [ctypes.c_void_p, P(Series), ctypes.c_int32, P(Options), P(Result), i32_p]
     L.dtwj_gen_spec_default.argtypes = [P(GenSpec)]
     L.dtwj_generate.argtypes = [P(GenSpec), d_p, i64_p, i64_p]
     _lib = L
@@ -145,6 +155,8 @@ class Top1:
     t_verify: float
     join_kernel_ms: float
     items: int
+    rows_per_item: int = 0
+    waves_per_cu: int = 0
     extra: dict = field(default_factory=dict)
 
     @property
@@ -161,11 +173,12 @@ def _result(r):
     return Top1(**{k: getattr(r, k) for k, _ in Result._fields_ if k != "reserved"})
 
 
-def make_options(device=0, seed_items=0, rows_per_item=0, waves_per_cu=0, verbose=False):
+def make_options(device=0, seed_items=0, rows_per_item=0, waves_per_cu=0, verbose=False, graph=0, autotune=0):
     o = Options()
     lib().dtwj_default_options(ctypes.byref(o))
     o.device, o.seed_items, o.rows_per_item, o.waves_per_cu, o.verbose = (
         device, seed_items, rows_per_item, waves_per_cu, int(bool(verbose)))
+    o.graph, o.autotune = int(graph), int(autotune)
     return o
 
 
@@ -184,6 +197,40 @@ def top1(series, m, w, **opts):
     r = Result()
     _check(lib().dtwj_top1(_dptr(a), a.size, int(m), int(w), ctypes.byref(make_options(**opts)), ctypes.byref(r)))
     return _result(r)
+
+
+def _batch_args(series_list, ms, ws):
+    arrs = [_as_series(x) for x in series_list]
+    k = len(arrs)
+    if k == 0 or len(ms) != k or len(ws) != k:
+        raise ValueError("need one m and one w per series")
+    items = (Series * k)()
+    for t, a in enumerate(arrs):
+        items[t].data = _dptr(a)
+        items[t].n = a.size
+        items[t].m = int(ms[t])
+        items[t].w = int(ws[t])
+    return arrs, items, (Result * k)(), (ctypes.c_int32 * k)()
+
+
+def _batch_results(out, status):
+    res = []
+    for t in range(len(status)):
+        if status[t] == DTWJ_OK:
+            res.append(_result(out[t]))
+        else:
+            res.append(DTWJError(int(status[t]), "series %d rejected" % t))
+    return res
+
+
+def top1_batch(series_list, ms, ws, **opts):
+    """Batched form: independent exact top-1 joins of many series (uneven lengths, mixed
+    (m, w)) in one persistent join launch fed from one device work queue, captured with the
+    preprocessing and seeding in one HIP graph.  Returns a list with, per series, a Top1 or the
+    DTWJError that series raised."""
+    arrs, items, out, status = _batch_args(series_list, ms, ws)
+    _check(lib().dtwj_top1_batch(items, len(arrs), ctypes.byref(make_options(**opts)), out, status))
+    return _batch_results(out, status)
 
 
 def top1_seeded(series, m, w, mp_ed, mp_idx_1based, **opts):
@@ -302,6 +349,13 @@ class Context:
         _check(lib().dtwj_top1_device(self._c, self._d, self.n, int(m), int(w), ctypes.byref(make_options(**opts)),
                                       ctypes.byref(r)))
         return _result(r)
+
+    def top1_batch(self, series_list, ms, ws, **opts):
+        """top1_batch on this context (keeps device memory and autotuner choices)."""
+        arrs, items, out, status = _batch_args(series_list, ms, ws)
+        _check(lib().dtwj_context_top1_batch(self._c, items, len(arrs), ctypes.byref(make_options(**opts)), out,
+                                             status))
+        return _batch_results(out, status)
 
     def close(self):
         if self._c:

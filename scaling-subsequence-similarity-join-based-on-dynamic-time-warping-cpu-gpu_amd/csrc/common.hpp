@@ -109,7 +109,8 @@ struct WinRow {
     float sig;       // sigma
 };
 
-// Everything the fused join kernel reads.  Device pointers unless noted.
+// One series of a join: everything the join kernels read.  Lives in device memory (an
+// array of them for a batch); device pointers unless noted.
 struct JoinArgs {
     // series and windows
     const double *a;       // n, FP64
@@ -122,6 +123,8 @@ struct JoinArgs {
     const float *lc;       // n, L - c0
     const float *mu_c;     // nw, mu - c0
     const float *blkmax;   // 3 x nblk: max sigma, max sv, max 1/sigma over 256-window blocks
+    const float *gstat;    // preprocessing statistics (value range, see Prep::gstat)
+    const int32_t *flags;  // preprocessing flags (non-finite input, constant window)
     int32_t nblk;
     // problem
     int64_t n, nw;
@@ -129,20 +132,19 @@ struct JoinArgs {
     // work decomposition
     const int64_t *item_start;  // G+1 cumulative item counts per diagonal group
     int32_t groups;
-    int64_t n_items;
     int32_t rows_per_item;
-    int32_t *work_counter;
+    int64_t n_items;
     // shared best-so-far and results
     unsigned long long *bsf;      // packed
     int4 *cands;                  // (i, j, cost bits, 0)
     int32_t *n_cands;
     int32_t cand_cap;
     unsigned long long *counters; // per-stage counters (see dtwj_api.cpp)
-    uint32_t *queue;              // grid x kJoinQueueCap: per-wave ball-survivor queues
-    // error model (host-computed)
+    // error model, filled on the device from gstat by the set-up kernel
     float amax;        // max |ac|, |uc|, |lc|
     float eps_z;       // bound on |z32 - z| for z computed from ac/mu_c/isig
     float rel_dtw;     // relative inflation of the best-so-far cost used as pruning threshold
+    int32_t valid;     // 0: the preprocessing flagged the series; the join skips it
 };
 
 // seed scan: per-item best ED pair

@@ -7,7 +7,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/dtwj.h"
@@ -32,7 +34,8 @@ namespace {
 using clk = std::chrono::steady_clock;
 inline double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
-constexpr int kCandCap = 1 << 20;
+constexpr int kCandCap = 1 << 20;       // candidate slots of a single-series join
+constexpr int kBatchCandCap = 1 << 14;  // ... per series of a batch
 constexpr int kDiagGroup = 256;  // diagonals per work item (join.hip DG)
 constexpr int kNCounters = 8;
 
@@ -54,38 +57,6 @@ struct Arena {
     }
 };
 
-size_t bytes_for(int64_t n, int64_t m, int64_t grid, int64_t groups, int64_t nseed) {
-    const int64_t nw = n - m + 1;
-    const int64_t nb = prep_scan_blocks(n);
-    const int64_t nblk = (nw + 255) / 256;
-    size_t b = 0;
-    auto add = [&](size_t bytes) { b += (bytes + 255) & ~(size_t)255; };
-    add(8 * n);           // a copy
-    add(8 * n * 3);       // U L v
-    add(8 * nw * 3);      // mu sig muv
-    add(8);               // c0
-    add(4 * n * 3);       // ac uc lc
-    add(4 * nw);          // mu_c
-    add(sizeof(WinRow) * nw);
-    add(4 * 3 * nblk);
-    add(4 * 8);
-    add(sizeof(dd_t) * 5 * (n + 1));
-    add(sizeof(dd_t) * 5 * nb);
-    add(8 * 2 * 1024);
-    add(4 * 4);
-    add(8);                          // bsf
-    add(4);                          // work counter
-    add(8 * (groups + 1));           // item_start
-    add(16 * (size_t)kCandCap);      // cands
-    add(4);                          // n_cands
-    add(8 * kNCounters);             // counters
-    add(8 * (nseed + 128));          // seed pairs
-    add(8 * 2 * (size_t)kCandCap);   // verify pair lists
-    add(8 * (size_t)kCandCap);       // verify costs
-    add(4 * (size_t)grid * kJoinQueueCap);  // per-wave survivor queues
-    return b + 4096;
-}
-
 }  // namespace
 
 }  // namespace dtwj
@@ -101,6 +72,8 @@ struct dtwj_context {
     double *upload = nullptr;
     int64_t upload_cap = 0;
     int num_cus = 0;
+    // batches: tile choices of the autotuner, keyed by the batch's shape
+    std::map<std::string, std::pair<int32_t, int32_t>> tuned;
 };
 
 extern "C" {
@@ -191,54 +164,93 @@ int validate(const double *series, int64_t n, int32_t m, int32_t w) {
     return DTWJ_OK;
 }
 
-struct Seeds {
-    std::vector<int2> pairs;  // explicit seed pairs (mp-seeded entry point)
-    bool gpu_scan = true;
+// ---------------------------------------------------------------- one series of a join
+// Sizes, device slices and the host copy of the join descriptor of one series.
+struct Job {
+    const double *d_a = nullptr;
+    int64_t n = 0, nw = 0, K = 0;
+    int32_t m = 0, w = 0;  // w already clamped to m - 1
+    int32_t groups = 0, rows_per_item = 0, n_seed = 0, cand_cap = 0;
+    int64_t n_items = 0;
+    std::vector<int64_t> item_start;
+    std::vector<int2> extra;  // explicit seed pairs (besides the scan's and (0, m))
+    Prep P;
+    JoinArgs A;               // host copy: pointers and sizes (error model filled on the device)
+    unsigned long long *d_bsf = nullptr, *d_counters = nullptr;
+    int4 *d_cands = nullptr;
+    int32_t *d_ncands = nullptr;
+    int64_t *d_item_start = nullptr;
+    int2 *d_seed_pairs = nullptr;
+    int32_t n_seed_pairs = 0;
 };
 
-int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w_in, const dtwj_options *opt,
-             const Seeds &seeds, dtwj_result *out, clk::time_point t_entry) {
-    const int32_t w = std::min(w_in, m - 1);
-    const int64_t nw = n - m + 1;
-    const int64_t K = n - 2 * (int64_t)m + 1;  // diagonal offsets m .. n-m
-    const int32_t groups = (int32_t)((K + kDiagGroup - 1) / kDiagGroup);
-    int32_t rows_per_item = (opt && opt->rows_per_item > 0) ? opt->rows_per_item : 2048;
-    rows_per_item = std::max(128, (rows_per_item + 127) / 128 * 128);
-    std::vector<int64_t> item_start(groups + 1, 0);
-    for (int32_t g = 0; g < groups; ++g) {
-        const int64_t rows_g = nw - ((int64_t)m + (int64_t)kDiagGroup * g);
-        item_start[g + 1] = item_start[g] + (rows_g + rows_per_item - 1) / rows_per_item;
-    }
-    const int64_t n_items = item_start[groups];
-    int32_t n_seed = (opt && opt->seed_items > 0) ? opt->seed_items : 4096;
-    if (!seeds.gpu_scan) n_seed = 0;
-    if (n_seed > n_items) n_seed = (int32_t)n_items;
-    const int64_t nseed_pairs = n_seed + (int64_t)seeds.pairs.size() + 1;
-    // persistent grid: one 64-lane wave per workgroup
-    const int wpc = (opt && opt->waves_per_cu > 0) ? opt->waves_per_cu : 8;
-    int64_t grid = (int64_t)c->num_cus * wpc;
-    if (grid > n_items) grid = n_items;
-    if (grid < 1) grid = 1;
+// Sweep tile height: a multiple of 128 rows, at most 2^20 (queue entries carry the row
+// offset inside the item in 22 bits).
+int32_t normalise_rows(int32_t r) {
+    r = r > 0 ? std::min(r, 1 << 20) : 2048;
+    return std::max(128, (r + 127) / 128 * 128);
+}
 
-    // ---- workspace
-    const size_t need = bytes_for(n, m, grid, groups, nseed_pairs);
-    if (need > c->mem_cap) {
-        if (c->mem) DTWJ_HIP(hipFree(c->mem));
-        c->mem = nullptr;
-        c->mem_cap = 0;
-        hipError_t e = hipMalloc(&c->mem, need);
-        if (e != hipSuccess) return hip_fail(e, "hipMalloc(workspace)", __FILE__, __LINE__);
-        c->mem_cap = need;
+void plan_job(Job &J, const double *d_a, int64_t n, int32_t m, int32_t w_in, int32_t rows_per_item, int32_t seed_req,
+              bool gpu_scan, const std::vector<int2> &extra, int32_t cand_cap) {
+    J.d_a = d_a;
+    J.n = n;
+    J.m = m;
+    J.w = std::min(w_in, m - 1);
+    J.nw = n - m + 1;
+    J.K = n - 2 * (int64_t)m + 1;  // diagonal offsets m .. n-m
+    J.groups = (int32_t)((J.K + kDiagGroup - 1) / kDiagGroup);
+    J.rows_per_item = normalise_rows(rows_per_item);
+    J.item_start.assign(J.groups + 1, 0);
+    for (int32_t g = 0; g < J.groups; ++g) {
+        const int64_t rows_g = J.nw - ((int64_t)m + (int64_t)kDiagGroup * g);
+        J.item_start[g + 1] = J.item_start[g] + (rows_g + J.rows_per_item - 1) / J.rows_per_item;
     }
-    Arena ar;
-    ar.base = (char *)c->mem;
-    ar.cap = c->mem_cap;
-    Prep P;
-    ar.take<double>(n);  // (slot for a host-input copy; d_a is provided by the caller)
-    P.a = d_a;
+    J.n_items = J.item_start[J.groups];
+    int32_t ns = seed_req > 0 ? seed_req : 4096;
+    if (!gpu_scan) ns = 0;
+    if (ns > J.n_items) ns = (int32_t)J.n_items;
+    J.n_seed = ns;
+    J.extra = extra;
+    J.n_seed_pairs = ns + 1 + (int32_t)extra.size();
+    J.cand_cap = cand_cap;
+}
+
+// Device bytes of one series (statistics, envelopes, tables, scans, join state).
+size_t job_bytes(const Job &J) {
+    const int64_t n = J.n, nw = J.nw;
+    const int64_t nb = prep_scan_blocks(n);
+    const int64_t nblk = (nw + 255) / 256;
+    size_t b = 0;
+    auto add = [&](size_t bytes) { b += (bytes + 255) & ~(size_t)255; };
+    add(8 * n * 3);       // U L v
+    add(8 * nw * 3);      // mu sig muv
+    add(8);               // c0
+    add(4 * n * 3);       // ac uc lc
+    add(4 * nw);          // mu_c
+    add(sizeof(WinRow) * nw);
+    add(4 * 3 * nblk);
+    add(4 * 8);           // gstat
+    add(sizeof(dd_t) * 5 * (n + 1));
+    add(sizeof(dd_t) * 5 * nb);
+    add(8 * 2 * 1024);
+    add(4 * 4);           // flags
+    add(8);               // bsf
+    add(8 * (J.groups + 1));
+    add(16 * (size_t)J.cand_cap);
+    add(4);               // n_cands
+    add(8 * kNCounters);
+    add(8 * ((size_t)J.n_seed_pairs + 1));
+    return b;
+}
+
+void take_job(Arena &ar, Job &J) {
+    Prep &P = J.P;
+    const int64_t n = J.n, nw = J.nw;
+    P.a = J.d_a;
     P.n = n;
-    P.m = m;
-    P.w = w;
+    P.m = J.m;
+    P.w = J.w;
     P.U = ar.take<double>(n);
     P.L = ar.take<double>(n);
     P.v = ar.take<double>(n);
@@ -258,35 +270,16 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
     P.bsum = ar.take<dd_t>(5 * (size_t)prep_scan_blocks(n));
     P.red = ar.take<double>(2 * 1024);
     P.flags = ar.take<int32_t>(4);
-    unsigned long long *d_bsf = ar.take<unsigned long long>(1);
-    int32_t *d_work = ar.take<int32_t>(1);
-    int64_t *d_item_start = ar.take<int64_t>(groups + 1);
-    int4 *d_cands = ar.take<int4>(kCandCap);
-    int32_t *d_ncands = ar.take<int32_t>(1);
-    unsigned long long *d_counters = ar.take<unsigned long long>(kNCounters);
-    int2 *d_seed_pairs = ar.take<int2>(nseed_pairs + 128);
-    int64_t *d_vi = ar.take<int64_t>(kCandCap);
-    int64_t *d_vj = ar.take<int64_t>(kCandCap);
-    double *d_vcost = ar.take<double>(kCandCap);
-    uint32_t *d_queue = ar.take<uint32_t>((size_t)grid * kJoinQueueCap);
+    J.d_bsf = ar.take<unsigned long long>(1);
+    J.d_item_start = ar.take<int64_t>(J.groups + 1);
+    J.d_cands = ar.take<int4>(J.cand_cap);
+    J.d_ncands = ar.take<int32_t>(1);
+    J.d_counters = ar.take<unsigned long long>(kNCounters);
+    J.d_seed_pairs = ar.take<int2>((size_t)J.n_seed_pairs + 1);
 
-    hipStream_t s = c->stream;
-    const auto t0 = clk::now();
-    // ---- preprocessing
-    if (int rc = prep_series(P, s)) return rc;
-    int32_t flags[4];
-    float gstat[8];
-    DTWJ_HIP(hipMemcpyAsync(flags, P.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
-    DTWJ_HIP(hipMemcpyAsync(gstat, P.gstat, sizeof(gstat), hipMemcpyDeviceToHost, s));
-    DTWJ_HIP(hipStreamSynchronize(s));
-    if (flags[0]) return fail(DTWJ_EINVAL, "series contains NaN or Inf");
-    if (flags[1]) return fail(DTWJ_EINVAL, "a window of length m is constant (standard deviation 0)");
-    const auto t1 = clk::now();
-
-    // ---- error model: |z32 - z| <= eps_z for every z value the FP32 stages compute
-    JoinArgs A;
+    JoinArgs &A = J.A;
     std::memset(&A, 0, sizeof(A));
-    A.a = d_a;
+    A.a = J.d_a;
     A.v = P.v;
     A.mu = P.mu;
     A.muv = P.muv;
@@ -296,87 +289,77 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
     A.lc = P.lc;
     A.mu_c = P.mu_c;
     A.blkmax = P.blkmax;
+    A.gstat = P.gstat;
+    A.flags = P.flags;
     A.nblk = P.nblk;
     A.n = n;
     A.nw = nw;
-    A.m = m;
-    A.w = w;
-    A.item_start = d_item_start;
-    A.groups = groups;
-    A.n_items = n_items;
-    A.rows_per_item = rows_per_item;
-    A.work_counter = d_work;
-    A.bsf = d_bsf;
-    A.cands = d_cands;
-    A.n_cands = d_ncands;
-    A.cand_cap = kCandCap;
-    A.counters = d_counters;
-    A.queue = d_queue;
-    A.amax = gstat[0];
-    A.eps_z = (float)(std::ldexp(1.0, -22) * (((double)gstat[0] + (double)gstat[1]) * (double)gstat[2] +
-                                             std::sqrt((double)m)));
-    A.rel_dtw = (float)(std::ldexp(1.0, -10) + 4.0 * m * std::ldexp(1.0, -24));
+    A.m = J.m;
+    A.w = J.w;
+    A.item_start = J.d_item_start;
+    A.groups = J.groups;
+    A.rows_per_item = J.rows_per_item;
+    A.n_items = J.n_items;
+    A.bsf = J.d_bsf;
+    A.cands = J.d_cands;
+    A.n_cands = J.d_ncands;
+    A.cand_cap = J.cand_cap;
+    A.counters = J.d_counters;
+}
 
-    // ---- reset shared state
-    const unsigned long long init_key = pack_bsf(3.0e38f, 0xffffffffu);
-    DTWJ_HIP(hipMemcpyAsync(d_bsf, &init_key, sizeof(init_key), hipMemcpyHostToDevice, s));
-    DTWJ_HIP(hipMemsetAsync(d_work, 0, sizeof(int32_t), s));
-    DTWJ_HIP(hipMemsetAsync(d_ncands, 0, sizeof(int32_t), s));
-    DTWJ_HIP(hipMemsetAsync(d_counters, 0, sizeof(unsigned long long) * kNCounters, s));
-    DTWJ_HIP(hipMemcpyAsync(d_item_start, item_start.data(), sizeof(int64_t) * (groups + 1), hipMemcpyHostToDevice, s));
-
-    // ---- seed: ED scan over a sample of items, explicit seeds, and the pair (0, m)
-    std::vector<int2> extra = seeds.pairs;
-    extra.push_back(make_int2(0, m));
-    if (n_seed > 0) {
-        SeedArgs S;
-        S.a = d_a;
-        S.mu = P.mu;
-        S.win = P.win;
-        S.n = n;
-        S.nw = nw;
-        S.m = m;
-        S.item_start = d_item_start;
-        S.groups = groups;
-        S.n_items = n_items;
-        S.rows_per_item = rows_per_item;
-        S.rows = std::min(rows_per_item, 512);
-        S.n_seed = n_seed;
-        if (int rc = launch_seed(S, d_seed_pairs, s)) return rc;
+// Host-to-device copies a job needs before its kernels run (item offsets, explicit seeds).
+int upload_job(Job &J, hipStream_t s) {
+    DTWJ_HIP(hipMemcpyAsync(J.d_item_start, J.item_start.data(), sizeof(int64_t) * (J.groups + 1),
+                            hipMemcpyHostToDevice, s));
+    std::vector<int2> fixed = J.extra;
+    if (J.n_seed == 0) fixed.insert(fixed.begin(), make_int2(0, J.m));  // else written by the scan
+    if (!fixed.empty()) {
+        const int32_t off = J.n_seed == 0 ? 0 : J.n_seed + 1;
+        DTWJ_HIP(hipMemcpyAsync(J.d_seed_pairs + off, fixed.data(), sizeof(int2) * fixed.size(),
+                                hipMemcpyHostToDevice, s));
     }
-    DTWJ_HIP(hipMemcpyAsync(d_seed_pairs + n_seed, extra.data(), sizeof(int2) * extra.size(), hipMemcpyHostToDevice,
-                            s));
-    if (int rc = launch_seed_dtw(A, d_seed_pairs, (int32_t)(n_seed + extra.size()), s)) return rc;
-    unsigned long long seed_key = 0;
-    DTWJ_HIP(hipMemcpyAsync(&seed_key, d_bsf, sizeof(seed_key), hipMemcpyDeviceToHost, s));
-    DTWJ_HIP(hipStreamSynchronize(s));
-    const auto t2 = clk::now();
+    return 0;
+}
 
-    // ---- fused join
-    DTWJ_HIP(hipEventRecord(c->ev0, s));
-    if (int rc = launch_join(A, (int32_t)grid, s)) return rc;
-    DTWJ_HIP(hipEventRecord(c->ev1, s));
+// Seed scan (ED over sampled items) + DTW of every seed pair.  d_args: the job's descriptor.
+int enqueue_seeds(Job &J, const JoinArgs *d_args, hipStream_t s) {
+    if (J.n_seed > 0) {
+        SeedArgs S;
+        S.a = J.d_a;
+        S.mu = J.P.mu;
+        S.win = J.P.win;
+        S.n = J.n;
+        S.nw = J.nw;
+        S.m = J.m;
+        S.item_start = J.d_item_start;
+        S.groups = J.groups;
+        S.n_items = J.n_items;
+        S.rows_per_item = J.rows_per_item;
+        S.rows = std::min(J.rows_per_item, 512);
+        S.n_seed = J.n_seed;
+        if (int rc = launch_seed(S, J.d_seed_pairs, s)) return rc;
+    }
+    return launch_seed_dtw(d_args, J.m, J.w, J.d_seed_pairs, J.n_seed_pairs, s);
+}
+
+// Reads the job's join state back, re-scores every candidate within the FP32 error of the
+// best in FP64 and fills *out.  d_vi/d_vj/d_vcost: scratch for cand_cap pairs.
+int finish_job(Job &J, int64_t *d_vi, int64_t *d_vj, double *d_vcost, hipStream_t s, dtwj_result *out) {
     int32_t ncands = 0;
     unsigned long long key = 0, counters[kNCounters];
-    DTWJ_HIP(hipMemcpyAsync(&ncands, d_ncands, sizeof(ncands), hipMemcpyDeviceToHost, s));
-    DTWJ_HIP(hipMemcpyAsync(&key, d_bsf, sizeof(key), hipMemcpyDeviceToHost, s));
-    DTWJ_HIP(hipMemcpyAsync(counters, d_counters, sizeof(counters), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipMemcpyAsync(&ncands, J.d_ncands, sizeof(ncands), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipMemcpyAsync(&key, J.d_bsf, sizeof(key), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipMemcpyAsync(counters, J.d_counters, sizeof(counters), hipMemcpyDeviceToHost, s));
     DTWJ_HIP(hipStreamSynchronize(s));
-    float join_ms = 0.f;
-    DTWJ_HIP(hipEventElapsedTime(&join_ms, c->ev0, c->ev1));
-    const auto t3 = clk::now();
-    if (ncands > kCandCap) return fail(DTWJ_EOVERFLOW, "candidate buffer overflow (too many near-ties)");
+    if (ncands > J.cand_cap) return fail(DTWJ_EOVERFLOW, "candidate buffer overflow (too many near-ties)");
     if (ncands <= 0) return fail(DTWJ_EDEVICE, "join produced no candidate");
-
-    // ---- FP64 re-verification of every candidate within the FP32 error of the best
     std::vector<int4> cands(ncands);
-    DTWJ_HIP(hipMemcpyAsync(cands.data(), d_cands, sizeof(int4) * ncands, hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipMemcpyAsync(cands.data(), J.d_cands, sizeof(int4) * ncands, hipMemcpyDeviceToHost, s));
     DTWJ_HIP(hipStreamSynchronize(s));
     const float best32 = bsf_cost(key);
-    auto err = [&](double D) {
-        const double e = A.eps_z;
-        return (double)A.rel_dtw * D + 4.0 * e * std::sqrt(2.0 * m * D) + 8.0 * m * e * e;
-    };
+    const double eps = J.A.eps_z, rel = J.A.rel_dtw;
+    const int m = J.m;
+    auto err = [&](double D) { return rel * D + 4.0 * eps * std::sqrt(2.0 * m * D) + 8.0 * m * eps * eps; };
     const double T2 = (double)best32 + err(best32);
     const double ub = T2 + err(T2);
     std::vector<int64_t> vi, vj;
@@ -412,24 +395,21 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
     std::vector<double> vcost(nv);
     DTWJ_HIP(hipMemcpyAsync(d_vi, vi.data(), sizeof(int64_t) * nv, hipMemcpyHostToDevice, s));
     DTWJ_HIP(hipMemcpyAsync(d_vj, vj.data(), sizeof(int64_t) * nv, hipMemcpyHostToDevice, s));
-    if (int rc = launch_dtw_pairs(A, P.sig, d_vi, d_vj, nv, 64, d_vcost, s)) return rc;
+    if (int rc = launch_dtw_pairs(J.A, J.P.sig, d_vi, d_vj, nv, 64, d_vcost, s)) return rc;
     DTWJ_HIP(hipMemcpyAsync(vcost.data(), d_vcost, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
     DTWJ_HIP(hipStreamSynchronize(s));
     int64_t best = -1;
     for (int64_t k = 0; k < nv; ++k) {
-        if (best < 0 || vcost[k] < vcost[best] || (vcost[k] == vcost[best] && (vi[k] < vi[best] ||
-                                                                                (vi[k] == vi[best] && vj[k] < vj[best]))))
+        if (best < 0 || vcost[k] < vcost[best] ||
+            (vcost[k] == vcost[best] && (vi[k] < vi[best] || (vi[k] == vi[best] && vj[k] < vj[best]))))
             best = k;
     }
-    const auto t4 = clk::now();
-
     std::memset(out, 0, sizeof(*out));
     out->i = vi[best];
     out->j = vj[best];
     out->dist = std::sqrt(vcost[best]);
     out->dist_fp32 = std::sqrt((double)v32[best]);
-    out->seed_dist = std::sqrt((double)bsf_cost(seed_key));
-    out->pairs_total = K * (K + 1) / 2;
+    out->pairs_total = J.K * (J.K + 1) / 2;
     out->pairs_swept = (int64_t)counters[0];
     out->pass_ball = (int64_t)counters[1];
     out->pass_kim = (int64_t)counters[2];
@@ -438,21 +418,312 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
     out->dtw_cells = (int64_t)counters[5];
     out->keogh_elems = (int64_t)counters[6];
     out->candidates = nv;
+    out->items = (int32_t)J.n_items;
+    out->rows_per_item = J.rows_per_item;
+    return DTWJ_OK;
+}
+
+// Copies the error model the set-up kernel computed into the host copy (used by finish_job).
+int fetch_error_model(Job &J, const JoinArgs *d_args, hipStream_t s) {
+    JoinArgs Ad;
+    DTWJ_HIP(hipMemcpyAsync(&Ad, d_args, sizeof(Ad), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    J.A.amax = Ad.amax;
+    J.A.eps_z = Ad.eps_z;
+    J.A.rel_dtw = Ad.rel_dtw;
+    J.A.valid = Ad.valid;
+    return 0;
+}
+
+int32_t grid_for(const dtwj_context *c, int32_t wpc, int64_t n_items) {
+    wpc = wpc > 0 ? wpc : 8;
+    int64_t grid = (int64_t)c->num_cus * wpc;
+    if (grid > n_items) grid = n_items;
+    return (int32_t)std::max<int64_t>(grid, 1);
+}
+
+int ensure_mem(dtwj_context *c, size_t need) {
+    if (need > c->mem_cap) {
+        if (c->mem) DTWJ_HIP(hipFree(c->mem));
+        c->mem = nullptr;
+        c->mem_cap = 0;
+        hipError_t e = hipMalloc(&c->mem, need);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(workspace)", __FILE__, __LINE__);
+        c->mem_cap = need;
+    }
+    return 0;
+}
+
+struct Seeds {
+    std::vector<int2> pairs;  // explicit seed pairs (mp-seeded entry point)
+    bool gpu_scan = true;
+};
+
+int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w_in, const dtwj_options *opt,
+             const Seeds &seeds, dtwj_result *out, clk::time_point t_entry) {
+    Job J;
+    plan_job(J, d_a, n, m, w_in, opt ? opt->rows_per_item : 0, opt ? opt->seed_items : 0, seeds.gpu_scan, seeds.pairs,
+             kCandCap);
+    const int32_t wpc = (opt && opt->waves_per_cu > 0) ? opt->waves_per_cu : 8;
+    const int32_t grid = grid_for(c, wpc, J.n_items);
+    JoinArgs *d_args = nullptr;
+    int64_t *d_item_base = nullptr, *d_vi = nullptr, *d_vj = nullptr;
+    int32_t *d_work = nullptr;
+    uint32_t *d_queue = nullptr;
+    double *d_vcost = nullptr;
+    auto carve = [&](Arena &ar) {
+        take_job(ar, J);
+        d_args = ar.take<JoinArgs>(1);
+        d_item_base = ar.take<int64_t>(2);
+        d_work = ar.take<int32_t>(1);
+        d_queue = ar.take<uint32_t>((size_t)grid * kJoinQueueCap);
+        d_vi = ar.take<int64_t>(kCandCap);
+        d_vj = ar.take<int64_t>(kCandCap);
+        d_vcost = ar.take<double>(kCandCap);
+    };
+    Arena dry;
+    carve(dry);
+    if (int rc = ensure_mem(c, dry.off + 256)) return rc;
+    Arena ar;
+    ar.base = (char *)c->mem;
+    ar.cap = c->mem_cap;
+    carve(ar);
+
+    hipStream_t s = c->stream;
+    const auto t0 = clk::now();
+    if (int rc = upload_job(J, s)) return rc;
+    const int64_t item_base[2] = {0, J.n_items};
+    DTWJ_HIP(hipMemcpyAsync(d_item_base, item_base, sizeof(item_base), hipMemcpyHostToDevice, s));
+    // ---- preprocessing
+    if (int rc = prep_series(J.P, s)) return rc;
+    int32_t flags[4];
+    DTWJ_HIP(hipMemcpyAsync(flags, J.P.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    if (flags[0]) return fail(DTWJ_EINVAL, "series contains NaN or Inf");
+    if (flags[1]) return fail(DTWJ_EINVAL, "a window of length m is constant (standard deviation 0)");
+    const auto t1 = clk::now();
+    // ---- join descriptor, error model, resets; seed
+    DTWJ_HIP(hipMemcpyAsync(d_args, &J.A, sizeof(JoinArgs), hipMemcpyHostToDevice, s));
+    if (int rc = launch_join_setup(d_args, 1, d_work, s)) return rc;
+    if (int rc = fetch_error_model(J, d_args, s)) return rc;
+    if (int rc = enqueue_seeds(J, d_args, s)) return rc;
+    unsigned long long seed_key = 0;
+    DTWJ_HIP(hipMemcpyAsync(&seed_key, J.d_bsf, sizeof(seed_key), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    const auto t2 = clk::now();
+    // ---- fused join
+    DTWJ_HIP(hipEventRecord(c->ev0, s));
+    if (int rc = launch_join(d_args, d_item_base, 1, J.m, J.w, d_work, d_queue, grid, s)) return rc;
+    DTWJ_HIP(hipEventRecord(c->ev1, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    float join_ms = 0.f;
+    DTWJ_HIP(hipEventElapsedTime(&join_ms, c->ev0, c->ev1));
+    const auto t3 = clk::now();
+    // ---- FP64 re-verification of every candidate within the FP32 error of the best
+    if (int rc = finish_job(J, d_vi, d_vj, d_vcost, s, out)) return rc;
+    const auto t4 = clk::now();
+    out->seed_dist = std::sqrt((double)bsf_cost(seed_key));
     out->t_prep = secs(t0, t1);
     out->t_seed = secs(t1, t2);
     out->t_join = secs(t2, t3);
     out->t_verify = secs(t3, t4);
     out->t_total = secs(t_entry, t4);
     out->join_kernel_ms = join_ms;
-    out->items = (int32_t)n_items;
+    out->waves_per_cu = wpc;
     if (opt && opt->verbose) {
         std::fprintf(stderr,
                      "[dtwj] n=%lld m=%d w=%d items=%lld swept=%lld ball=%lld kim=%lld keogh=%lld dtw_ab=%lld "
                      "cells=%lld cands=%lld seed=%.6f best=%.6f join=%.3f ms\n",
-                     (long long)n, m, w, (long long)n_items, (long long)out->pairs_swept, (long long)out->pass_ball,
-                     (long long)out->pass_kim, (long long)out->pass_keogh, (long long)out->dtw_abandoned,
-                     (long long)out->dtw_cells, (long long)nv, out->seed_dist, out->dist, join_ms);
+                     (long long)n, m, J.w, (long long)J.n_items, (long long)out->pairs_swept,
+                     (long long)out->pass_ball, (long long)out->pass_kim, (long long)out->pass_keogh,
+                     (long long)out->dtw_abandoned, (long long)out->dtw_cells, (long long)out->candidates,
+                     out->seed_dist, out->dist, join_ms);
     }
+    return DTWJ_OK;
+}
+
+// ---------------------------------------------------------------- batched form
+// Every series gets its own statistics, tables, best-so-far and candidate list; their work
+// items are concatenated into one list (item_base) that one persistent join launch drains.
+// Preprocessing, set-up, seeds and the join are enqueued on one stream, captured in one HIP
+// graph unless opt->graph < 0.  Per-series validation failures are reported in status[] and
+// the series is left out; *join_ms receives the device time of the captured work.
+int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const dtwj_options *opt, int32_t rows_req,
+              int32_t wpc, dtwj_result *out, int32_t *status, float *join_ms) {
+    std::vector<Job> jobs;
+    std::vector<int32_t> owner;  // job -> batch index
+    jobs.reserve(count);
+    for (int32_t k = 0; k < count; ++k) {
+        std::memset(&out[k], 0, sizeof(dtwj_result));
+        const dtwj_series &S = series[k];
+        status[k] = validate(S.data, S.n, S.m, S.w);
+        if (status[k] == DTWJ_OK) {
+            for (int64_t t = 0; t < S.n; ++t)
+                if (!std::isfinite(S.data[t])) {
+                    status[k] = fail(DTWJ_EINVAL, "series contains NaN or Inf");
+                    break;
+                }
+        }
+        if (status[k] != DTWJ_OK) continue;
+        jobs.emplace_back();
+        plan_job(jobs.back(), nullptr, S.n, S.m, S.w, rows_req, opt ? opt->seed_items : 0, true, {}, kBatchCandCap);
+        owner.push_back(k);
+    }
+    const int32_t nj = (int32_t)jobs.size();
+    if (nj == 0) return DTWJ_OK;
+    std::vector<int64_t> item_base(nj + 1, 0);
+    int32_t max_m = 0, max_w = 0;
+    for (int32_t t = 0; t < nj; ++t) {
+        item_base[t + 1] = item_base[t] + jobs[t].n_items;
+        max_m = std::max(max_m, jobs[t].m);
+        max_w = std::max(max_w, jobs[t].w);
+    }
+    const int32_t grid = grid_for(c, wpc, item_base[nj]);
+    std::vector<double *> d_data(nj);
+    JoinArgs *d_args = nullptr;
+    int64_t *d_item_base = nullptr, *d_vi = nullptr, *d_vj = nullptr;
+    int32_t *d_work = nullptr;
+    uint32_t *d_queue = nullptr;
+    double *d_vcost = nullptr;
+    auto carve = [&](Arena &ar) {
+        for (int32_t t = 0; t < nj; ++t) {
+            d_data[t] = ar.take<double>(jobs[t].n);
+            jobs[t].d_a = d_data[t];
+            take_job(ar, jobs[t]);
+        }
+        d_args = ar.take<JoinArgs>(nj);
+        d_item_base = ar.take<int64_t>(nj + 1);
+        d_work = ar.take<int32_t>(1);
+        d_queue = ar.take<uint32_t>((size_t)grid * kJoinQueueCap);
+        d_vi = ar.take<int64_t>(kBatchCandCap);
+        d_vj = ar.take<int64_t>(kBatchCandCap);
+        d_vcost = ar.take<double>(kBatchCandCap);
+    };
+    Arena dry;
+    carve(dry);
+    if (int rc = ensure_mem(c, dry.off + 256)) return rc;
+    Arena ar;
+    ar.base = (char *)c->mem;
+    ar.cap = c->mem_cap;
+    carve(ar);
+
+    hipStream_t s = c->stream;
+    std::vector<JoinArgs> hargs(nj);
+    for (int32_t t = 0; t < nj; ++t) {
+        const dtwj_series &S = series[owner[t]];
+        DTWJ_HIP(hipMemcpyAsync(d_data[t], S.data, sizeof(double) * S.n, hipMemcpyHostToDevice, s));
+        if (int rc = upload_job(jobs[t], s)) return rc;
+        hargs[t] = jobs[t].A;
+    }
+    DTWJ_HIP(hipMemcpyAsync(d_args, hargs.data(), sizeof(JoinArgs) * nj, hipMemcpyHostToDevice, s));
+    DTWJ_HIP(hipMemcpyAsync(d_item_base, item_base.data(), sizeof(int64_t) * (nj + 1), hipMemcpyHostToDevice, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+
+    auto enqueue_all = [&]() -> int {
+        for (int32_t t = 0; t < nj; ++t)
+            if (int rc = prep_series(jobs[t].P, s)) return rc;
+        if (int rc = launch_join_setup(d_args, nj, d_work, s)) return rc;
+        for (int32_t t = 0; t < nj; ++t)
+            if (int rc = enqueue_seeds(jobs[t], d_args + t, s)) return rc;
+        return launch_join(d_args, d_item_base, nj, max_m, max_w, d_work, d_queue, grid, s);
+    };
+    const bool use_graph = !(opt && opt->graph < 0);
+    DTWJ_HIP(hipEventRecord(c->ev0, s));
+    if (use_graph) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        DTWJ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_all();
+        hipError_t ec = hipStreamEndCapture(s, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (ec != hipSuccess) return hip_fail(ec, "hipStreamEndCapture", __FILE__, __LINE__);
+        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return hip_fail(ei, "hipGraphInstantiate", __FILE__, __LINE__);
+        }
+        DTWJ_HIP(hipEventRecord(c->ev0, s));
+        hipError_t el = hipGraphLaunch(exec, s);
+        DTWJ_HIP(hipEventRecord(c->ev1, s));
+        hipError_t es = hipStreamSynchronize(s);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        if (el != hipSuccess) return hip_fail(el, "hipGraphLaunch", __FILE__, __LINE__);
+        if (es != hipSuccess) return hip_fail(es, "batch graph", __FILE__, __LINE__);
+    } else {
+        if (int rc = enqueue_all()) return rc;
+        DTWJ_HIP(hipEventRecord(c->ev1, s));
+        DTWJ_HIP(hipStreamSynchronize(s));
+    }
+    float ms = 0.f;
+    DTWJ_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (join_ms) *join_ms = ms;
+    for (int32_t t = 0; t < nj; ++t) {
+        const int32_t k = owner[t];
+        int32_t flags[4];
+        DTWJ_HIP(hipMemcpyAsync(flags, jobs[t].P.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+        if (int rc = fetch_error_model(jobs[t], d_args + t, s)) return rc;
+        if (flags[0] || flags[1]) {
+            status[k] = fail(DTWJ_EINVAL, flags[0] ? "series contains NaN or Inf"
+                                                   : "a window of length m is constant (standard deviation 0)");
+            continue;
+        }
+        status[k] = finish_job(jobs[t], d_vi, d_vj, d_vcost, s, &out[k]);
+        out[k].join_kernel_ms = ms;
+        out[k].waves_per_cu = wpc;
+    }
+    return DTWJ_OK;
+}
+
+// Autotuner of the batched form: times the join on a sample of the batch (up to 16 series
+// spread over the batch, in its order) for each candidate tile height and occupancy, and
+// keeps the fastest.  The choice is cached in the context, keyed by the batch's (m, w) mix
+// and size, so repeated batches of the same shape do not pay for it again.
+int autotune_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const dtwj_options *opt,
+                   int32_t *rows_out, int32_t *wpc_out) {
+    std::vector<std::pair<int32_t, int32_t>> mw;
+    int64_t total = 0;
+    for (int32_t k = 0; k < count; ++k) {
+        mw.emplace_back(series[k].m, series[k].w);
+        total += series[k].n;
+    }
+    std::sort(mw.begin(), mw.end());
+    mw.erase(std::unique(mw.begin(), mw.end()), mw.end());
+    std::string key = std::to_string(count) + ":" + std::to_string(total >> 16);
+    for (auto &p : mw) key += ":" + std::to_string(p.first) + "/" + std::to_string(p.second);
+    auto it = c->tuned.find(key);
+    if (it != c->tuned.end()) {
+        *rows_out = it->second.first;
+        *wpc_out = it->second.second;
+        return DTWJ_OK;
+    }
+    std::vector<dtwj_series> sample;
+    const int32_t ns = std::min<int32_t>(count, 16);
+    for (int32_t t = 0; t < ns; ++t) sample.push_back(series[(int64_t)t * count / ns]);
+    const int32_t cand[][2] = {{1024, 8}, {2048, 8}, {4096, 8}, {2048, 6}, {2048, 4}};
+    std::vector<dtwj_result> res(ns);
+    std::vector<int32_t> st(ns);
+    dtwj_options o = opt ? *opt : dtwj_options{};
+    o.graph = -1;
+    float best = 0.f;
+    int32_t br = 2048, bw = 8;
+    bool have = false;
+    for (const auto &cf : cand) {
+        float ms = 0.f;
+        if (int rc = run_batch(c, sample.data(), ns, &o, cf[0], cf[1], res.data(), st.data(), &ms)) return rc;
+        if (!have || ms < best) {
+            best = ms;
+            br = cf[0];
+            bw = cf[1];
+            have = true;
+        }
+    }
+    c->tuned[key] = {br, bw};
+    *rows_out = br;
+    *wpc_out = bw;
     return DTWJ_OK;
 }
 
@@ -543,42 +814,48 @@ int dtwj_motif_discovery(const double *series, int64_t n, int32_t m, int32_t w, 
     return DTWJ_OK;
 }
 
+int dtwj_context_top1_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const dtwj_options *opt,
+                            dtwj_result *out, int32_t *status) {
+    const auto te = clk::now();
+    if (!c || !series || count <= 0 || !out || !status) return fail(DTWJ_EINVAL, "bad batch arguments");
+    DTWJ_HIP(hipSetDevice(c->device));
+    int32_t rows = opt ? opt->rows_per_item : 0, wpc = opt ? opt->waves_per_cu : 0;
+    if (opt && opt->autotune > 0)
+        if (int rc = autotune_batch(c, series, count, opt, &rows, &wpc)) return rc;
+    float ms = 0.f;
+    if (int rc = run_batch(c, series, count, opt, rows, wpc > 0 ? wpc : 8, out, status, &ms)) return rc;
+    const double total = secs(te, clk::now());
+    for (int32_t k = 0; k < count; ++k) out[k].t_total = total;
+    return DTWJ_OK;
+}
+
+int dtwj_top1_batch(const dtwj_series *series, int32_t count, const dtwj_options *opt, dtwj_result *out,
+                    int32_t *status) {
+    if (!series || count <= 0 || !out || !status) return fail(DTWJ_EINVAL, "bad batch arguments");
+    dtwj_context *c = nullptr;
+    if (int rc = with_context(opt ? opt->device : 0, &c)) return rc;
+    int rc = dtwj_context_top1_batch(c, series, count, opt, out, status);
+    dtwj_context_destroy(c);
+    return rc;
+}
+
 // ---------------------------------------------------------------- building blocks
 static int building_block_prep(const double *series, int64_t n, int32_t m, int32_t w, dtwj_context **cout,
                                Prep &P, const double **d_a) {
     if (int rc = with_context(0, cout)) return rc;
     dtwj_context *c = *cout;
     if (int rc = dtwj_context_upload(c, series, n, d_a)) return rc;
-    const int64_t nw = n - m + 1;
-    const size_t need = bytes_for(n, m, 1, 1, 1);
-    DTWJ_HIP(hipMalloc(&c->mem, need));
-    c->mem_cap = need;
+    Job J;
+    plan_job(J, *d_a, n, m, w, 0, 1, false, {}, 1);
+    J.w = w;
+    Arena dry;
+    take_job(dry, J);
+    if (int rc = ensure_mem(c, dry.off + 256)) return rc;
     Arena ar;
     ar.base = (char *)c->mem;
-    ar.cap = need;
-    P.a = *d_a;
-    P.n = n;
-    P.m = m;
-    P.w = w;
-    P.U = ar.take<double>(n);
-    P.L = ar.take<double>(n);
-    P.v = ar.take<double>(n);
-    P.mu = ar.take<double>(nw);
-    P.sig = ar.take<double>(nw);
-    P.muv = ar.take<double>(nw);
-    P.c0 = ar.take<double>(1);
-    P.ac = ar.take<float>(n);
-    P.uc = ar.take<float>(n);
-    P.lc = ar.take<float>(n);
-    P.mu_c = ar.take<float>(nw);
-    P.win = ar.take<WinRow>(nw);
-    P.nblk = (int32_t)((nw + 255) / 256);
-    P.blkmax = ar.take<float>(3 * (size_t)P.nblk);
-    P.gstat = ar.take<float>(8);
-    P.P = ar.take<dd_t>(5 * (size_t)(n + 1));
-    P.bsum = ar.take<dd_t>(5 * (size_t)prep_scan_blocks(n));
-    P.red = ar.take<double>(2 * 1024);
-    P.flags = ar.take<int32_t>(4);
+    ar.cap = c->mem_cap;
+    take_job(ar, J);
+    P = J.P;
     if (int rc = prep_series(P, c->stream)) return rc;
     DTWJ_HIP(hipStreamSynchronize(c->stream));
     return DTWJ_OK;

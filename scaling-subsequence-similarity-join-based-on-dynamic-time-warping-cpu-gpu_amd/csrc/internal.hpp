@@ -34,16 +34,21 @@ int64_t prep_scan_blocks(int64_t n);
 int prep_series(Prep &P, hipStream_t s);
 
 // Seed scan (join.hip): z-normalised ED on a sample of work items; pairs[k] = closest pair
-// of seed item k, or (-1, -1).
+// of seed item k, or (-1, -1); pairs[n_seed] = (0, m).
 int launch_seed(const SeedArgs &args, int2 *pairs, hipStream_t s);
-// DTW of a pair list (seeds); atomicMin into *bsf and pushes candidates like the join.
-int launch_seed_dtw(const JoinArgs &args, const int2 *pairs, int32_t npairs, hipStream_t s);
-// The fused persistent join kernel.
-int launch_join(const JoinArgs &args, int32_t grid, hipStream_t s);
+// DTW of a pair list (seeds) of the series d_args (device); atomicMin into its best-so-far and
+// pushes candidates like the join.
+int launch_seed_dtw(const JoinArgs *d_args, int32_t m, int32_t w, const int2 *pairs, int32_t npairs, hipStream_t s);
+// Error model, validity and state resets of nseries device join descriptors; zeroes *d_work.
+int launch_join_setup(JoinArgs *d_series, int32_t nseries, int32_t *d_work, hipStream_t s);
+// The fused persistent join over nseries series (device descriptors, cumulative item counts).
+int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t nseries, int32_t max_m, int32_t max_w,
+                int32_t *d_work, uint32_t *d_queue, int32_t grid, hipStream_t s);
 size_t join_lds_bytes(int32_t m);
 
 // DTW of an explicit pair list (dtw.hip).  prec 32: FP32 cells from ac/mu_c/isig;
-// prec 64: FP64 cells from a/mu/sig.  cost: npairs doubles (device).
+// prec 64: FP64 cells from a/mu/sig.  cost: npairs doubles (device).  Reads only the
+// series / statistics pointers and m, w of args (a host copy).
 int launch_dtw_pairs(const JoinArgs &args, const double *sig64, const int64_t *pi, const int64_t *pj,
                      int64_t npairs, int32_t prec, double *cost, hipStream_t s);
 

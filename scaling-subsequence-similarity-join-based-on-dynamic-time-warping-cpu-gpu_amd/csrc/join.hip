@@ -1,15 +1,16 @@
-// The fused top-1 join on gfx950: a persistent diagonal sweep over the pair triangle with
-// an in-wave pruning cascade (reference: the host tile loop of
+// The fused top-1 join on gfx950: a persistent diagonal sweep over the pair triangles of one
+// or many series with an in-wave pruning cascade (reference: the host tile loop of
 // GPU_version1/src/new_dtw_motifGUI_malloc.cu:622-740 driving GLOBAL_DIAG,
 // GPU_version1/lib/GLOBAL_DIAG.cu:29-73, CUB compaction and the per-pair LB_Keogh + DTW kernel,
 // GPU_version1/src/dtw_kernal.cu:174-267).
 //
-// Work item = (diagonal group g, row segment s): 256 consecutive diagonal offsets
+// Work item = (series s, diagonal group g, row segment): 256 consecutive diagonal offsets
 // d = d0 + 4*lane + q (q = 0..3, four diagonals per lane so the per-column data is reused
-// from registers) over rows_per_item consecutive rows i (pair (i, j = i + d)).  Items are
-// claimed from a device counter; the best-so-far is a packed 64-bit word (cost bits | i) read
-// at every LDS chunk and every cascade batch and updated with atomicMin, so every wave prunes
-// with the best value found anywhere on the device.  No host synchronisation inside the join.
+// from registers) over rows_per_item consecutive rows i (pair (i, j = i + d)).  The items of
+// all series form one list claimed from a device counter (the device work queue of a batch).
+// Each series has its own best-so-far, a packed 64-bit word (cost bits | i) read at every LDS
+// chunk and every cascade batch and updated with atomicMin, so every wave prunes with the best
+// value found anywhere on the device.  No host synchronisation inside the join.
 //
 // Stages per pair (each a valid lower bound of the squared banded DTW cost):
 //  1. two-sided ball bound (all pairs; cf. diag_fast, GPU_version1/lib/diag_fast.cu:13-90):
@@ -24,13 +25,15 @@
 //  2. LB_Kim over the 3x3 corners at both ends (cf. CPU_version1/lb_kim.cpp:10-73).
 //  3. two-sided LB_Keogh against the raw-series envelope normalised by the other window's
 //     statistics (cf. lb_keogh_stride_with_nomalise, GPU_version1/src/LB_keogh_warp.cu:74-96),
-//     early-abandoned every 16 positions.
+//     early-abandoned every 16 positions; a lane whose pair is decided takes the next queued
+//     pair, so lanes stay busy whatever the abandon points.
 //  4. banded DTW on the wavefront (dtw_device.hpp), early-abandoned against the best-so-far.
-// Ball survivors are appended (ballot/mbcnt compaction, 4-byte packed entries) to a per-wave
-// queue in global memory.  The queue is drained completely at the end of every 64-row chunk
-// (earlier only if it could overflow): the chunk's series / envelope windows and window
-// statistics are then staged in LDS over the sweep's column tables, so stages 2-4 read LDS,
-// not scattered global memory, one pair per lane on full 64-lane batches.
+// Ball survivors are appended to a per-wave queue in global memory, one 4-byte entry per lane
+// and row (row | lane | 4-bit mask of the lane's surviving diagonals).  The queue is drained
+// completely at the end of every 64-row chunk (earlier only if it could overflow): the chunk's
+// series / envelope windows and window statistics are then staged in LDS over the sweep's
+// column tables, the entries are expanded into a pair list, and stages 2-4 read LDS, not
+// scattered global memory, one pair per lane.
 #include <type_traits>
 
 #include "dtw_device.hpp"
@@ -47,15 +50,23 @@ constexpr int RC = 64;                 // rows per LDS chunk
 constexpr int JW = RC + DG + 8;        // staged column entries per chunk
 constexpr int NJARR = 6;               // column arrays: dfa dga dfv dgv 1/sigma thetaB
 constexpr int kSweepFloats = NJARR * JW + NJARR * RC;  // column + row tables
-constexpr int Q2CAP = 128;
-constexpr int Q3CAP = 128;
-constexpr int QLIM = kJoinQueueCap - 8 * DG;  // drain before an 8-row group could overflow
+constexpr int PLCAP = 64 * KD;         // expanded pairs of one 64-entry queue batch
+constexpr int Q2CAP = 256;             // LB_Keogh queue (packed pairs)
+constexpr int Q2RUN = Q2CAP - 64;      // run LB_Keogh once this many pairs are queued
+constexpr int Q3CAP = 128;             // DTW queue (packed pairs)
+constexpr int QLIM = kJoinQueueCap - 8 * 64;  // drain before an 8-row group could overflow
 
 enum { C_SWEPT = 0, C_BALL, C_KIM, C_KEOGH, C_DTW_AB, C_CELLS, C_KEOGH_ELEMS, C_NCOUNTERS };
 
-// LDS floats of the cascade staging for window length m (overlays the column tables).
+// LDS floats of the cascade staging for window length m (overlays the column tables):
+// series / upper / lower envelope of the chunk's row windows and column windows, and the
+// centred mean and 1/sigma of those windows.
 __host__ __device__ constexpr int cascade_floats(int m) {
-    return 3 * (RC + m) + 3 * (RC + DG + m) + 3 * RC + 3 * (RC + DG);
+    return 3 * (RC + m) + 3 * (RC + DG + m) + 2 * RC + 2 * (RC + DG);
+}
+
+__host__ __device__ constexpr int union_floats(int m) {
+    return cascade_floats(m) > kSweepFloats ? cascade_floats(m) : kSweepFloats;
 }
 
 struct Thr {
@@ -122,6 +133,17 @@ __device__ inline unsigned int mbcnt64(unsigned long long msk) {
     return __builtin_amdgcn_mbcnt_hi((unsigned int)(msk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)msk, 0u));
 }
 
+// Inclusive prefix sum of x over the 64 lanes.
+__device__ inline int wave_scan_incl(int x) {
+    const int lane = (int)threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
 // Maximum of blkmax[which] over the 256-window blocks covering windows [lo, hi].
 __device__ inline float range_max(const JoinArgs &A, int which, int64_t lo, int64_t hi) {
     float r = 0.f;
@@ -129,50 +151,72 @@ __device__ inline float range_max(const JoinArgs &A, int which, int64_t lo, int6
     return r;
 }
 
+// Packed (row, column) offsets of a pair inside the staged chunk (both < 2^16).
+__device__ inline uint32_t pack_pair(int li, int lj) { return (uint32_t)li | ((uint32_t)lj << 16); }
+__device__ inline int pair_row(uint32_t p) { return (int)(p & 0xffffu); }
+__device__ inline int pair_col(uint32_t p) { return (int)(p >> 16); }
+
+// series: nseries join descriptors; item_base: nseries + 1 cumulative item counts (the
+// device work queue is the range [0, item_base[nseries]) claimed through work_counter);
+// uni: floats of the LDS union area, sized for the longest window of the launch.
 template <int Q>
-__global__ __launch_bounds__(64) void k_join(JoinArgs A) {
+__global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base,
+                                             int32_t nseries, int32_t uni, int32_t *work_counter, uint32_t *queue) {
     extern __shared__ __align__(16) float smem[];  // 16-byte aligned: the sweep reads float4 rows
     const int lane = (int)threadIdx.x;
-    const int m = A.m;
-    const int64_t nw = A.nw;
-    const int64_t n = A.n;
-    // LDS: [union: column tables | cascade staging] [q2] [q3] [zbuf 2m]
-    const int uni = cascade_floats(m) > kSweepFloats ? cascade_floats(m) : kSweepFloats;
-    float *jarr = smem;              // NJARR x JW column tables of the sweep
-    float *rarr = jarr + NJARR * JW; // NJARR x RC row tables of the sweep
-    // cascade staging (valid only while draining): series / envelope windows of the chunk's
-    // rows (positions c0 ..) and columns (positions c0+d0 ..), and window statistics
-    const int CI = RC + m, CJ = RC + DG + m;
-    float *sai = smem, *sui = sai + CI, *sli = sui + CI;
-    float *saj = sli + CI, *suj = saj + CJ, *slj = suj + CJ;
-    float *rmu = slj + CJ, *ris = rmu + RC, *rsg = ris + RC;
-    float *cmu = rsg + RC, *cis = cmu + (RC + DG), *csg = cis + (RC + DG);
-    int2 *q2 = reinterpret_cast<int2 *>(smem + uni);
-    int2 *q3 = q2 + Q2CAP;
-    float *zbuf = reinterpret_cast<float *>(q3 + Q3CAP);  // 2m
-    uint32_t *q1 = A.queue + (size_t)blockIdx.x * kJoinQueueCap;  // this wave's survivor queue
+    uint32_t *q1 = queue + (size_t)blockIdx.x * kJoinQueueCap;  // this wave's survivor queue
 
     // Stage counters live in LDS (updated by lane 0 on the rare paths) so that they occupy no
-    // scalar registers in the sweep; LB_Keogh positions are counted per lane.
+    // scalar registers in the sweep; LB_Keogh positions are counted per lane.  They are added
+    // to the series' counters whenever the wave moves to another series, and at the end.
     __shared__ __align__(16) unsigned long long wcnt[8];  // 64 bytes keep the dynamic area aligned
-    if (lane < C_NCOUNTERS) wcnt[lane] = 0ull;
+    if (lane < 8) wcnt[lane] = 0ull;
     auto count = [&](int which, unsigned long long v) {
         if (lane == 0) wcnt[which] += v;
     };
     unsigned long long c_kelem = 0;  // per lane: LB_Keogh positions evaluated
+
+    int cur = -1;  // series of the current item
+    const JoinArgs *Ap = series;
+    auto flush_counters = [&]() {
+        if (cur < 0) return;
+        __syncthreads();
+        if (lane == 0) {
+            for (int c = 0; c < C_KEOGH_ELEMS; ++c)
+                if (wcnt[c]) atomicAdd(&Ap->counters[c], wcnt[c]);
+        }
+        if (c_kelem) atomicAdd(&Ap->counters[C_KEOGH_ELEMS], c_kelem);
+        c_kelem = 0;
+        __syncthreads();
+        if (lane < 8) wcnt[lane] = 0ull;
+        __syncthreads();
+    };
+
     int q1n = 0, q2n = 0, q3n = 0;
     int64_t i_beg = 0, d0 = 0, c0 = 0;
-    Thr th = load_threshold(A);
+    int m = 0;
+    Thr th{};
+    float *jarr = smem;               // NJARR x JW column tables of the sweep
+    float *rarr = jarr + NJARR * JW;  // NJARR x RC row tables of the sweep
+    // cascade staging (valid only while draining), laid out per series from its window length
+    float *sai = smem, *sui = smem, *sli = smem, *saj = smem, *suj = smem, *slj = smem;
+    float *rmu = smem, *ris = smem, *cmu = smem, *cis = smem;
+    uint32_t *plist = reinterpret_cast<uint32_t *>(smem + uni);  // expanded pairs of a batch
+    uint32_t *q2 = plist + PLCAP;                                // LB_Keogh queue
+    uint32_t *q3 = q2 + Q2CAP;                                   // DTW queue
+    float *zbuf = reinterpret_cast<float *>(q3 + Q3CAP);         // 2m z-values
 
     // ---- stage 4: DTW of every queued pair, one pair at a time on the whole wave
     auto process_q3 = [&]() {
+        const JoinArgs &A = *Ap;
         for (int e = 0; e < q3n; ++e) {
-            const int2 pr = q3[e];  // local row / column offsets
+            const uint32_t pr = q3[e];
+            const int li = pair_row(pr), lj = pair_col(pr);
             th = load_threshold(A);
-            const float mi = rmu[pr.x], ii = ris[pr.x], mj = cmu[pr.y], ij = cis[pr.y];
+            const float mi = rmu[li], ii = ris[li], mj = cmu[lj], ij = cis[lj];
             for (int k = lane; k < m; k += 64) {
-                zbuf[k] = (sai[pr.x + k] - mi) * ii;
-                zbuf[m + k] = (saj[pr.y + k] - mj) * ij;
+                zbuf[k] = (sai[li + k] - mi) * ii;
+                zbuf[m + k] = (saj[lj + k] - mj) * ij;
             }
             __syncthreads();
             int64_t cells;
@@ -182,7 +226,7 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
             if (ab) {
                 count(C_DTW_AB, 1ull);
             } else if (lane == 0) {
-                const int gi = (int)(c0 + pr.x), gj = (int)(c0 + d0 + pr.y);
+                const int gi = (int)(c0 + li), gj = (int)(c0 + d0 + lj);
                 int slot = atomicAdd(A.n_cands, 1);
                 if (slot < A.cand_cap) A.cands[slot] = make_int4(gi, gj, __float_as_int(cost), 0);
                 atomicMin(A.bsf, pack_bsf(cost, (uint32_t)gi));
@@ -192,69 +236,88 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
         q3n = 0;
     };
 
-    // ---- stage 3: two-sided LB_Keogh, one pair per lane
-    auto process_q2 = [&](bool force) {
-        while (q2n >= 64 || (force && q2n > 0)) {
-            __syncthreads();
-            const int batch = q2n < 64 ? q2n : 64;
-            const int base = q2n - batch;
-            const bool act = lane < batch;
-            int li = 0, lj = 0;  // local row / column offsets
-            if (act) {
-                int2 pr = q2[base + lane];
-                li = pr.x;
-                lj = pr.y;
-            }
-            q2n = base;
-            th = load_threshold(A);
-            bool alive = act;
-            float accA = 0.f, accB = 0.f, thrA = 0.f, thrB = 0.f, rA = 0.f, bA = 0.f, rB = 0.f, bB = 0.f;
-            if (act) {
-                const float si = rsg[li], sj = csg[lj], isi = ris[li], isj = cis[lj];
+    // ---- stage 3: two-sided LB_Keogh over the queued pairs, one pair per lane; a lane takes
+    // the next pair as soon as its pair is abandoned or complete.  Consumes the whole queue.
+    auto process_q2 = [&]() {
+        const JoinArgs &A = *Ap;
+        __syncthreads();  // q2 entries were written by other lanes
+        th = load_threshold(A);
+        const float sm = sqrtf((float)m);
+        int taken = 0;     // queue entries handed out (uniform)
+        bool act = false;  // this lane holds a pair
+        int li = 0, lj = 0, k = 0;
+        float accA = 0.f, accB = 0.f, thrA = 0.f, thrB = 0.f, rA = 0.f, bA = 0.f, rB = 0.f, bB = 0.f;
+        for (;;) {
+            // hand out queued pairs to idle lanes
+            const bool need = !act;
+            const unsigned long long nk = __builtin_amdgcn_ballot_w64(need);
+            const int idx = taken + (int)mbcnt64(nk);
+            if (need && idx < q2n) {
+                const uint32_t pr = q2[idx];
+                li = pair_row(pr);
+                lj = pair_col(pr);
+                const float isi = ris[li], isj = cis[lj];
                 const float mci = rmu[li], mcj = cmu[lj];
-                rA = sj * isi;
+                // raw envelope of window i mapped into the units of window j, and the reverse
+                rA = isi / isj;  // sigma_j / sigma_i
                 bA = mcj - mci * rA;
-                rB = si * isj;
+                rB = isj / isi;  // sigma_i / sigma_j
                 bB = mci - mcj * rB;
-                const float sm = sqrtf((float)m);
                 const float eA = 0x1p-22f * ((A.amax * (1.f + rA) + fabsf(bA)) * isj + sm);
                 const float eB = 0x1p-22f * ((A.amax * (1.f + rB) + fabsf(bB)) * isi + sm);
                 const float T2 = th.T2;
-                thrA = (T2 + 2.f * eA * sqrtf(m * T2) + m * eA * eA + m * 0x1p-22f * T2) * (sj * sj);
-                thrB = (T2 + 2.f * eB * sqrtf(m * T2) + m * eB * eB + m * 0x1p-22f * T2) * (si * si);
+                const float sj2 = 1.f / (isj * isj), si2 = 1.f / (isi * isi);
+                thrA = (T2 + 2.f * eA * sqrtf(m * T2) + m * eA * eA + m * 0x1p-22f * T2) * sj2 * (1.f + 0x1p-20f);
+                thrB = (T2 + 2.f * eB * sqrtf(m * T2) + m * eB * eB + m * 0x1p-22f * T2) * si2 * (1.f + 0x1p-20f);
+                accA = accB = 0.f;
+                k = 0;
+                act = true;
             }
-            for (int k0 = 0; k0 < m; k0 += 16) {
-                if (__ballot(alive) == 0ull) break;
-                if (alive) {
-                    const int kend = k0 + 16 < m ? k0 + 16 : m;
-                    c_kelem += (unsigned long long)(kend - k0);
-                    for (int k = k0; k < kend; ++k) {
-                        const float x = saj[lj + k];
-                        const float hi = fmaf(sui[li + k], rA, bA), lo = fmaf(sli[li + k], rA, bA);
-                        const float t = fmaxf(fmaxf(x - hi, lo - x), 0.f);
-                        accA = fmaf(t, t, accA);
-                        const float y = sai[li + k];
-                        const float hj = fmaf(suj[lj + k], rB, bB), lj_ = fmaf(slj[lj + k], rB, bB);
-                        const float u = fmaxf(fmaxf(y - hj, lj_ - y), 0.f);
-                        accB = fmaf(u, u, accB);
-                    }
-                    alive = (accA <= thrA) && (accB <= thrB);
+            taken += __popcll(nk);
+            if (taken > q2n) taken = q2n;
+            if (__builtin_amdgcn_ballot_w64(act) == 0ull) break;
+            if (act) {
+                const int kend = k + 16 < m ? k + 16 : m;
+                c_kelem += (unsigned long long)(kend - k);
+                for (; k < kend; ++k) {
+                    const float x = saj[lj + k];
+                    const float hi = fmaf(sui[li + k], rA, bA), lo = fmaf(sli[li + k], rA, bA);
+                    const float t = fmaxf(fmaxf(x - hi, lo - x), 0.f);
+                    accA = fmaf(t, t, accA);
+                    const float y = sai[li + k];
+                    const float hj = fmaf(suj[lj + k], rB, bB), lj_ = fmaf(slj[lj + k], rB, bB);
+                    const float u = fmaxf(fmaxf(y - hj, lj_ - y), 0.f);
+                    accB = fmaf(u, u, accB);
                 }
             }
-            const unsigned long long msk = __ballot(alive);
-            if (msk) {
-                if (alive) q3[q3n + (int)mbcnt64(msk)] = make_int2(li, lj);
-                q3n += __popcll(msk);
-                count(C_KEOGH, __popcll(msk));
-                __syncthreads();
-                process_q3();
+            const bool dead = act && !((accA <= thrA) && (accB <= thrB));
+            const bool done = act && !dead && k >= m;
+            if (dead) act = false;
+            const unsigned long long dk = __builtin_amdgcn_ballot_w64(done);
+            if (dk) {
+                if (done) q3[q3n + (int)mbcnt64(dk)] = pack_pair(li, lj);
+                q3n += __popcll(dk);
+                count(C_KEOGH, __popcll(dk));
+                if (done) act = false;
+                if (q3n > Q3CAP - 64) {
+                    __syncthreads();
+                    process_q3();
+                }
             }
+        }
+        q2n = 0;
+        if (q3n > 0) {
+            __syncthreads();
+            process_q3();
         }
     };
 
-    // ---- drain: stage the chunk in LDS, then LB_Kim (stage 2) on 64-lane batches of the
-    // sweep's queue, feeding LB_Keogh and DTW; everything is flushed.
+    // ---- drain: stage the chunk in LDS, expand the queue entries into pairs, LB_Kim (stage 2)
+    // on 64-lane batches, feeding LB_Keogh and DTW; everything is flushed.
     auto drain = [&]() {
+        const JoinArgs &A = *Ap;
+        const int64_t n = A.n, nw = A.nw;
+        const int CI = RC + m, CJ = RC + DG + m;
         __syncthreads();  // queue entries were stored by other lanes; column tables retire
         for (int x = lane; x < CI; x += 64) {
             const int64_t p = c0 + x;
@@ -275,31 +338,42 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
                 const int64_t r = c0 + x;
                 const bool ok = r < nw;
                 rmu[x] = ok ? A.mu_c[r] : 0.f;
-                ris[x] = ok ? A.win[r].isig : 0.f;
-                rsg[x] = ok ? A.win[r].sig : 0.f;
+                ris[x] = ok ? A.win[r].isig : 1.f;
             }
             const int64_t cc = c0 + d0 + x;
             const bool okc = cc < nw;
             cmu[x] = okc ? A.mu_c[cc] : 0.f;
-            cis[x] = okc ? A.win[cc].isig : 0.f;
-            csg[x] = okc ? A.win[cc].sig : 0.f;
+            cis[x] = okc ? A.win[cc].isig : 1.f;
         }
         __syncthreads();
-        for (int h = 0; h < q1n || q2n > 0; h += 64) {
-            const int batch = (q1n - h) < 64 ? (q1n - h) : 64;
-            if (batch > 0) {
-                const bool act = lane < batch;
-                int li = 0, lj = 0;
-                if (act) {
-                    const uint32_t e = q1[h + lane];
-                    li = (int)((int64_t)(e >> 8) - (c0 - i_beg));  // row offset in the chunk
-                    lj = li + (int)(e & 255u);                       // column offset
-                }
-                th = load_threshold(A);
+        const int row0 = (int)(c0 - i_beg);  // chunk start relative to the item start
+        for (int h = 0; h < q1n; h += 64) {
+            // expand up to 64 entries (row | lane | 4-bit mask) into pairs
+            uint32_t e = 0;
+            if (h + lane < q1n) e = q1[h + lane];
+            const uint32_t mask = e & 15u;
+            const int li = (int)(e >> 10) - row0;
+            const int lb = li + KD * (int)((e >> 4) & 63u);
+            const int cnt = __popc(mask);
+            const int incl = wave_scan_incl(cnt);
+            const int total = __shfl(incl, 63, 64);
+            int pos = incl - cnt;
+#pragma unroll
+            for (int q = 0; q < KD; ++q)
+                if ((mask >> q) & 1u) plist[pos++] = pack_pair(li, lb + q);
+            count(C_BALL, (unsigned long long)total);
+            __syncthreads();
+            th = load_threshold(A);
+            for (int p = 0; p < total; p += 64) {
+                const bool act = p + lane < total;
+                int pi = 0, pj = 0;
                 bool alive = false;
                 if (act) {
-                    const float mi = rmu[li], ii = ris[li], mj = cmu[lj], ij = cis[lj];
-                    const float *ai = sai + li, *aj = saj + lj;
+                    const uint32_t pr = plist[p + lane];
+                    pi = pair_row(pr);
+                    pj = pair_col(pr);
+                    const float mi = rmu[pi], ii = ris[pi], mj = cmu[pj], ij = cis[pj];
+                    const float *ai = sai + pi, *aj = saj + pj;
                     // x: window i (columns), y: window j (rows)
                     const float x0 = (ai[0] - mi) * ii, x1 = (ai[1] - mi) * ii, x2 = (ai[2] - mi) * ii;
                     const float y0 = (aj[0] - mj) * ij, y1 = (aj[1] - mj) * ij, y2 = (aj[2] - mj) * ij;
@@ -326,24 +400,60 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
                     const float kim = (c00 + bl) + (e00 + br);
                     alive = kim <= th.kim;
                 }
-                const unsigned long long msk = __ballot(alive);
+                const unsigned long long msk = __builtin_amdgcn_ballot_w64(alive);
                 if (msk) {
-                    if (alive) q2[q2n + (int)mbcnt64(msk)] = make_int2(li, lj);
+                    if (alive) q2[q2n + (int)mbcnt64(msk)] = pack_pair(pi, pj);
                     q2n += __popcll(msk);
                     count(C_KIM, __popcll(msk));
+                    if (q2n >= Q2RUN) process_q2();
                 }
             }
-            process_q2(h + 64 >= q1n);
+            __syncthreads();  // plist is rewritten by the next batch
         }
+        if (q2n > 0) process_q2();
         q1n = 0;
         __syncthreads();
     };
 
     for (;;) {
         int item_l = 0;
-        if (lane == 0) item_l = atomicAdd(A.work_counter, 1);
-        const int64_t item = (int64_t)__builtin_amdgcn_readfirstlane(item_l);
-        if (item >= A.n_items) break;
+        if (lane == 0) item_l = atomicAdd(work_counter, 1);
+        const int64_t gitem = (int64_t)__builtin_amdgcn_readfirstlane(item_l);
+        if (gitem >= item_base[nseries]) break;
+        // series of this item (usually the one of the previous item)
+        int s = cur;
+        if (s < 0 || gitem < item_base[s] || gitem >= item_base[s + 1]) {
+            int lo = 0, hi = nseries - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (item_base[mid] <= gitem)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            s = lo;
+        }
+        if (s != cur) {
+            flush_counters();
+            cur = s;
+            Ap = series + s;
+            m = Ap->m;
+            const int CI = RC + m, CJ = RC + DG + m;
+            sai = smem;
+            sui = sai + CI;
+            sli = sui + CI;
+            saj = sli + CI;
+            suj = saj + CJ;
+            slj = suj + CJ;
+            rmu = slj + CJ;
+            ris = rmu + RC;
+            cmu = ris + RC;
+            cis = cmu + (RC + DG);
+        }
+        const JoinArgs &A = *Ap;
+        if (!A.valid) continue;
+        const int64_t nw = A.nw, n = A.n;
+        const int64_t item = gitem - item_base[s];
         int g;
         int64_t sidx;
         decode_item(A.item_start, A.groups, item, g, sidx);
@@ -434,8 +544,8 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
             }
             // the whole chunk is inside the triangle for every lane
             const bool chunk_full = c0 + nrows - 1 + d0 + DG - 1 <= nw - 1;
-            const uint32_t code0 = (uint32_t)(c0 - i_beg) << 8;
-            int rl = 0;  // next local row (multiple of 4)
+            const uint32_t code0 = (uint32_t)(c0 - i_beg) << 10;
+            int rl = 0;  // next local row (multiple of 8)
             while (rl < nrows) {
                 th = load_threshold(A);
                 const float T = th.T;
@@ -500,8 +610,9 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
                 // column slots end every group in the registers they started in.  FULL: every
                 // pair is inside the triangle.  Test per pair: X^A / sigma_j >= thetaA_i and
                 // X^B / sigma_i >= thetaB_j as signs of single-rounding FMAs; a lane-wide
-                // min/max reduces the four pairs to one compare per row, and lane masks are only
-                // formed on the rare rows with a survivor.
+                // min/max reduces the four pairs to one compare per row, and a queue entry
+                // (row | lane | mask of the lane's surviving diagonals) is only formed on rows
+                // with a survivor.
                 auto group = [&](auto full_tag) {
                     constexpr bool FULL = decltype(full_tag)::value;
 #pragma unroll
@@ -535,18 +646,16 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
                                 if (!FULL && r0 + u > limr[q]) t[q] = -1.0f;
                             }
                             const float tmax = fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3]));
-                            if (__builtin_amdgcn_ballot_w64(tmax >= 0.0f) != 0ull) {
-                                const uint32_t rowcode = code0 + ((uint32_t)(r0 + u) << 8);
-#pragma unroll
-                                for (int q = 0; q < KD; ++q) {
-                                    const bool hit = t[q] >= 0.0f;
-                                    const unsigned long long mk = __builtin_amdgcn_ballot_w64(hit);
-                                    if (mk) {
-                                        if (hit) q1[q1n + (int)mbcnt64(mk)] = rowcode | (uint32_t)(KD * lane + q);
-                                        q1n += __popcll(mk);
-                                        count(C_BALL, __popcll(mk));
-                                    }
+                            const bool any = tmax >= 0.0f;
+                            const unsigned long long mk = __builtin_amdgcn_ballot_w64(any);
+                            if (mk != 0ull) {
+                                if (any) {
+                                    const uint32_t bits = (t[0] >= 0.0f ? 1u : 0u) | (t[1] >= 0.0f ? 2u : 0u) |
+                                                          (t[2] >= 0.0f ? 4u : 0u) | (t[3] >= 0.0f ? 8u : 0u);
+                                    q1[q1n + (int)mbcnt64(mk)] =
+                                        code0 + ((uint32_t)(r0 + u) << 10) + ((uint32_t)lane << 4) + bits;
                                 }
+                                q1n += __popcll(mk);
                             }
 #pragma unroll
                             for (int q = 0; q < KD; ++q) {
@@ -582,17 +691,38 @@ __global__ __launch_bounds__(64) void k_join(JoinArgs A) {
             }
         }
     }
-    if (lane == 0) {
-        for (int c = 0; c < C_KEOGH_ELEMS; ++c) atomicAdd(&A.counters[c], wcnt[c]);
-    }
-    if (c_kelem) atomicAdd(&A.counters[C_KEOGH_ELEMS], c_kelem);
+    flush_counters();
+}
+
+// ---- set-up of every series of a join: the FP32 error model from the preprocessing's value
+// range, the validity flag, and the best-so-far / candidate / counter resets (a kernel, so
+// that a captured graph holds no memset node).  One thread per series.
+__global__ void k_join_setup(JoinArgs *series, int32_t nseries, int32_t *work_counter) {
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s == 0) *work_counter = 0;
+    if (s >= nseries) return;
+    JoinArgs &A = series[s];
+    const float *g = A.gstat;
+    const double m = (double)A.m;
+    A.amax = g[0];
+    A.eps_z = (float)(0x1p-22 * (((double)g[0] + (double)g[1]) * (double)g[2] + sqrt(m)));
+    A.rel_dtw = (float)(0x1p-10 + 4.0 * m * 0x1p-24);
+    A.valid = (A.flags[0] == 0 && A.flags[1] == 0) ? 1 : 0;
+    *A.bsf = pack_bsf(3.0e38f, 0xffffffffu);
+    *A.n_cands = 0;
+    for (int c = 0; c < C_NCOUNTERS; ++c) A.counters[c] = 0ull;
 }
 
 // ---- seed scan: z-normalised Euclidean distance (an upper bound of banded DTW) over a
-// sample of work items; each wave reports its closest pair.
+// sample of work items; each wave reports its closest pair.  Block n_seed writes the fixed
+// seed pair (0, m).
 __global__ __launch_bounds__(64) void k_seed(SeedArgs S, int2 *pairs) {
     const int lane = (int)threadIdx.x;
     const int m = S.m;
+    if ((int)blockIdx.x == S.n_seed) {
+        if (lane == 0) pairs[blockIdx.x] = make_int2(0, m);
+        return;
+    }
     const int64_t nw = S.nw;
     const int64_t item = ((int64_t)blockIdx.x * S.n_items) / S.n_seed;
     int g;
@@ -650,12 +780,14 @@ __global__ __launch_bounds__(64) void k_seed(SeedArgs S, int2 *pairs) {
 
 // ---- DTW of a list of candidate pairs (seeds); pushes results like the join does.
 template <int Q>
-__global__ __launch_bounds__(64) void k_seed_dtw(JoinArgs A, const int2 *pairs, int npairs) {
+__global__ __launch_bounds__(64) void k_seed_dtw(const JoinArgs *__restrict__ Ap, const int2 *pairs, int npairs) {
     extern __shared__ float zs[];
+    const JoinArgs &A = *Ap;
     const int lane = (int)threadIdx.x;
     const int m = A.m;
+    if (blockIdx.x >= (unsigned)npairs || !A.valid) return;
     const int2 pr = pairs[blockIdx.x];
-    if (blockIdx.x >= (unsigned)npairs || pr.x < 0) return;
+    if (pr.x < 0) return;
     const int i = pr.x, j = pr.y;
     const Thr th = load_threshold(A);
     const float mi = A.mu_c[i], ii = A.win[i].isig, mj = A.mu_c[j], ij = A.win[j].isig;
@@ -677,45 +809,57 @@ __global__ __launch_bounds__(64) void k_seed_dtw(JoinArgs A, const int2 *pairs, 
 }  // namespace
 
 size_t join_lds_bytes(int32_t m) {
-    const int uni = cascade_floats(m) > kSweepFloats ? cascade_floats(m) : kSweepFloats;
-    return sizeof(float) * (size_t)uni + sizeof(int2) * (Q2CAP + Q3CAP) + sizeof(float) * 2 * (size_t)m;
+    return sizeof(float) * (size_t)union_floats(m) + sizeof(uint32_t) * (PLCAP + Q2CAP + Q3CAP) +
+           sizeof(float) * 2 * (size_t)m;
 }
 
-int launch_join(const JoinArgs &A, int32_t grid, hipStream_t s) {
-    const int Q = dtw_q_for(A.w);
-    const size_t lds = join_lds_bytes(A.m);
+int launch_join_setup(JoinArgs *d_series, int32_t nseries, int32_t *d_work, hipStream_t s) {
+    hipLaunchKernelGGL(k_join_setup, dim3((unsigned)((nseries + 63) / 64)), dim3(64), 0, s, d_series, nseries, d_work);
+    DTWJ_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t nseries, int32_t max_m, int32_t max_w,
+                int32_t *d_work, uint32_t *d_queue, int32_t grid, hipStream_t s) {
+    const int Q = dtw_q_for(max_w);
+    const size_t lds = join_lds_bytes(max_m);
+    const int32_t uni = union_floats(max_m);
     if (lds + 64 > 160 * 1024) {  // + the kernel's static stage counters
         set_error("window too long for the join's LDS staging");
         return DTWJ_EINVAL;
     }
+#define DTWJ_LAUNCH_JOIN(QQ)                                                                                  \
+    hipLaunchKernelGGL(k_join<QQ>, dim3(grid), dim3(64), lds, s, d_series, d_item_base, nseries, uni, d_work, \
+                       d_queue)
     switch (Q) {
-        case 1: hipLaunchKernelGGL(k_join<1>, dim3(grid), dim3(64), lds, s, A); break;
-        case 2: hipLaunchKernelGGL(k_join<2>, dim3(grid), dim3(64), lds, s, A); break;
-        case 4: hipLaunchKernelGGL(k_join<4>, dim3(grid), dim3(64), lds, s, A); break;
-        case 8: hipLaunchKernelGGL(k_join<8>, dim3(grid), dim3(64), lds, s, A); break;
-        case 16: hipLaunchKernelGGL(k_join<16>, dim3(grid), dim3(64), lds, s, A); break;
+        case 1: DTWJ_LAUNCH_JOIN(1); break;
+        case 2: DTWJ_LAUNCH_JOIN(2); break;
+        case 4: DTWJ_LAUNCH_JOIN(4); break;
+        case 8: DTWJ_LAUNCH_JOIN(8); break;
+        case 16: DTWJ_LAUNCH_JOIN(16); break;
         default: set_error("band too wide: 2w+1 must be <= 1024"); return DTWJ_EINVAL;
     }
+#undef DTWJ_LAUNCH_JOIN
     DTWJ_HIP(hipGetLastError());
     return 0;
 }
 
 int launch_seed(const SeedArgs &S, int2 *pairs, hipStream_t s) {
-    hipLaunchKernelGGL(k_seed, dim3(S.n_seed), dim3(64), 0, s, S, pairs);
+    hipLaunchKernelGGL(k_seed, dim3(S.n_seed + 1), dim3(64), 0, s, S, pairs);
     DTWJ_HIP(hipGetLastError());
     return 0;
 }
 
-int launch_seed_dtw(const JoinArgs &A, const int2 *pairs, int32_t npairs, hipStream_t s) {
+int launch_seed_dtw(const JoinArgs *d_args, int32_t m, int32_t w, const int2 *pairs, int32_t npairs, hipStream_t s) {
     if (npairs <= 0) return 0;
-    const int Q = dtw_q_for(A.w);
-    const size_t lds = sizeof(float) * 2 * (size_t)A.m;
+    const int Q = dtw_q_for(w);
+    const size_t lds = sizeof(float) * 2 * (size_t)m;
     switch (Q) {
-        case 1: hipLaunchKernelGGL(k_seed_dtw<1>, dim3(npairs), dim3(64), lds, s, A, pairs, npairs); break;
-        case 2: hipLaunchKernelGGL(k_seed_dtw<2>, dim3(npairs), dim3(64), lds, s, A, pairs, npairs); break;
-        case 4: hipLaunchKernelGGL(k_seed_dtw<4>, dim3(npairs), dim3(64), lds, s, A, pairs, npairs); break;
-        case 8: hipLaunchKernelGGL(k_seed_dtw<8>, dim3(npairs), dim3(64), lds, s, A, pairs, npairs); break;
-        case 16: hipLaunchKernelGGL(k_seed_dtw<16>, dim3(npairs), dim3(64), lds, s, A, pairs, npairs); break;
+        case 1: hipLaunchKernelGGL(k_seed_dtw<1>, dim3(npairs), dim3(64), lds, s, d_args, pairs, npairs); break;
+        case 2: hipLaunchKernelGGL(k_seed_dtw<2>, dim3(npairs), dim3(64), lds, s, d_args, pairs, npairs); break;
+        case 4: hipLaunchKernelGGL(k_seed_dtw<4>, dim3(npairs), dim3(64), lds, s, d_args, pairs, npairs); break;
+        case 8: hipLaunchKernelGGL(k_seed_dtw<8>, dim3(npairs), dim3(64), lds, s, d_args, pairs, npairs); break;
+        case 16: hipLaunchKernelGGL(k_seed_dtw<16>, dim3(npairs), dim3(64), lds, s, d_args, pairs, npairs); break;
         default: set_error("band too wide: 2w+1 must be <= 1024"); return DTWJ_EINVAL;
     }
     DTWJ_HIP(hipGetLastError());
