@@ -11,7 +11,7 @@ namespace dtwj {
 
 namespace {
 
-template <typename T, int Q>
+template <typename T, int QP>
 __global__ __launch_bounds__(64) void k_dtw_pairs(JoinArgs A, const double *__restrict__ sig64,
                                                   const int64_t *__restrict__ pi, const int64_t *__restrict__ pj,
                                                   int64_t npairs, double *cost) {
@@ -36,24 +36,23 @@ __global__ __launch_bounds__(64) void k_dtw_pairs(JoinArgs A, const double *__re
         }
     }
     __syncthreads();
-    int64_t cells;
-    bool ab;
-    T r = dtw_wave<T, Q>(zi, zj, A.m, A.w, dev_inf<T>(), cells, ab);
+    const PlainZ<T> z{zi, zj};
+    int t_stop = 0;
+    const T r = dtw_band<T, QP>(A.m, A.w, 64, 1, dev_inf<T>(), z, t_stop);
     if (lane == 0) cost[p] = (double)r;
 }
 
 template <typename T>
 int launch_pairs_t(const JoinArgs &A, const double *sig64, const int64_t *pi, const int64_t *pj, int64_t npairs,
                    double *cost, hipStream_t s) {
-    const int Q = dtw_q_for(A.w);
+    const int QP = dtw_qp_for(A.w);
     const size_t lds = 2 * (size_t)A.m * sizeof(T);
     const dim3 grid((unsigned)npairs), block(64);
-    switch (Q) {
+    switch (QP) {
         case 1: hipLaunchKernelGGL((k_dtw_pairs<T, 1>), grid, block, lds, s, A, sig64, pi, pj, npairs, cost); break;
         case 2: hipLaunchKernelGGL((k_dtw_pairs<T, 2>), grid, block, lds, s, A, sig64, pi, pj, npairs, cost); break;
         case 4: hipLaunchKernelGGL((k_dtw_pairs<T, 4>), grid, block, lds, s, A, sig64, pi, pj, npairs, cost); break;
         case 8: hipLaunchKernelGGL((k_dtw_pairs<T, 8>), grid, block, lds, s, A, sig64, pi, pj, npairs, cost); break;
-        case 16: hipLaunchKernelGGL((k_dtw_pairs<T, 16>), grid, block, lds, s, A, sig64, pi, pj, npairs, cost); break;
         default: set_error("band too wide: 2w+1 must be <= 1024"); return DTWJ_EINVAL;
     }
     DTWJ_HIP(hipGetLastError());

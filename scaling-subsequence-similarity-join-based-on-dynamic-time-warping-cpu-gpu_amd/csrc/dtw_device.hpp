@@ -1,19 +1,25 @@
-// Banded DTW of one pair on one 64-lane wavefront (device code).
+// Banded DTW on one 64-lane wavefront (device code).
 //
-// Cells are visited by anti-diagonals t = r + c.  Lane l owns the band offsets
-// delta = c - r = l + 64 q - w for q = 0..Q-1 (Q = ceil((2w+1)/64) is a template parameter,
-// so any band up to 64*Q-1 cells runs without recompiling; it replaces the reference's
-// compile-time REGISTER_NUM, GPU_version1/src/DTW_stairs.cu:5).  A lane keeps one value per
-// owned offset: the cell of the latest anti-diagonal of that offset's parity.  Cell (r, c)
-// on anti-diagonal t needs (r-1, c-1) [same offset, t-2: own register], (r-1, c) [offset+1,
-// t-1: right neighbour] and (r, c-1) [offset-1, t-1: left neighbour]; neighbours arrive by
-// one-lane wave shifts.  Recurrence as in the reference kernels
-// (GPU_version1/src/DTW_stairs.cu:446-599): D = (x_r - y_c)^2 + min(diag, up, left),
-// out-of-band cells infinite.
+// Cells are visited by anti-diagonals t = r + c, t = 0 .. 2m-2.  A cell's band offset is
+// delta = c - r, |delta| <= w, and the cells of anti-diagonal t have delta = t (mod 2).  Each
+// lane owns QP pairs of adjacent offsets (e, o = e + 1), e = 2 (x QP + k) - w for lane x of its
+// segment and k = 0..QP-1; every step therefore evaluates exactly one cell per pair on every
+// lane: the even-parity offsets on steps with t = w (mod 2), the odd ones on the others.
+// Recurrence (as the reference kernels, GPU_version1/src/DTW_stairs.cu:446-599):
+//   D(r, c) = (x_r - y_c)^2 + min(D(r-1, c-1), D(r-1, c), D(r, c-1)),   D(-1, -1) = 0,
+// with (r-1, c-1) = same offset two steps back (own register), (r-1, c) = offset + 1 one step
+// back and (r, c-1) = offset - 1 one step back: own registers except across the lane edge,
+// which arrives by one DPP wave shift per step.  QP is a template parameter, so any band up to
+// 128 QP offsets runs without recompiling (it replaces the compile-time REGISTER_NUM of
+// GPU_version1/src/DTW_stairs.cu:5).  Offsets beyond w are forced to +inf.
 //
-// Early abandon: every path crosses anti-diagonal t or t-1, so once the minimum over all
-// lanes' registers exceeds ub the result can only exceed ub; the routine then returns +inf
-// (the reference abandons with a ballot at the band's middle, DTW_stairs.cu:225-237).
+// With QP = 1 and w <= 63 the wave holds several pairs: segments of seg_len >= w + 1 lanes, one
+// pair per segment; the last lane of a segment owns an out-of-band offset, so the shifts never
+// carry values between pairs.
+//
+// Early abandon: every path crosses anti-diagonal t or t-1, so once the minimum over a
+// segment's registers exceeds ub its result can only exceed ub; the segment is then reported
+// abandoned (the reference abandons with a ballot at the band's middle, DTW_stairs.cu:225-237).
 #pragma once
 
 #include "common.hpp"
@@ -41,100 +47,188 @@ __device__ inline T wave_min(T x) {
     return x;
 }
 
-// zi, zj: m values each (LDS).  Returns the squared cost D[m-1][m-1], or +inf when abandoned.
-// cells: number of cells evaluated (uniform).  Must be called by all 64 lanes.
-template <typename T, int Q>
-__device__ T dtw_wave(const T *zi, const T *zj, int m, int w, T ub, int64_t &cells, bool &abandoned) {
+// Lane L receives src of lane L - 1 (lane 0: edge) / of lane L + 1 (lane 63: edge): DPP wave
+// shifts, the out-of-range lane keeping `edge`.
+__device__ inline uint32_t dpp_from_left_u32(uint32_t edge, uint32_t src) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)src, 0x138, 0xf, 0xf, false);
+}
+__device__ inline uint32_t dpp_from_right_u32(uint32_t edge, uint32_t src) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)src, 0x130, 0xf, 0xf, false);
+}
+template <typename T>
+__device__ inline T from_left(T edge, T src);
+template <typename T>
+__device__ inline T from_right(T edge, T src);
+template <>
+__device__ inline float from_left<float>(float edge, float src) {
+    return __builtin_bit_cast(float, dpp_from_left_u32(__builtin_bit_cast(uint32_t, edge), __builtin_bit_cast(uint32_t, src)));
+}
+template <>
+__device__ inline float from_right<float>(float edge, float src) {
+    return __builtin_bit_cast(float, dpp_from_right_u32(__builtin_bit_cast(uint32_t, edge), __builtin_bit_cast(uint32_t, src)));
+}
+template <>
+__device__ inline double from_left<double>(double edge, double src) {
+    const uint64_t e = __builtin_bit_cast(uint64_t, edge), s = __builtin_bit_cast(uint64_t, src);
+    const uint32_t lo = dpp_from_left_u32((uint32_t)e, (uint32_t)s);
+    const uint32_t hi = dpp_from_left_u32((uint32_t)(e >> 32), (uint32_t)(s >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <>
+__device__ inline double from_right<double>(double edge, double src) {
+    const uint64_t e = __builtin_bit_cast(uint64_t, edge), s = __builtin_bit_cast(uint64_t, src);
+    const uint32_t lo = dpp_from_right_u32((uint32_t)e, (uint32_t)s);
+    const uint32_t hi = dpp_from_right_u32((uint32_t)(e >> 32), (uint32_t)(s >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+__device__ inline float min3(float a, float b, float c) { return fminf(fminf(a, b), c); }
+__device__ inline double min3(double a, double b, double c) { return fmin(fmin(a, b), c); }
+
+// z-values of a DTW pair already normalised in memory.
+template <typename T>
+struct PlainZ {
+    const T *zi, *zj;
+    __device__ T row(int r) const { return zi[r]; }
+    __device__ T col(int c) const { return zj[c]; }
+};
+
+// Cells (r, c) with r + c <= t, |r - c| <= w, 0 <= r, c <= m - 1 (w <= m - 1): the cells a
+// DTW evaluated when it stopped after anti-diagonal t.
+__host__ __device__ inline int64_t dtw_cells_upto(int m, int w, int64_t t) {
+    auto G = [w](int64_t s) -> int64_t {  // same count without the upper bounds
+        if (s < 0) return 0;
+        if (s <= w) return (s + 1) * (s + 2) / 2;
+        return (int64_t)(w + 1) * (w + 2) / 2 + (s - w) * w + (s - w) / 2;
+    };
+    const int64_t total = (int64_t)m * (2 * w + 1) - (int64_t)w * (w + 1);
+    if (t >= 2 * (int64_t)m - 2) return total;
+    if (t <= m - 1) return G(t);
+    return total - G(2 * (int64_t)m - 3 - t);
+}
+
+// Offset pairs per lane for band radius w (one pair per wave): 128 QP >= 2w + 1.
+inline int dtw_qp_for(int w) {
+    const int band = 2 * w + 1;
+    if (band <= 128) return 1;
+    if (band <= 256) return 2;
+    if (band <= 512) return 4;
+    if (band <= 1024) return 8;
+    return 0;
+}
+
+// Banded DTW of up to `nseg` pairs at once (QP = 1: segments of seg_len lanes, seg_len >= w+1;
+// QP > 1: nseg = 1, seg_len = 64).  z.row(r) / z.col(c) give the z-values of the lane's pair
+// (0 <= r, c <= m-1; they are only called with clamped indices).  Returns, on every lane, the
+// squared cost D(m-1, m-1) of the lane's pair, or +inf when the pair was abandoned against ub
+// (then t_stop is the last anti-diagonal evaluated, else 2m-2).
+//
+// Steps come in pairs (even offsets at t, odd offsets at t+1): both cells of a lane's offset
+// pair lie on the same row r = (t - e)/2, so a pair of steps reads one row value and two
+// column values.
+template <typename T, int QP, typename Z>
+__device__ __forceinline__ T dtw_band(int m, int w, int seg_len, int nseg, T ub, const Z &z, int &t_stop) {
     const int lane = (int)(threadIdx.x & 63);
     const T INF = dev_inf<T>();
-    T D[Q];
-    int dl[Q];
+    const int seg = lane / seg_len;
+    const int x = lane - seg * seg_len;
+    const bool valid = seg < nseg;
+    T E[QP], O[QP];
+    int e_off[QP];
+    bool inE[QP], inO[QP];
+    // the slot of offset 0 holds D(-1, -1) = 0 before the first step
+    const int idx0 = w >> 1;              // pair index of offset 0
+    const bool zero_even = (w & 1) == 0;  // offset 0 is the even member of its pair
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        dl[q] = lane + 64 * q - w;
-        D[q] = INF;
-        if (dl[q] == 0) {
-            T d = zi[0] - zj[0];
-            D[q] = d * d;
-        }
+    for (int k = 0; k < QP; ++k) {
+        const int xk = x * QP + k;
+        e_off[k] = 2 * xk - w;
+        inE[k] = valid && e_off[k] <= w;
+        inO[k] = valid && e_off[k] + 1 <= w;
+        E[k] = (valid && xk == idx0 && zero_even) ? (T)0 : INF;
+        O[k] = (valid && xk == idx0 && !zero_even) ? (T)0 : INF;
     }
-    cells = 1;
-    abandoned = false;
+    const int mm1 = m - 1;
     const int t_end = 2 * m - 2;
-    for (int t = 1; t <= t_end; ++t) {
-        T up[Q], lf[Q];
+    auto clampi = [mm1](int v) { return min(max(v, 0), mm1); };
+    auto odd_step = [&](int r[QP], T zr[QP]) __attribute__((always_inline)) {
+        // odd offsets: up = next even (t-1), left = own even (t-1), diag = own odd (t-2)
+        const T right_in = from_right<T>(INF, E[0]);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            T dn = __shfl_down(D[q], 1, 64);  // lane l receives lane l+1
-            T uu = __shfl_up(D[q], 1, 64);    // lane l receives lane l-1
-            if (Q > 1) {
-                T from0 = __shfl(D[(q + 1 < Q) ? q + 1 : q], 0, 64);
-                T from63 = __shfl(D[(q > 0) ? q - 1 : q], 63, 64);
-                up[q] = (lane == 63) ? ((q + 1 < Q) ? from0 : INF) : dn;
-                lf[q] = (lane == 0) ? ((q > 0) ? from63 : INF) : uu;
+        for (int k = 0; k < QP; ++k) {
+            const T up = k + 1 < QP ? E[k + 1] : right_in;
+            const T d = zr[k] - z.col(clampi(r[k] + e_off[k] + 1));
+            const T v = fma(d, d, min3(O[k], up, E[k]));
+            O[k] = inO[k] ? v : INF;
+        }
+    };
+    bool alive = valid;
+    t_stop = t_end;
+    int t = 0;
+    int r[QP];
+    T zr[QP];
+    if (!zero_even) {
+        // t = 0 has odd parity: only the odd offsets, on row (0 - e - 1) / 2
+#pragma unroll
+        for (int k = 0; k < QP; ++k) {
+            r[k] = (0 - e_off[k] - 1) >> 1;
+            zr[k] = z.row(clampi(r[k]));
+        }
+        odd_step(r, zr);
+        t = 1;
+    }
+#pragma unroll
+    for (int k = 0; k < QP; ++k) r[k] = (t - e_off[k]) >> 1;
+    for (; t <= t_end; t += 2) {
+        // even offsets at t: up = own odd (t-1), left = previous odd (t-1), diag = own even (t-2)
+        const T left_in = from_left<T>(INF, O[QP - 1]);
+#pragma unroll
+        for (int k = 0; k < QP; ++k) {
+            zr[k] = z.row(clampi(r[k]));
+            const T left = k > 0 ? O[k - 1] : left_in;
+            const T d = zr[k] - z.col(clampi(r[k] + e_off[k]));
+            const T v = fma(d, d, min3(E[k], O[k], left));
+            E[k] = inE[k] ? v : INF;
+        }
+        if (t + 1 <= t_end) odd_step(r, zr);
+#pragma unroll
+        for (int k = 0; k < QP; ++k) ++r[k];
+        if ((t & 15) >= 14 && t + 1 < t_end) {
+            // segment minimum over both anti-diagonals held in the registers
+            T v = INF;
+#pragma unroll
+            for (int k = 0; k < QP; ++k) v = min3(v, E[k], O[k]);
+            if (nseg == 1) {
+                v = wave_min(v);
             } else {
-                up[q] = (lane == 63) ? INF : dn;
-                lf[q] = (lane == 0) ? INF : uu;
-            }
-        }
-        int active = 0;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const int dd = dl[q];
-            if (((t - dd) & 1) == 0 && dd >= -w && dd <= w) {
-                const int r = (t - dd) >> 1;
-                const int c = r + dd;
-                if (r >= 0 && c >= 0 && r < m && c < m) {
-                    T d = zi[r] - zj[c];
-                    T best = D[q] < up[q] ? D[q] : up[q];
-                    best = best < lf[q] ? best : lf[q];
-                    D[q] = d * d + best;
-                    ++active;
-                } else {
-                    D[q] = INF;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const T y = __shfl_down(v, off, 64);
+                    if (x + off < seg_len) v = y < v ? y : v;
                 }
+                v = __shfl(v, seg * seg_len, 64);
             }
-        }
-        // count evaluated cells (uniform)
-        unsigned long long b1 = __ballot(active >= 1);
-        cells += (int64_t)__popcll(b1);
-        if (Q > 1) {
-            for (int k = 2; k <= Q; ++k) cells += (int64_t)__popcll(__ballot(active >= k));
-        }
-        if ((t & 15) == 0) {
-            T mn = D[0];
-#pragma unroll
-            for (int q = 1; q < Q; ++q) mn = D[q] < mn ? D[q] : mn;
-            mn = wave_min(mn);
-            if (mn > ub) {
-                abandoned = true;
-                return INF;
+            if (alive && v > ub) {
+                alive = false;
+                t_stop = t + 1;
             }
+            if (__builtin_amdgcn_ballot_w64(alive) == 0ull) break;
         }
     }
-    // result: offset 0 lives in lane w % 64, register w / 64
-    const int qsel = w >> 6;
-    T res = INF;
+    // result: offset 0 of the lane's segment
+    const int src_lane = valid ? seg * seg_len + idx0 / QP : lane;
+    const int k0 = idx0 % QP;
+    T mine = INF;
 #pragma unroll
-    for (int q = 0; q < Q; ++q)
-        if (q == qsel) res = D[q];
-    res = __shfl(res, w & 63, 64);
+    for (int k = 0; k < QP; ++k)
+        if (k == k0) mine = zero_even ? E[k] : O[k];
+    const T res = __shfl(mine, src_lane, 64);
+    if (!alive || !valid) return INF;
     if (res > ub) {
-        abandoned = true;
+        t_stop = t_end;
         return INF;
     }
     return res;
-}
-
-// Lanes per band: Q such that 2w+1 <= 64*Q.
-inline int dtw_q_for(int w) {
-    int band = 2 * w + 1;
-    if (band <= 64) return 1;
-    if (band <= 128) return 2;
-    if (band <= 256) return 4;
-    if (band <= 512) return 8;
-    if (band <= 1024) return 16;
-    return 0;
 }
 
 }  // namespace dtwj

@@ -678,7 +678,7 @@ int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const d
     return DTWJ_OK;
 }
 
-// Autotuner of the batched form: times the join on a sample of the batch (up to 16 series
+// Autotuner of the batched form: times the join on a sample of the batch (up to 32 series
 // spread over the batch, in its order) for each candidate tile height and occupancy, and
 // keeps the fastest.  The choice is cached in the context, keyed by the batch's (m, w) mix
 // and size, so repeated batches of the same shape do not pay for it again.
@@ -701,9 +701,9 @@ int autotune_batch(dtwj_context *c, const dtwj_series *series, int32_t count, co
         return DTWJ_OK;
     }
     std::vector<dtwj_series> sample;
-    const int32_t ns = std::min<int32_t>(count, 16);
+    const int32_t ns = std::min<int32_t>(count, 32);
     for (int32_t t = 0; t < ns; ++t) sample.push_back(series[(int64_t)t * count / ns]);
-    const int32_t cand[][2] = {{1024, 8}, {2048, 8}, {4096, 8}, {2048, 6}, {2048, 4}};
+    const int32_t cand[][2] = {{2048, 8}, {1024, 8}, {4096, 8}, {2048, 6}};
     std::vector<dtwj_result> res(ns);
     std::vector<int32_t> st(ns);
     dtwj_options o = opt ? *opt : dtwj_options{};

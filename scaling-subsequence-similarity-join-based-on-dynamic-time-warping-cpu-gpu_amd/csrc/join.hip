@@ -151,6 +151,14 @@ __device__ inline float range_max(const JoinArgs &A, int which, int64_t lo, int6
     return r;
 }
 
+// z-values of a DTW pair normalised on the fly from the staged chunk (x - mean) / sigma.
+struct StagedZ {
+    const float *xi, *xj;
+    float mi, ii, mj, ij;
+    __device__ float row(int r) const { return (xi[r] - mi) * ii; }
+    __device__ float col(int c) const { return (xj[c] - mj) * ij; }
+};
+
 // Packed (row, column) offsets of a pair inside the staged chunk (both < 2^16).
 __device__ inline uint32_t pack_pair(int li, int lj) { return (uint32_t)li | ((uint32_t)lj << 16); }
 __device__ inline int pair_row(uint32_t p) { return (int)(p & 0xffffu); }
@@ -171,14 +179,14 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
     // to the series' counters whenever the wave moves to another series, and at the end.
     __shared__ __align__(16) unsigned long long wcnt[8];  // 64 bytes keep the dynamic area aligned
     if (lane < 8) wcnt[lane] = 0ull;
-    auto count = [&](int which, unsigned long long v) {
+    auto count = [&](int which, unsigned long long v) __attribute__((always_inline)) {
         if (lane == 0) wcnt[which] += v;
     };
     unsigned long long c_kelem = 0;  // per lane: LB_Keogh positions evaluated
 
     int cur = -1;  // series of the current item
     const JoinArgs *Ap = series;
-    auto flush_counters = [&]() {
+    auto flush_counters = [&]() __attribute__((always_inline)) {
         if (cur < 0) return;
         __syncthreads();
         if (lane == 0) {
@@ -204,41 +212,57 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
     uint32_t *plist = reinterpret_cast<uint32_t *>(smem + uni);  // expanded pairs of a batch
     uint32_t *q2 = plist + PLCAP;                                // LB_Keogh queue
     uint32_t *q3 = q2 + Q2CAP;                                   // DTW queue
-    float *zbuf = reinterpret_cast<float *>(q3 + Q3CAP);         // 2m z-values
 
-    // ---- stage 4: DTW of every queued pair, one pair at a time on the whole wave
-    auto process_q3 = [&]() {
+    // ---- stage 4: DTW of every queued pair.  Bands up to 63 run several pairs per wave
+    // (segments of w + 1 lanes); wider bands one pair per wave with QP offset pairs per lane.
+    auto process_q3 = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
-        for (int e = 0; e < q3n; ++e) {
-            const uint32_t pr = q3[e];
-            const int li = pair_row(pr), lj = pair_col(pr);
+        const int w = A.w;
+        const int seg_len = w <= 63 ? w + 1 : 64;
+        const int per_wave = 64 / seg_len;
+        const int seg = lane / seg_len;
+        const bool leader = lane == seg * seg_len;
+        for (int e0 = 0; e0 < q3n; e0 += per_wave) {
+            const int nseg = (q3n - e0) < per_wave ? (q3n - e0) : per_wave;
             th = load_threshold(A);
-            const float mi = rmu[li], ii = ris[li], mj = cmu[lj], ij = cis[lj];
-            for (int k = lane; k < m; k += 64) {
-                zbuf[k] = (sai[li + k] - mi) * ii;
-                zbuf[m + k] = (saj[lj + k] - mj) * ij;
+            int li = 0, lj = 0;
+            float mi = 0.f, ii = 0.f, mj = 0.f, ij = 0.f;
+            if (seg < nseg) {
+                const uint32_t pr = q3[e0 + seg];
+                li = pair_row(pr);
+                lj = pair_col(pr);
+                mi = rmu[li];
+                ii = ris[li];
+                mj = cmu[lj];
+                ij = cis[lj];
             }
-            __syncthreads();
-            int64_t cells;
-            bool ab;
-            const float cost = dtw_wave<float, Q>(zbuf, zbuf + m, m, A.w, th.dtw_ub, cells, ab);
-            count(C_CELLS, (unsigned long long)cells);
-            if (ab) {
-                count(C_DTW_AB, 1ull);
-            } else if (lane == 0) {
+            const StagedZ z{sai + li, saj + lj, mi, ii, mj, ij};
+            int t_stop = 0;
+            float res;
+            if (w <= 63)
+                res = dtw_band<float, 1>(m, w, seg_len, nseg, th.dtw_ub, z, t_stop);
+            else
+                res = dtw_band<float, Q>(m, w, 64, 1, th.dtw_ub, z, t_stop);
+            const bool lead = leader && seg < nseg;
+            const bool ab = lead && !(res <= th.dtw_ub);
+            unsigned long long cells = lead ? (unsigned long long)dtw_cells_upto(m, w, t_stop) : 0ull;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) cells += __shfl_xor(cells, off, 64);
+            count(C_CELLS, cells);
+            count(C_DTW_AB, __popcll(__builtin_amdgcn_ballot_w64(ab)));
+            if (lead && !ab) {
                 const int gi = (int)(c0 + li), gj = (int)(c0 + d0 + lj);
                 int slot = atomicAdd(A.n_cands, 1);
-                if (slot < A.cand_cap) A.cands[slot] = make_int4(gi, gj, __float_as_int(cost), 0);
-                atomicMin(A.bsf, pack_bsf(cost, (uint32_t)gi));
+                if (slot < A.cand_cap) A.cands[slot] = make_int4(gi, gj, __float_as_int(res), 0);
+                atomicMin(A.bsf, pack_bsf(res, (uint32_t)gi));
             }
-            __syncthreads();
         }
         q3n = 0;
     };
 
     // ---- stage 3: two-sided LB_Keogh over the queued pairs, one pair per lane; a lane takes
     // the next pair as soon as its pair is abandoned or complete.  Consumes the whole queue.
-    auto process_q2 = [&]() {
+    auto process_q2 = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
         __syncthreads();  // q2 entries were written by other lanes
         th = load_threshold(A);
@@ -277,17 +301,25 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
             if (taken > q2n) taken = q2n;
             if (__builtin_amdgcn_ballot_w64(act) == 0ull) break;
             if (act) {
-                const int kend = k + 16 < m ? k + 16 : m;
-                c_kelem += (unsigned long long)(kend - k);
-                for (; k < kend; ++k) {
-                    const float x = saj[lj + k];
-                    const float hi = fmaf(sui[li + k], rA, bA), lo = fmaf(sli[li + k], rA, bA);
+                auto term = [&](int kk) __attribute__((always_inline)) {
+                    const float x = saj[lj + kk];
+                    const float hi = fmaf(sui[li + kk], rA, bA), lo = fmaf(sli[li + kk], rA, bA);
                     const float t = fmaxf(fmaxf(x - hi, lo - x), 0.f);
                     accA = fmaf(t, t, accA);
-                    const float y = sai[li + k];
-                    const float hj = fmaf(suj[lj + k], rB, bB), lj_ = fmaf(slj[lj + k], rB, bB);
+                    const float y = sai[li + kk];
+                    const float hj = fmaf(suj[lj + kk], rB, bB), lj_ = fmaf(slj[lj + kk], rB, bB);
                     const float u = fmaxf(fmaxf(y - hj, lj_ - y), 0.f);
                     accB = fmaf(u, u, accB);
+                };
+                if (k + 16 <= m) {
+                    // fixed block: the 96 LDS reads are independent and issue back to back
+#pragma unroll
+                    for (int kk = 0; kk < 16; ++kk) term(k + kk);
+                    k += 16;
+                    c_kelem += 16ull;
+                } else {
+                    c_kelem += (unsigned long long)(m - k);
+                    for (; k < m; ++k) term(k);
                 }
             }
             const bool dead = act && !((accA <= thrA) && (accB <= thrB));
@@ -314,7 +346,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
 
     // ---- drain: stage the chunk in LDS, expand the queue entries into pairs, LB_Kim (stage 2)
     // on 64-lane batches, feeding LB_Keogh and DTW; everything is flushed.
-    auto drain = [&]() {
+    auto drain = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
         const int64_t n = A.n, nw = A.nw;
         const int CI = RC + m, CJ = RC + DG + m;
@@ -469,48 +501,74 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
 
         // ---- exact FP64 anchor of X^A, X^B at row i_beg: for the lane's four consecutive
         // columns j = i_beg + dq0 + q, X^A_q = sum_k (v_{i+k} - muv_i) a_{j+k} and
-        // X^B_q = sum_k (a_{i+k} - mu_i) v_{j+k}.  A column past the triangle (j > nw - 1)
-        // reads clamped positions; its values are never used.  n < 2^31 (checked by the host).
+        // X^B_q = sum_k (a_{i+k} - mu_i) v_{j+k}.  Two passes (X^A, then X^B) over windows staged
+        // in LDS (the union area is free at this point) with coalesced loads: the row window
+        // (m values) and the column span (DG + m + 3 values from i_beg + d0).  Positions past
+        // the series end read as 0; they only reach columns past the triangle, never used.
         float XA[KD], XB[KD];
         {
-            double xa[KD], xb[KD];
+            double *lrow = reinterpret_cast<double *>(smem);
+            double *lcol = lrow + m;
+            const int span = DG + m + KD;
+            const int64_t jb0 = i_beg + d0;
+#pragma unroll 1
+            for (int pass = 0; pass < 2; ++pass) {
+                const global_ptr<double> prow = in_vgprs(pass == 0 ? A.v : A.a);
+                const global_ptr<double> pcol = in_vgprs(pass == 0 ? A.a : A.v);
+                const double c0v = pass == 0 ? A.muv[i_beg] : A.mu[i_beg];
+                __syncthreads();
+                for (int x0 = 0; x0 < m; x0 += 256) {
+                    double t4[4];
 #pragma unroll
-            for (int q = 0; q < KD; ++q) xa[q] = xb[q] = 0.0;
-            const double mv0 = A.muv[i_beg], m0 = A.mu[i_beg];
-            const int ib = (int)i_beg, last = (int)n - 1;
-            const global_ptr<double> pa = in_vgprs(A.a), pv = in_vgprs(A.v);
-            const int jb = (int)(i_beg + dq0);
-            // sliding window over positions jb + k .. jb + k + 3
-            double av[KD], vv[KD];
+                    for (int u = 0; u < 4; ++u) {
+                        const int x = x0 + 64 * u + lane;
+                        t4[u] = x < m ? prow[i_beg + x] - c0v : 0.0;
+                    }
 #pragma unroll
-            for (int q = 0; q < KD - 1; ++q) {
-                const int t = min(jb + q, last);
-                av[q + 1] = pa[t];
-                vv[q + 1] = pv[t];
-            }
-#pragma unroll 4
-            for (int k = 0; k < m; ++k) {
-                const double cA = pv[ib + k] - mv0;
-                const double cB = pa[ib + k] - m0;
-#pragma unroll
-                for (int q = 0; q < KD - 1; ++q) {
-                    av[q] = av[q + 1];
-                    vv[q] = vv[q + 1];
+                    for (int u = 0; u < 4; ++u) {
+                        const int x = x0 + 64 * u + lane;
+                        if (x < m) lrow[x] = t4[u];
+                    }
                 }
-                const int t = min(jb + k + KD - 1, last);
-                av[KD - 1] = pa[t];
-                vv[KD - 1] = pv[t];
+                for (int x0 = 0; x0 < span; x0 += 256) {
+                    double t4[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int64_t p = jb0 + x0 + 64 * u + lane;
+                        t4[u] = (x0 + 64 * u + lane < span && p < n) ? pcol[p] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int x = x0 + 64 * u + lane;
+                        if (x < span) lcol[x] = t4[u];
+                    }
+                }
+                __syncthreads();
+                double acc[KD];
+#pragma unroll
+                for (int q = 0; q < KD; ++q) acc[q] = 0.0;
+                const double *cl = lcol + KD * lane;
+                double win[KD];
+#pragma unroll
+                for (int q = 0; q < KD - 1; ++q) win[q + 1] = cl[q];
+#pragma unroll 8
+                for (int k = 0; k < m; ++k) {
+#pragma unroll
+                    for (int q = 0; q < KD - 1; ++q) win[q] = win[q + 1];
+                    win[KD - 1] = cl[k + KD - 1];
+                    const double ck = lrow[k];
+#pragma unroll
+                    for (int q = 0; q < KD; ++q) acc[q] = fma(ck, win[q], acc[q]);
+                }
 #pragma unroll
                 for (int q = 0; q < KD; ++q) {
-                    xa[q] = fma(cA, av[q], xa[q]);
-                    xb[q] = fma(cB, vv[q], xb[q]);
+                    if (pass == 0)
+                        XA[q] = (float)acc[q];
+                    else
+                        XB[q] = (float)acc[q];
                 }
             }
-#pragma unroll
-            for (int q = 0; q < KD; ++q) {
-                XA[q] = (float)xa[q];
-                XB[q] = (float)xb[q];
-            }
+            __syncthreads();
         }
         // pairs of the item inside the triangle: row i has min(DG, nw - d0 - i) >= 1 columns
         {
@@ -613,7 +671,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 // min/max reduces the four pairs to one compare per row, and a queue entry
                 // (row | lane | mask of the lane's surviving diagonals) is only formed on rows
                 // with a survivor.
-                auto group = [&](auto full_tag) {
+                auto group = [&](auto full_tag) __attribute__((always_inline)) {
                     constexpr bool FULL = decltype(full_tag)::value;
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
@@ -778,8 +836,9 @@ __global__ __launch_bounds__(64) void k_seed(SeedArgs S, int2 *pairs) {
     if (lane == 0) pairs[blockIdx.x] = msk ? make_int2(wi, wj) : make_int2(-1, -1);
 }
 
-// ---- DTW of a list of candidate pairs (seeds); pushes results like the join does.
-template <int Q>
+// ---- DTW of a list of candidate pairs (seeds), one pair per wave; pushes results like the
+// join does.
+template <int QP>
 __global__ __launch_bounds__(64) void k_seed_dtw(const JoinArgs *__restrict__ Ap, const int2 *pairs, int npairs) {
     extern __shared__ float zs[];
     const JoinArgs &A = *Ap;
@@ -796,21 +855,20 @@ __global__ __launch_bounds__(64) void k_seed_dtw(const JoinArgs *__restrict__ Ap
         zs[m + k] = (A.ac[j + k] - mj) * ij;
     }
     __syncthreads();
-    int64_t cells;
-    bool ab;
-    const float cost = dtw_wave<float, Q>(zs, zs + m, m, A.w, th.dtw_ub, cells, ab);
-    if (!ab && lane == 0) {
+    const PlainZ<float> z{zs, zs + m};
+    int t_stop = 0;
+    const float res = dtw_band<float, QP>(m, A.w, 64, 1, th.dtw_ub, z, t_stop);
+    if (res <= th.dtw_ub && lane == 0) {
         int slot = atomicAdd(A.n_cands, 1);
-        if (slot < A.cand_cap) A.cands[slot] = make_int4(i, j, __float_as_int(cost), 0);
-        atomicMin(A.bsf, pack_bsf(cost, (uint32_t)i));
+        if (slot < A.cand_cap) A.cands[slot] = make_int4(i, j, __float_as_int(res), 0);
+        atomicMin(A.bsf, pack_bsf(res, (uint32_t)i));
     }
 }
 
 }  // namespace
 
 size_t join_lds_bytes(int32_t m) {
-    return sizeof(float) * (size_t)union_floats(m) + sizeof(uint32_t) * (PLCAP + Q2CAP + Q3CAP) +
-           sizeof(float) * 2 * (size_t)m;
+    return sizeof(float) * (size_t)union_floats(m) + sizeof(uint32_t) * (PLCAP + Q2CAP + Q3CAP);
 }
 
 int launch_join_setup(JoinArgs *d_series, int32_t nseries, int32_t *d_work, hipStream_t s) {
@@ -821,7 +879,7 @@ int launch_join_setup(JoinArgs *d_series, int32_t nseries, int32_t *d_work, hipS
 
 int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t nseries, int32_t max_m, int32_t max_w,
                 int32_t *d_work, uint32_t *d_queue, int32_t grid, hipStream_t s) {
-    const int Q = dtw_q_for(max_w);
+    const int Q = dtw_qp_for(max_w);
     const size_t lds = join_lds_bytes(max_m);
     const int32_t uni = union_floats(max_m);
     if (lds + 64 > 160 * 1024) {  // + the kernel's static stage counters
@@ -836,7 +894,6 @@ int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t ns
         case 2: DTWJ_LAUNCH_JOIN(2); break;
         case 4: DTWJ_LAUNCH_JOIN(4); break;
         case 8: DTWJ_LAUNCH_JOIN(8); break;
-        case 16: DTWJ_LAUNCH_JOIN(16); break;
         default: set_error("band too wide: 2w+1 must be <= 1024"); return DTWJ_EINVAL;
     }
 #undef DTWJ_LAUNCH_JOIN
@@ -852,14 +909,13 @@ int launch_seed(const SeedArgs &S, int2 *pairs, hipStream_t s) {
 
 int launch_seed_dtw(const JoinArgs *d_args, int32_t m, int32_t w, const int2 *pairs, int32_t npairs, hipStream_t s) {
     if (npairs <= 0) return 0;
-    const int Q = dtw_q_for(w);
+    const int QP = dtw_qp_for(w);
     const size_t lds = sizeof(float) * 2 * (size_t)m;
-    switch (Q) {
+    switch (QP) {
         case 1: hipLaunchKernelGGL(k_seed_dtw<1>, dim3(npairs), dim3(64), lds, s, d_args, pairs, npairs); break;
         case 2: hipLaunchKernelGGL(k_seed_dtw<2>, dim3(npairs), dim3(64), lds, s, d_args, pairs, npairs); break;
         case 4: hipLaunchKernelGGL(k_seed_dtw<4>, dim3(npairs), dim3(64), lds, s, d_args, pairs, npairs); break;
         case 8: hipLaunchKernelGGL(k_seed_dtw<8>, dim3(npairs), dim3(64), lds, s, d_args, pairs, npairs); break;
-        case 16: hipLaunchKernelGGL(k_seed_dtw<16>, dim3(npairs), dim3(64), lds, s, d_args, pairs, npairs); break;
         default: set_error("band too wide: 2w+1 must be <= 1024"); return DTWJ_EINVAL;
     }
     DTWJ_HIP(hipGetLastError());
