@@ -124,6 +124,60 @@ typedef struct dtwj_series {
 int dtwj_top1_batch(const dtwj_series *series, int32_t count, const dtwj_options *opt, dtwj_result *out,
                     int32_t *status);
 
+/* ---- learned proposal ------------------------------------------------------------
+ * No reference counterpart (the reference has no learned component; the nearest analogue
+ * is its heuristic alpha search, CPU_version1/new_dtw_motifGUI.cpp:411-468).  A 40 -> 64 -> 1
+ * MLP (BF16 weights on the GPU, FP32 accumulation) predicts, from PAA-16 features of both
+ * windows and cheap bounds, log(DTW distance / ED distance) of a candidate pair (<= 0: banded
+ * DTW never exceeds the z-normalised Euclidean distance); candidates are ranked by
+ * log ED + prediction, so an untrained (zero) network ranks by ED.  It only orders candidates:
+ * the pairs it ranks first are verified by exact DTW and seed the plain join's best-so-far,
+ * so the answer is the plain join's. */
+#define DTWJ_MLP_IN 40
+#define DTWJ_MLP_HID 64
+
+typedef struct dtwj_mlp {
+    float w1[DTWJ_MLP_HID][DTWJ_MLP_IN]; /* hidden x input */
+    float b1[DTWJ_MLP_HID];
+    float w2[DTWJ_MLP_HID];
+    float b2;
+    float feat_mean[DTWJ_MLP_IN];        /* the network sees (feature - mean) * scale */
+    float feat_scale[DTWJ_MLP_IN];
+} dtwj_mlp;
+
+typedef struct dtwj_proposal_report {
+    int64_t candidates;  /* pairs scored by the network */
+    int32_t verified;    /* top-ranked pairs verified by exact DTW (the join's seeds) */
+    int32_t reserved0;
+    double seed_dist;    /* best verified distance in the network's order */
+    double seed_dist_ed; /* best verified distance when as many pairs are taken in ED order */
+    double regret;       /* seed_dist / final distance - 1 (0: the network found the top-1) */
+    double regret_ed;    /* the same for the ED order */
+    double t_propose;    /* seconds for candidates, features, network and verification */
+    double reserved[3];
+} dtwj_proposal_report;
+
+/* Candidate pairs of the learned proposal and their raw features (count x DTWJ_MLP_IN,
+ * row-major): the closest pair by z-normalised ED of every lane of n_items sampled sweep
+ * items (64 per item).  At most cap pairs are returned. */
+int dtwj_proposal_candidates(const double *series, int64_t n, int32_t m, int32_t w, int32_t n_items, int64_t cap,
+                             int64_t *pi, int64_t *pj, float *features, int64_t *count);
+/* Network output (predicted log(DTW / ED) distance ratio) for count raw feature rows, on the GPU. */
+int dtwj_mlp_forward(const dtwj_mlp *model, const float *features, int64_t count, float *out);
+/* Gradient of the mean squared error over count rows (one full batch on the GPU, the
+ * kernels training uses); fills grad's w1, b1, w2, b2 and *loss. */
+int dtwj_mlp_gradient(const dtwj_mlp *model, const float *features, const float *targets, int64_t count,
+                      dtwj_mlp *grad, double *loss);
+/* Trains model in place on the GPU (Adam, mini-batches of `batch` rows, `epochs` passes,
+ * shuffled with `seed`); model->feat_mean / feat_scale must be set by the caller. */
+int dtwj_mlp_train(dtwj_mlp *model, const float *features, const float *targets, int64_t count, int32_t epochs,
+                   int32_t batch, float lr, uint64_t seed, double *final_loss);
+/* The learned form of the top-1 join: candidates from n_items sampled items are scored by
+ * the network, the n_verify best-ranked are verified by exact DTW and seed the plain join. */
+int dtwj_top1_learned(const double *series, int64_t n, int32_t m, int32_t w, const dtwj_mlp *model,
+                      int32_t n_items, int32_t n_verify, const dtwj_options *opt, dtwj_result *out,
+                      dtwj_proposal_report *report);
+
 /* Reusable device context: keeps device buffers between calls (for repeated joins). */
 typedef struct dtwj_context dtwj_context;
 int dtwj_context_create(int32_t device, dtwj_context **ctx);

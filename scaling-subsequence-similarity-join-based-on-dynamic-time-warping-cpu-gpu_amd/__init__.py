@@ -52,6 +52,39 @@ class Result(ctypes.Structure):
                 ("reserved", ctypes.c_int32 * 5)]
 
 
+MLP_IN, MLP_HID = 40, 64
+
+
+class MLP(ctypes.Structure):
+    """dtwj_mlp: the learned proposal's 40 -> 64 -> 1 network (FP32 master copy)."""
+    _fields_ = [("w1", (ctypes.c_float * MLP_IN) * MLP_HID), ("b1", ctypes.c_float * MLP_HID),
+                ("w2", ctypes.c_float * MLP_HID), ("b2", ctypes.c_float), ("feat_mean", ctypes.c_float * MLP_IN),
+                ("feat_scale", ctypes.c_float * MLP_IN)]
+
+    def to_dict(self):
+        return {"w1": np.ctypeslib.as_array(self.w1).tolist(), "b1": list(self.b1), "w2": list(self.w2),
+                "b2": float(self.b2), "feat_mean": list(self.feat_mean), "feat_scale": list(self.feat_scale)}
+
+    @classmethod
+    def from_dict(cls, d):
+        m = cls()
+        np.ctypeslib.as_array(m.w1)[:] = np.asarray(d["w1"], dtype=np.float32)
+        for name in ("b1", "w2", "feat_mean", "feat_scale"):
+            np.ctypeslib.as_array(getattr(m, name))[:] = np.asarray(d[name], dtype=np.float32)
+        m.b2 = float(d["b2"])
+        return m
+
+    def arrays(self):
+        return (np.ctypeslib.as_array(self.w1).copy(), np.ctypeslib.as_array(self.b1).copy(),
+                np.ctypeslib.as_array(self.w2).copy(), float(self.b2))
+
+
+class ProposalReport(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_int64), ("verified", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("seed_dist", ctypes.c_double), ("seed_dist_ed", ctypes.c_double), ("regret", ctypes.c_double),
+                ("regret_ed", ctypes.c_double), ("t_propose", ctypes.c_double), ("reserved", ctypes.c_double * 3)]
+
+
 class Series(ctypes.Structure):
     _fields_ = [("data", ctypes.POINTER(ctypes.c_double)), ("n", ctypes.c_int64), ("m", ctypes.c_int32),
                 ("w", ctypes.c_int32)]
@@ -70,7 +103,8 @@ EXPORTED = [
     "dtwj_context_destroy", "dtwj_top1_device", "dtwj_context_upload", "dtwj_motif_discovery", "dtwj_load_text",
     "dtwj_free", "dtwj_last_error", "dtwj_device_count", "dtwj_sliding_stats", "dtwj_envelope", "dtwj_dtw_pairs",
     "dtwj_gen_spec_default", "dtwj_generate", "dtwj_top1_batch",
-    "dtwj_context_top1_batch",
+    "dtwj_context_top1_batch", "dtwj_proposal_candidates", "dtwj_mlp_forward", "dtwj_mlp_gradient",
+    "dtwj_mlp_train", "dtwj_top1_learned",
 ]
 
 _lib = None
@@ -109,6 +143,15 @@ def lib():
                                  ctypes.c_int32, d_p]
     L.dtwj_top1_batch.argtypes = [P(Series), ctypes.c_int32, P(Options), P(Result), i32_p]
     L.dtwj_context_top1_batch.argtypes = [ctypes.c_void_p, P(Series), ctypes.c_int32, P(Options), P(Result), i32_p]
+    f_p = P(ctypes.c_float)
+    L.dtwj_proposal_candidates.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_int64, i64_p, i64_p, f_p, i64_p]
+    L.dtwj_mlp_forward.argtypes = [P(MLP), f_p, ctypes.c_int64, f_p]
+    L.dtwj_mlp_gradient.argtypes = [P(MLP), f_p, f_p, ctypes.c_int64, P(MLP), d_p]
+    L.dtwj_mlp_train.argtypes = [P(MLP), f_p, f_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                 ctypes.c_uint64, d_p]
+    L.dtwj_top1_learned.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, P(MLP), ctypes.c_int32,
+                                    ctypes.c_int32, P(Options), P(Result), P(ProposalReport)]
     L.dtwj_gen_spec_default.argtypes = [P(GenSpec)]
     L.dtwj_generate.argtypes = [P(GenSpec), d_p, i64_p, i64_p]
     _lib = L
@@ -231,6 +274,123 @@ def top1_batch(series_list, ms, ws, **opts):
     arrs, items, out, status = _batch_args(series_list, ms, ws)
     _check(lib().dtwj_top1_batch(items, len(arrs), ctypes.byref(make_options(**opts)), out, status))
     return _batch_results(out, status)
+
+
+# ---- learned proposal ---------------------------------------------------------------
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def proposal_candidates(series, m, w, n_items=4096):
+    """Candidate pairs (closest ED pair of every lane of n_items sampled sweep items) and their
+    raw features, computed on the GPU.  Returns (pi, pj, features[count, 40])."""
+    a = _as_series(series)
+    cap = 64 * max(int(n_items), 1)
+    pi = np.zeros(cap, dtype=np.int64)
+    pj = np.zeros(cap, dtype=np.int64)
+    feats = np.zeros((cap, MLP_IN), dtype=np.float32)
+    cnt = ctypes.c_int64()
+    _check(lib().dtwj_proposal_candidates(_dptr(a), a.size, int(m), int(w), int(n_items), cap,
+                                          pi.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                          pj.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _fptr(feats),
+                                          ctypes.byref(cnt)))
+    k = cnt.value
+    return pi[:k].copy(), pj[:k].copy(), feats[:k].copy()
+
+
+def _feats(x):
+    f = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    if f.ndim != 2 or f.shape[1] != MLP_IN:
+        raise ValueError("features must be (count, %d)" % MLP_IN)
+    return f
+
+
+def proposal_targets(features, cost, m):
+    """Training target of the network for candidates with exact squared DTW cost `cost`:
+    log(DTW distance / ED distance), ED^2 = m expm1(feature 36)."""
+    f = np.asarray(features, dtype=np.float64)
+    ed2 = m * np.expm1(f[:, 36])
+    return (0.5 * (np.log(np.maximum(cost, 1e-30)) - np.log(np.maximum(ed2, 1e-30)))).astype(np.float32)
+
+
+def proposal_rank_key(features, y, m):
+    """Ranking key of the learned proposal: log ED distance + network output."""
+    f = np.asarray(features, dtype=np.float64)
+    return 0.5 * np.log(m * np.expm1(f[:, 36]) + 1e-30) + np.asarray(y, dtype=np.float64)
+
+
+def mlp_forward(model, features):
+    """Network output (predicted log(DTW / ED) distance ratio) of every feature row, on the GPU."""
+    f = _feats(features)
+    out = np.zeros(f.shape[0], dtype=np.float32)
+    _check(lib().dtwj_mlp_forward(ctypes.byref(model), _fptr(f), f.shape[0], _fptr(out)))
+    return out
+
+
+def mlp_gradient(model, features, targets):
+    """Mean-squared-error gradient over all rows in one GPU pass (the training kernels).
+    Returns (grad MLP, loss)."""
+    f = _feats(features)
+    t = np.ascontiguousarray(np.asarray(targets, dtype=np.float32))
+    g = MLP()
+    loss = ctypes.c_double()
+    _check(lib().dtwj_mlp_gradient(ctypes.byref(model), _fptr(f), _fptr(t), f.shape[0], ctypes.byref(g),
+                                   ctypes.byref(loss)))
+    return g, loss.value
+
+
+def mlp_train(model, features, targets, epochs=4, batch=4096, lr=1e-3, seed=0):
+    """Trains model in place on the GPU (Adam); returns the final full-batch loss."""
+    f = _feats(features)
+    t = np.ascontiguousarray(np.asarray(targets, dtype=np.float32))
+    loss = ctypes.c_double()
+    _check(lib().dtwj_mlp_train(ctypes.byref(model), _fptr(f), _fptr(t), f.shape[0], int(epochs), int(batch),
+                                float(lr), int(seed), ctypes.byref(loss)))
+    return loss.value
+
+
+def mlp_init(features, seed=0, targets=None):
+    """A fresh network whose feature normalisation is fitted to `features`; with `targets`,
+    the output starts near their mean (small output weights, bias = mean)."""
+    f = _feats(features).astype(np.float64)
+    m = MLP()
+    rng = np.random.default_rng(seed)
+    np.ctypeslib.as_array(m.w1)[:] = (rng.standard_normal((MLP_HID, MLP_IN)) * np.sqrt(2.0 / MLP_IN)).astype(np.float32)
+    w2_scale = np.sqrt(1.0 / MLP_HID) * (0.05 if targets is not None else 1.0)
+    np.ctypeslib.as_array(m.w2)[:] = (rng.standard_normal(MLP_HID) * w2_scale).astype(np.float32)
+    if targets is not None:
+        m.b2 = float(np.mean(targets))
+    mean = f.mean(axis=0)
+    std = f.std(axis=0)
+    np.ctypeslib.as_array(m.feat_mean)[:] = mean.astype(np.float32)
+    np.ctypeslib.as_array(m.feat_scale)[:] = (1.0 / np.where(std > 1e-12, std, 1.0)).astype(np.float32)
+    return m
+
+
+def load_mlp(path):
+    import json
+    with open(path) as fh:
+        return MLP.from_dict(json.load(fh))
+
+
+def save_mlp(model, path):
+    import json
+    with open(path, "w") as fh:
+        json.dump(model.to_dict(), fh)
+
+
+def top1_learned(series, m, w, model, n_items=4096, n_verify=1024, **opts):
+    """The learned form of the join: the network orders candidate pairs, the n_verify best are
+    verified by exact DTW and seed the plain join (whose answer is returned).  Returns
+    (Top1, report dict)."""
+    a = _as_series(series)
+    r = Result()
+    rep = ProposalReport()
+    _check(lib().dtwj_top1_learned(_dptr(a), a.size, int(m), int(w), ctypes.byref(model), int(n_items),
+                                   int(n_verify), ctypes.byref(make_options(**opts)), ctypes.byref(r),
+                                   ctypes.byref(rep)))
+    report = {k: getattr(rep, k) for k, _ in ProposalReport._fields_ if not k.startswith("reserved")}
+    return _result(r), report
 
 
 def top1_seeded(series, m, w, mp_ed, mp_idx_1based, **opts):

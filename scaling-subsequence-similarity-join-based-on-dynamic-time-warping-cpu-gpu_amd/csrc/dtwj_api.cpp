@@ -8,6 +8,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <numeric>
+#include <random>
 #include <string>
 #include <utility>
 #include <vector>
@@ -454,6 +456,183 @@ int ensure_mem(dtwj_context *c, size_t need) {
     return 0;
 }
 
+
+// ---------------------------------------------------------------- learned proposal
+// Device buffers of one proposal pass (candidates, raw features, network input and output).
+struct Proposal {
+    Job J;
+    int2 *d_cand = nullptr;  // n_seed * 64 (scan output), then the de-duplicated list
+    float *d_F = nullptr;    // count x DTWJ_MLP_IN raw features
+    uint16_t *d_X = nullptr; // rows16 x pad BF16 network input
+    float *d_P = nullptr;    // network parameters (FP32)
+    uint16_t *d_W1b = nullptr;
+    float *d_y = nullptr;    // network output
+    float *d_norm = nullptr; // feature mean, scale
+    int64_t *d_vi = nullptr, *d_vj = nullptr;
+    double *d_vcost = nullptr;
+    std::vector<int2> cands;  // de-duplicated candidate pairs (host)
+};
+
+// Statistics, candidate scan and raw features of one series (device-resident series d_a).
+int propose(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w, int32_t n_items, int32_t n_verify,
+            Proposal &Pr) {
+    Job &J = Pr.J;
+    plan_job(J, d_a, n, m, w, 0, n_items > 0 ? n_items : 4096, true, {}, 1);
+    const size_t ncand = (size_t)J.n_seed * 64;
+    const size_t rows16 = (ncand + 15) / 16 * 16;
+    const int np = mlp_param_count(), pad = mlp_feature_pad();
+    const int nv = std::max(n_verify, 1) * 2;
+    auto carve = [&](Arena &ar) {
+        take_job(ar, J);
+        Pr.d_cand = ar.take<int2>(ncand + 1);
+        Pr.d_F = ar.take<float>(ncand * DTWJ_MLP_IN + 1);
+        Pr.d_X = ar.take<uint16_t>(rows16 * pad + 16);
+        Pr.d_P = ar.take<float>(np);
+        Pr.d_W1b = ar.take<uint16_t>((size_t)DTWJ_MLP_HID * pad);
+        Pr.d_y = ar.take<float>(rows16 + 16);
+        Pr.d_norm = ar.take<float>(2 * DTWJ_MLP_IN);
+        Pr.d_vi = ar.take<int64_t>(nv);
+        Pr.d_vj = ar.take<int64_t>(nv);
+        Pr.d_vcost = ar.take<double>(nv);
+    };
+    Arena dry;
+    carve(dry);
+    if (int rc = ensure_mem(c, dry.off + 256)) return rc;
+    Arena ar;
+    ar.base = (char *)c->mem;
+    ar.cap = c->mem_cap;
+    carve(ar);
+    hipStream_t s = c->stream;
+    if (int rc = upload_job(J, s)) return rc;
+    if (int rc = prep_series(J.P, s)) return rc;
+    int32_t flags[4];
+    DTWJ_HIP(hipMemcpyAsync(flags, J.P.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    if (flags[0]) return fail(DTWJ_EINVAL, "series contains NaN or Inf");
+    if (flags[1]) return fail(DTWJ_EINVAL, "a window of length m is constant (standard deviation 0)");
+    SeedArgs S;
+    S.a = J.d_a;
+    S.mu = J.P.mu;
+    S.win = J.P.win;
+    S.n = J.n;
+    S.nw = J.nw;
+    S.m = J.m;
+    S.item_start = J.d_item_start;
+    S.groups = J.groups;
+    S.n_items = J.n_items;
+    S.rows_per_item = J.rows_per_item;
+    S.rows = std::min(J.rows_per_item, 512);
+    S.n_seed = J.n_seed;
+    if (int rc = launch_seed_lanes(S, Pr.d_cand, s)) return rc;
+    std::vector<int2> raw(ncand);
+    DTWJ_HIP(hipMemcpyAsync(raw.data(), Pr.d_cand, sizeof(int2) * ncand, hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    Pr.cands.clear();
+    for (const int2 &p : raw)
+        if (p.x >= 0) Pr.cands.push_back(p);
+    std::sort(Pr.cands.begin(), Pr.cands.end(),
+              [](const int2 &a, const int2 &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+    Pr.cands.erase(std::unique(Pr.cands.begin(), Pr.cands.end(),
+                               [](const int2 &a, const int2 &b) { return a.x == b.x && a.y == b.y; }),
+                   Pr.cands.end());
+    const int cnt = (int)Pr.cands.size();
+    if (cnt > 0) {
+        DTWJ_HIP(hipMemcpyAsync(Pr.d_cand, Pr.cands.data(), sizeof(int2) * cnt, hipMemcpyHostToDevice, s));
+        if (int rc = launch_prop_features(J.A, Pr.d_cand, cnt, Pr.d_F, s)) return rc;
+    }
+    return DTWJ_OK;
+}
+
+// Network output for the first `count` rows of d_F.
+int score(dtwj_context *c, const dtwj_mlp &M, const float *d_F, int count, Proposal &Pr) {
+    hipStream_t s = c->stream;
+    const int rows16 = (count + 15) / 16 * 16;
+    std::vector<float> P(mlp_param_count()), norm(2 * DTWJ_MLP_IN);
+    mlp_to_params(M, P.data());
+    for (int f = 0; f < DTWJ_MLP_IN; ++f) {
+        norm[f] = M.feat_mean[f];
+        norm[DTWJ_MLP_IN + f] = M.feat_scale[f];
+    }
+    DTWJ_HIP(hipMemcpyAsync(Pr.d_P, P.data(), sizeof(float) * P.size(), hipMemcpyHostToDevice, s));
+    DTWJ_HIP(hipMemcpyAsync(Pr.d_norm, norm.data(), sizeof(float) * norm.size(), hipMemcpyHostToDevice, s));
+    if (int rc = launch_params_to_bf16(Pr.d_P, Pr.d_W1b, s)) return rc;
+    if (int rc = launch_prop_pack(d_F, count, rows16, Pr.d_norm, Pr.d_norm + DTWJ_MLP_IN, Pr.d_X, s)) return rc;
+    return launch_mlp_forward(Pr.d_X, count, Pr.d_W1b, Pr.d_P, Pr.d_y, s);
+}
+
+// Exact (FP64) DTW cost of a pair list of the job's series.
+int verify_pairs(dtwj_context *c, Job &J, const std::vector<int2> &pairs, Proposal &Pr, std::vector<double> &cost) {
+    hipStream_t s = c->stream;
+    const int64_t np = (int64_t)pairs.size();
+    cost.assign(np, 0.0);
+    if (np == 0) return DTWJ_OK;
+    std::vector<int64_t> vi(np), vj(np);
+    for (int64_t k = 0; k < np; ++k) {
+        vi[k] = pairs[k].x;
+        vj[k] = pairs[k].y;
+    }
+    DTWJ_HIP(hipMemcpyAsync(Pr.d_vi, vi.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice, s));
+    DTWJ_HIP(hipMemcpyAsync(Pr.d_vj, vj.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice, s));
+    if (int rc = launch_dtw_pairs(J.A, J.P.sig, Pr.d_vi, Pr.d_vj, np, 64, Pr.d_vcost, s)) return rc;
+    DTWJ_HIP(hipMemcpyAsync(cost.data(), Pr.d_vcost, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    return DTWJ_OK;
+}
+
+// Device buffers of the network on their own (forward, gradient, training).
+struct MlpBuffers {
+    float *F = nullptr, *T = nullptr, *P = nullptr, *G = nullptr, *M1 = nullptr, *M2 = nullptr, *loss = nullptr;
+    float *y = nullptr, *norm = nullptr;
+    uint16_t *X = nullptr, *W1b = nullptr;
+    int32_t *perm = nullptr;
+};
+
+int mlp_setup(dtwj_context *c, const dtwj_mlp &M, const float *feats, const float *targets, int64_t count,
+              MlpBuffers &B) {
+    const int np = mlp_param_count(), pad = mlp_feature_pad();
+    const size_t rows16 = ((size_t)count + 15) / 16 * 16;
+    auto carve = [&](Arena &ar) {
+        B.F = ar.take<float>((size_t)count * DTWJ_MLP_IN + 1);
+        B.T = ar.take<float>((size_t)count + 1);
+        B.P = ar.take<float>(np);
+        B.G = ar.take<float>(np + 1);
+        B.M1 = ar.take<float>(np);
+        B.M2 = ar.take<float>(np);
+        B.loss = ar.take<float>(1);
+        B.y = ar.take<float>(rows16 + 16);
+        B.norm = ar.take<float>(2 * DTWJ_MLP_IN);
+        B.X = ar.take<uint16_t>(rows16 * pad + 16);
+        B.W1b = ar.take<uint16_t>((size_t)DTWJ_MLP_HID * pad);
+        B.perm = ar.take<int32_t>((size_t)count + 1);
+    };
+    Arena dry;
+    carve(dry);
+    if (int rc = ensure_mem(c, dry.off + 256)) return rc;
+    Arena ar;
+    ar.base = (char *)c->mem;
+    ar.cap = c->mem_cap;
+    carve(ar);
+    hipStream_t s = c->stream;
+    std::vector<float> P(np), norm(2 * DTWJ_MLP_IN);
+    mlp_to_params(M, P.data());
+    for (int f = 0; f < DTWJ_MLP_IN; ++f) {
+        norm[f] = M.feat_mean[f];
+        norm[DTWJ_MLP_IN + f] = M.feat_scale[f];
+    }
+    DTWJ_HIP(hipMemcpyAsync(B.F, feats, sizeof(float) * count * DTWJ_MLP_IN, hipMemcpyHostToDevice, s));
+    if (targets) DTWJ_HIP(hipMemcpyAsync(B.T, targets, sizeof(float) * count, hipMemcpyHostToDevice, s));
+    DTWJ_HIP(hipMemcpyAsync(B.P, P.data(), sizeof(float) * np, hipMemcpyHostToDevice, s));
+    DTWJ_HIP(hipMemcpyAsync(B.norm, norm.data(), sizeof(float) * norm.size(), hipMemcpyHostToDevice, s));
+    if (int rc = launch_params_to_bf16(B.P, B.W1b, s)) return rc;
+    return launch_prop_pack(B.F, (int)count, (int)rows16, B.norm, B.norm + DTWJ_MLP_IN, B.X, s);
+}
+
+int check_mlp_args(const dtwj_mlp *model, const float *feats, int64_t count) {
+    if (!model || !feats) return fail(DTWJ_EINVAL, "null model or features");
+    if (count <= 0 || count > (int64_t)1 << 28) return fail(DTWJ_EINVAL, "feature row count out of range");
+    return DTWJ_OK;
+}
+
 struct Seeds {
     std::vector<int2> pairs;  // explicit seed pairs (mp-seeded entry point)
     bool gpu_scan = true;
@@ -835,6 +1014,210 @@ int dtwj_top1_batch(const dtwj_series *series, int32_t count, const dtwj_options
     dtwj_context *c = nullptr;
     if (int rc = with_context(opt ? opt->device : 0, &c)) return rc;
     int rc = dtwj_context_top1_batch(c, series, count, opt, out, status);
+    dtwj_context_destroy(c);
+    return rc;
+}
+
+
+int dtwj_proposal_candidates(const double *series, int64_t n, int32_t m, int32_t w, int32_t n_items, int64_t cap,
+                             int64_t *pi, int64_t *pj, float *features, int64_t *count) {
+    if (!pi || !pj || !features || !count || cap <= 0) return fail(DTWJ_EINVAL, "bad output arguments");
+    if (int rc = validate(series, n, m, w)) return rc;
+    dtwj_context *c = nullptr;
+    if (int rc = with_context(0, &c)) return rc;
+    const double *d = nullptr;
+    Proposal Pr;
+    int rc = dtwj_context_upload(c, series, n, &d);
+    if (rc == DTWJ_OK) rc = propose(c, d, n, m, w, n_items, 1, Pr);
+    if (rc == DTWJ_OK) {
+        const int64_t cnt = std::min<int64_t>((int64_t)Pr.cands.size(), cap);
+        for (int64_t k = 0; k < cnt; ++k) {
+            pi[k] = Pr.cands[k].x;
+            pj[k] = Pr.cands[k].y;
+        }
+        if (cnt > 0) {
+            // the features were computed on the context's (non-blocking) stream
+            hipError_t e = hipMemcpyAsync(features, Pr.d_F, sizeof(float) * cnt * DTWJ_MLP_IN, hipMemcpyDeviceToHost,
+                                          c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) rc = hip_fail(e, "features", __FILE__, __LINE__);
+        }
+        *count = cnt;
+    }
+    dtwj_context_destroy(c);
+    return rc;
+}
+
+int dtwj_mlp_forward(const dtwj_mlp *model, const float *features, int64_t count, float *out) {
+    if (int rc = check_mlp_args(model, features, count)) return rc;
+    if (!out) return fail(DTWJ_EINVAL, "null output");
+    dtwj_context *c = nullptr;
+    if (int rc = with_context(0, &c)) return rc;
+    MlpBuffers B;
+    int rc = mlp_setup(c, *model, features, nullptr, count, B);
+    if (rc == DTWJ_OK) rc = launch_mlp_forward(B.X, (int)count, B.W1b, B.P, B.y, c->stream);
+    if (rc == DTWJ_OK) {
+        hipError_t e = hipMemcpyAsync(out, B.y, sizeof(float) * count, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "mlp forward", __FILE__, __LINE__);
+    }
+    dtwj_context_destroy(c);
+    return rc;
+}
+
+int dtwj_mlp_gradient(const dtwj_mlp *model, const float *features, const float *targets, int64_t count,
+                      dtwj_mlp *grad, double *loss) {
+    if (int rc = check_mlp_args(model, features, count)) return rc;
+    if (!targets || !grad) return fail(DTWJ_EINVAL, "null targets or gradient");
+    dtwj_context *c = nullptr;
+    if (int rc = with_context(0, &c)) return rc;
+    MlpBuffers B;
+    int rc = mlp_setup(c, *model, features, targets, count, B);
+    if (rc == DTWJ_OK) rc = launch_mlp_grad(B.X, B.T, nullptr, (int)count, B.W1b, B.P, B.G, B.loss, c->stream);
+    if (rc == DTWJ_OK) {
+        std::vector<float> G(mlp_param_count());
+        float l = 0.f;
+        hipError_t e = hipMemcpyAsync(G.data(), B.G, sizeof(float) * G.size(), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&l, B.loss, sizeof(float), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "mlp gradient", __FILE__, __LINE__);
+        } else {
+            std::memset(grad, 0, sizeof(*grad));
+            params_to_mlp(G.data(), *grad);
+            if (loss) *loss = l;
+        }
+    }
+    dtwj_context_destroy(c);
+    return rc;
+}
+
+int dtwj_mlp_train(dtwj_mlp *model, const float *features, const float *targets, int64_t count, int32_t epochs,
+                   int32_t batch, float lr, uint64_t seed, double *final_loss) {
+    if (int rc = check_mlp_args(model, features, count)) return rc;
+    if (!targets || epochs <= 0 || batch <= 0 || !(lr > 0.f)) return fail(DTWJ_EINVAL, "bad training arguments");
+    dtwj_context *c = nullptr;
+    if (int rc = with_context(0, &c)) return rc;
+    hipStream_t s = c->stream;
+    MlpBuffers B;
+    int rc = mlp_setup(c, *model, features, targets, count, B);
+    const int np = mlp_param_count();
+    std::vector<int32_t> perm((size_t)count);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::mt19937_64 rng(seed);
+    int step = 0;
+    if (rc == DTWJ_OK) {
+        hipError_t e = hipMemsetAsync(B.M1, 0, sizeof(float) * np, s);
+        if (e == hipSuccess) e = hipMemsetAsync(B.M2, 0, sizeof(float) * np, s);
+        if (e != hipSuccess) rc = hip_fail(e, "adam state", __FILE__, __LINE__);
+    }
+    for (int32_t ep = 0; ep < epochs && rc == DTWJ_OK; ++ep) {
+        std::shuffle(perm.begin(), perm.end(), rng);
+        hipError_t e = hipMemcpyAsync(B.perm, perm.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "permutation", __FILE__, __LINE__);
+            break;
+        }
+        for (int64_t b0 = 0; b0 < count && rc == DTWJ_OK; b0 += batch) {
+            const int bsz = (int)std::min<int64_t>(batch, count - b0);
+            rc = launch_mlp_grad(B.X, B.T, B.perm + b0, bsz, B.W1b, B.P, B.G, B.loss, s);
+            if (rc == DTWJ_OK) rc = launch_mlp_adam(B.P, B.G, B.M1, B.M2, lr, ++step, B.W1b, s);
+        }
+        // the host copy of the permutation is reused: wait before the next shuffle
+        if (rc == DTWJ_OK) {
+            e = hipStreamSynchronize(s);
+            if (e != hipSuccess) rc = hip_fail(e, "training epoch", __FILE__, __LINE__);
+        }
+    }
+    if (rc == DTWJ_OK) rc = launch_mlp_grad(B.X, B.T, nullptr, (int)count, B.W1b, B.P, B.G, B.loss, s);
+    if (rc == DTWJ_OK) {
+        std::vector<float> P(np);
+        float l = 0.f;
+        hipError_t e = hipMemcpyAsync(P.data(), B.P, sizeof(float) * np, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(&l, B.loss, sizeof(float), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "trained parameters", __FILE__, __LINE__);
+        } else {
+            params_to_mlp(P.data(), *model);
+            if (final_loss) *final_loss = l;
+        }
+    }
+    dtwj_context_destroy(c);
+    return rc;
+}
+
+int dtwj_top1_learned(const double *series, int64_t n, int32_t m, int32_t w, const dtwj_mlp *model,
+                      int32_t n_items, int32_t n_verify, const dtwj_options *opt, dtwj_result *out,
+                      dtwj_proposal_report *report) {
+    const auto te = clk::now();
+    if (!model || !out) return fail(DTWJ_EINVAL, "null model or result");
+    if (int rc = validate(series, n, m, w)) return rc;
+    for (int64_t t = 0; t < n; ++t)
+        if (!std::isfinite(series[t])) return fail(DTWJ_EINVAL, "series contains NaN or Inf");
+    if (n_verify <= 0) n_verify = 1024;
+    dtwj_context *c = nullptr;
+    if (int rc = with_context(opt ? opt->device : 0, &c)) return rc;
+    const double *d = nullptr;
+    Proposal Pr;
+    dtwj_proposal_report rep;
+    std::memset(&rep, 0, sizeof(rep));
+    int rc = dtwj_context_upload(c, series, n, &d);
+    if (rc == DTWJ_OK) rc = propose(c, d, n, m, w, n_items, n_verify, Pr);
+    const int cnt = (int)Pr.cands.size();
+    std::vector<int2> learned, by_ed;
+    if (rc == DTWJ_OK && cnt > 0) {
+        rc = score(c, *model, Pr.d_F, cnt, Pr);
+        std::vector<float> y(cnt), ed(cnt);
+        if (rc == DTWJ_OK) {
+            hipStream_t s = c->stream;
+            hipError_t e = hipMemcpyAsync(y.data(), Pr.d_y, sizeof(float) * cnt, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess)
+                e = hipMemcpy2DAsync(ed.data(), sizeof(float), Pr.d_F + 36, sizeof(float) * DTWJ_MLP_IN, sizeof(float),
+                                     cnt, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) rc = hip_fail(e, "proposal scores", __FILE__, __LINE__);
+        }
+        if (rc == DTWJ_OK) {
+            // rank: log ED distance + predicted log(DTW / ED); ED^2 = m expm1(feature 36)
+            std::vector<float> key(cnt);
+            for (int k = 0; k < cnt; ++k) key[k] = y[k] + 0.5f * std::log((float)m * std::expm1(ed[k]) + 1e-30f);
+            const int take = std::min(n_verify, cnt);
+            std::vector<int> order(cnt);
+            std::iota(order.begin(), order.end(), 0);
+            std::partial_sort(order.begin(), order.begin() + take, order.end(),
+                              [&](int a, int b) { return key[a] < key[b]; });
+            for (int k = 0; k < take; ++k) learned.push_back(Pr.cands[order[k]]);
+            std::iota(order.begin(), order.end(), 0);
+            std::partial_sort(order.begin(), order.begin() + take, order.end(),
+                              [&](int a, int b) { return ed[a] < ed[b]; });
+            for (int k = 0; k < take; ++k) by_ed.push_back(Pr.cands[order[k]]);
+            std::vector<double> cl, ce;
+            rc = verify_pairs(c, Pr.J, learned, Pr, cl);
+            if (rc == DTWJ_OK) rc = verify_pairs(c, Pr.J, by_ed, Pr, ce);
+            if (rc == DTWJ_OK) {
+                rep.seed_dist = std::sqrt(*std::min_element(cl.begin(), cl.end()));
+                rep.seed_dist_ed = std::sqrt(*std::min_element(ce.begin(), ce.end()));
+            }
+        }
+    }
+    rep.candidates = cnt;
+    rep.verified = (int32_t)learned.size();
+    rep.t_propose = secs(te, clk::now());
+    if (rc == DTWJ_OK) {
+        // the plain join, seeded with the verified proposals (its own ED scan replaced)
+        Seeds seeds;
+        seeds.pairs = learned;
+        seeds.gpu_scan = learned.empty();
+        rc = run_join(c, d, n, m, w, opt, seeds, out, te);
+    }
+    if (rc == DTWJ_OK) {
+        if (rep.seed_dist > 0) {
+            rep.regret = rep.seed_dist / out->dist - 1.0;
+            rep.regret_ed = rep.seed_dist_ed / out->dist - 1.0;
+        }
+        if (report) *report = rep;
+    }
     dtwj_context_destroy(c);
     return rc;
 }

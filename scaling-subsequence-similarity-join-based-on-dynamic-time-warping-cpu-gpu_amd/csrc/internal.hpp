@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "../../include/dtwj.h"
 
 namespace dtwj {
 
@@ -36,6 +37,8 @@ int prep_series(Prep &P, hipStream_t s);
 // Seed scan (join.hip): z-normalised ED on a sample of work items; pairs[k] = closest pair
 // of seed item k, or (-1, -1); pairs[n_seed] = (0, m).
 int launch_seed(const SeedArgs &args, int2 *pairs, hipStream_t s);
+// Same scan, every lane reporting the closest pair of its four diagonals: pairs[64 k + lane].
+int launch_seed_lanes(const SeedArgs &args, int2 *pairs, hipStream_t s);
 // DTW of a pair list (seeds) of the series d_args (device); atomicMin into its best-so-far and
 // pushes candidates like the join.
 int launch_seed_dtw(const JoinArgs *d_args, int32_t m, int32_t w, const int2 *pairs, int32_t npairs, hipStream_t s);
@@ -51,5 +54,19 @@ size_t join_lds_bytes(int32_t m);
 // series / statistics pointers and m, w of args (a host copy).
 int launch_dtw_pairs(const JoinArgs &args, const double *sig64, const int64_t *pi, const int64_t *pj,
                      int64_t npairs, int32_t prec, double *cost, hipStream_t s);
+
+// Learned proposal (proposal.hip).
+int mlp_param_count();
+int mlp_feature_pad();
+void mlp_to_params(const dtwj_mlp &M, float *P);
+void params_to_mlp(const float *P, dtwj_mlp &M);
+int launch_prop_features(const JoinArgs &A, const int2 *cand, int count, float *F, hipStream_t s);
+int launch_prop_pack(const float *F, int count, int rows, const float *mean, const float *scale, uint16_t *X,
+                     hipStream_t s);
+int launch_mlp_forward(const uint16_t *X, int count, const uint16_t *W1b, const float *P, float *out, hipStream_t s);
+int launch_mlp_grad(const uint16_t *X, const float *target, const int32_t *idx, int bsz, const uint16_t *W1b,
+                    const float *P, float *G, float *loss, hipStream_t s);
+int launch_mlp_adam(float *P, const float *G, float *M1, float *M2, float lr, int step, uint16_t *W1b, hipStream_t s);
+int launch_params_to_bf16(const float *P, uint16_t *W1b, hipStream_t s);
 
 }  // namespace dtwj

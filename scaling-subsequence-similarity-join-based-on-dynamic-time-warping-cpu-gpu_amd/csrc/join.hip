@@ -772,13 +772,15 @@ __global__ void k_join_setup(JoinArgs *series, int32_t nseries, int32_t *work_co
 }
 
 // ---- seed scan: z-normalised Euclidean distance (an upper bound of banded DTW) over a
-// sample of work items; each wave reports its closest pair.  Block n_seed writes the fixed
-// seed pair (0, m).
+// sample of work items; each wave reports its closest pair (PER_LANE: every lane reports the
+// closest pair of its own four diagonals, the candidate source of the learned proposal).
+// Block n_seed writes the fixed seed pair (0, m).
+template <bool PER_LANE>
 __global__ __launch_bounds__(64) void k_seed(SeedArgs S, int2 *pairs) {
     const int lane = (int)threadIdx.x;
     const int m = S.m;
     if ((int)blockIdx.x == S.n_seed) {
-        if (lane == 0) pairs[blockIdx.x] = make_int2(0, m);
+        if (lane == 0 && !PER_LANE) pairs[blockIdx.x] = make_int2(0, m);
         return;
     }
     const int64_t nw = S.nw;
@@ -823,6 +825,10 @@ __global__ __launch_bounds__(64) void k_seed(SeedArgs S, int2 *pairs) {
                 C[q] = C[q] + rj.dfa * ri.dga + ri.dfa * rj.dga;
             }
         }
+    }
+    if (PER_LANE) {
+        pairs[(size_t)blockIdx.x * 64 + lane] = make_int2(bi, bj);
+        return;
     }
     // wave argmin
     float v = best;
@@ -902,7 +908,13 @@ int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t ns
 }
 
 int launch_seed(const SeedArgs &S, int2 *pairs, hipStream_t s) {
-    hipLaunchKernelGGL(k_seed, dim3(S.n_seed + 1), dim3(64), 0, s, S, pairs);
+    hipLaunchKernelGGL(k_seed<false>, dim3(S.n_seed + 1), dim3(64), 0, s, S, pairs);
+    DTWJ_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_seed_lanes(const SeedArgs &S, int2 *pairs, hipStream_t s) {
+    hipLaunchKernelGGL(k_seed<true>, dim3(S.n_seed), dim3(64), 0, s, S, pairs);
     DTWJ_HIP(hipGetLastError());
     return 0;
 }
