@@ -12,7 +12,8 @@ of the launch (pairs swept x 12, ball survivors x 84, LB_Keogh positions x 20, D
 / its HIP-event time, peak 157.3 TFLOP/s.  traffic = HBM bytes per launch from the rocprofv3
 FETCH_SIZE / WRITE_SIZE summary under profiles/ for the same workload (null otherwise).
 A second workload (band ratio 0.05: seed 3, 5 plants, w = 13) is timed beside the headline one
-and reported under "wide_band".
+and reported under "wide_band"; the same workload with the learned proposal against the plain
+join under "learned"; the 512-series batched form under "batched".
 cpu_baseline: the repository's FP64 CPU restatement of the reference join (oracle/, the
 "port"), single-threaded, on a bounded prefix of the same series, extrapolated by pair count.
 """
@@ -69,6 +70,74 @@ def run_workload(d, n, m, w, seed, plants, steps, warmup, opts):
     return series, results, walls, planted
 
 
+def learned_leg(d, n, m, steps, warmup):
+    """Learned proposal vs the plain GPU join on the band-ratio-0.05 workload (seed 3, 5 planted
+    motifs, w = 13), both from host memory, with the committed trained network."""
+    w = 13
+    series, _, _ = d.generate(n, 3, n_plants=5, plant_len=m)
+    model = d.load_mlp(os.path.join(PKG_DIR, "models", "proposal_m256_w13.json"))
+    runs_p, runs_l = [], []
+    for k in range(warmup + steps):
+        t0 = time.perf_counter()
+        rp = d.top1(series, m, w)
+        tp = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        rl, rep = d.top1_learned(series, m, w, model, n_items=4096, n_verify=1024)
+        tl = time.perf_counter() - t0
+        if k >= warmup:
+            runs_p.append((tp, rp))
+            runs_l.append((tl, rl, rep))
+    rp, rl, rep = runs_p[-1][1], runs_l[-1][1], runs_l[-1][2]
+
+    def rates(r):
+        return {"ball": r.pass_ball / r.pairs_swept, "kim": r.pass_kim / r.pairs_swept,
+                "keogh_to_dtw": r.pass_keogh / r.pairs_swept, "dtw_cells": r.dtw_cells}
+
+    return {
+        "config": {"workload": "top1_dtw_join_learned_vs_plain", "n": n, "m": m, "w": w, "seed": 3,
+                   "planted_pairs": 5, "model": "models/proposal_m256_w13.json (trained on seeds 1000..1099)"},
+        "same_top1": (rl.i, rl.j, rl.dist) == (rp.i, rp.j, rp.dist),
+        "top1": {"i": rl.i, "j": rl.j, "dist": rl.dist},
+        "plain": {"time_to_exact_top1_s": float(np.mean([t for t, _ in runs_p])), "seed_dist": rp.seed_dist,
+                  "join_kernel_s": rp.join_kernel_ms * 1e-3, "pass_rates": rates(rp)},
+        "learned": {"time_to_exact_top1_s": float(np.mean([t for t, _, _ in runs_l])), "seed_dist": rep["seed_dist"],
+                    "join_kernel_s": rl.join_kernel_ms * 1e-3, "propose_s": rep["t_propose"],
+                    "pass_rates": rates(rl), "candidates_scored": rep["candidates"], "verified": rep["verified"],
+                    "regret": rep["regret"], "regret_ed_order": rep["regret_ed"]},
+    }
+
+
+def batched_leg(d, count):
+    """The batched form: `count` float32 random walks (seeds 100.., lengths uniform in
+    [8192, 262144], m in {64,128,256,512}, band ratio in {0.01,0.05,0.1}, one planted motif each)
+    in one persistent graph-captured launch; autotuned on a first call, then timed."""
+    series, ms, ws = [], [], []
+    for k in range(count):
+        s = 100 + k
+        r = np.random.default_rng(s)
+        n = int(r.integers(8192, 262145))
+        m = int(r.choice([64, 128, 256, 512]))
+        ratio = float(r.choice([0.01, 0.05, 0.1]))
+        a, _, _ = d.generate(n, s, n_plants=1, plant_len=m)
+        series.append(a.astype(np.float32).astype(np.float64))
+        ms.append(m)
+        ws.append(d.band_from_ratio(m, ratio))
+    ctx = d.Context()
+    ctx.top1_batch(series, ms, ws, autotune=1)  # tuning + warm-up
+    t0 = time.perf_counter()
+    res = ctx.top1_batch(series, ms, ws, autotune=1)
+    sec = time.perf_counter() - t0
+    ctx.close()
+    ok = [r for r in res if not isinstance(r, Exception)]
+    pairs = sum(r.pairs_total for r in ok)
+    cells = sum(r.dtw_cells for r in ok)
+    return {"config": {"workload": "top1_dtw_join_batched", "series": count, "seeds": [100, 100 + count - 1],
+                       "lengths": [8192, 262144], "m": [64, 128, 256, 512], "band_ratio": [0.01, 0.05, 0.1]},
+            "time_s": sec, "graph_device_s": ok[0].join_kernel_ms * 1e-3, "series_ok": len(ok),
+            "pairs_total": pairs, "pairs_per_s": pairs / sec, "dtw_cells": cells, "dtw_cells_per_s": cells / sec,
+            "tuned": {"rows_per_item": ok[0].rows_per_item, "waves_per_cu": ok[0].waves_per_cu}}
+
+
 def load_pkg():
     spec = importlib.util.spec_from_file_location("dtwj_amd", os.path.join(PKG_DIR, "__init__.py"))
     mod = importlib.util.module_from_spec(spec)
@@ -115,6 +184,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-wide-band", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--rows-per-item", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     args = ap.parse_args()
@@ -180,6 +252,10 @@ def main():
             "pass_rates": {"ball": r5.pass_ball / r5.pairs_swept, "kim": r5.pass_kim / r5.pairs_swept,
                            "keogh_to_dtw": r5.pass_keogh / r5.pairs_swept},
         }
+    if not args.no_learned:
+        line["learned"] = learned_leg(d, args.n, args.m, args.steps, args.warmup)
+    if not args.no_batched:
+        line["batched"] = batched_leg(d, args.batch_count)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(series, args.m, args.w, args.cpu_budget)
     print(json.dumps(line), flush=True)

@@ -467,6 +467,7 @@ struct Proposal {
     float *d_P = nullptr;    // network parameters (FP32)
     uint16_t *d_W1b = nullptr;
     float *d_y = nullptr;    // network output
+    float *d_key = nullptr;  // ranking keys: learned, then ED
     float *d_norm = nullptr; // feature mean, scale
     int64_t *d_vi = nullptr, *d_vj = nullptr;
     double *d_vcost = nullptr;
@@ -490,6 +491,7 @@ int propose(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w,
         Pr.d_P = ar.take<float>(np);
         Pr.d_W1b = ar.take<uint16_t>((size_t)DTWJ_MLP_HID * pad);
         Pr.d_y = ar.take<float>(rows16 + 16);
+        Pr.d_key = ar.take<float>(2 * (rows16 + 16));
         Pr.d_norm = ar.take<float>(2 * DTWJ_MLP_IN);
         Pr.d_vi = ar.take<int64_t>(nv);
         Pr.d_vj = ar.take<int64_t>(nv);
@@ -1168,20 +1170,18 @@ int dtwj_top1_learned(const double *series, int64_t n, int32_t m, int32_t w, con
     std::vector<int2> learned, by_ed;
     if (rc == DTWJ_OK && cnt > 0) {
         rc = score(c, *model, Pr.d_F, cnt, Pr);
-        std::vector<float> y(cnt), ed(cnt);
+        // rank: log ED distance + predicted log(DTW / ED) (keys computed on the GPU)
+        if (rc == DTWJ_OK) rc = launch_prop_keys(Pr.d_F, Pr.d_y, cnt, m, Pr.d_key, Pr.d_key + cnt, c->stream);
+        std::vector<float> key(cnt), ed(cnt);
         if (rc == DTWJ_OK) {
             hipStream_t s = c->stream;
-            hipError_t e = hipMemcpyAsync(y.data(), Pr.d_y, sizeof(float) * cnt, hipMemcpyDeviceToHost, s);
+            hipError_t e = hipMemcpyAsync(key.data(), Pr.d_key, sizeof(float) * cnt, hipMemcpyDeviceToHost, s);
             if (e == hipSuccess)
-                e = hipMemcpy2DAsync(ed.data(), sizeof(float), Pr.d_F + 36, sizeof(float) * DTWJ_MLP_IN, sizeof(float),
-                                     cnt, hipMemcpyDeviceToHost, s);
+                e = hipMemcpyAsync(ed.data(), Pr.d_key + cnt, sizeof(float) * cnt, hipMemcpyDeviceToHost, s);
             if (e == hipSuccess) e = hipStreamSynchronize(s);
             if (e != hipSuccess) rc = hip_fail(e, "proposal scores", __FILE__, __LINE__);
         }
         if (rc == DTWJ_OK) {
-            // rank: log ED distance + predicted log(DTW / ED); ED^2 = m expm1(feature 36)
-            std::vector<float> key(cnt);
-            for (int k = 0; k < cnt; ++k) key[k] = y[k] + 0.5f * std::log((float)m * std::expm1(ed[k]) + 1e-30f);
             const int take = std::min(n_verify, cnt);
             std::vector<int> order(cnt);
             std::iota(order.begin(), order.end(), 0);

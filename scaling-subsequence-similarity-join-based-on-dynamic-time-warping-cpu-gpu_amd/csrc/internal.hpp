@@ -68,5 +68,6 @@ int launch_mlp_grad(const uint16_t *X, const float *target, const int32_t *idx, 
                     const float *P, float *G, float *loss, hipStream_t s);
 int launch_mlp_adam(float *P, const float *G, float *M1, float *M2, float lr, int step, uint16_t *W1b, hipStream_t s);
 int launch_params_to_bf16(const float *P, uint16_t *W1b, hipStream_t s);
+int launch_prop_keys(const float *F, const float *y, int count, int m, float *key, float *key_ed, hipStream_t s);
 
 }  // namespace dtwj

@@ -292,6 +292,17 @@ __global__ void k_mlp_adam(float *__restrict__ P, const float *__restrict__ G, f
     if (t < P_B1) W1b[t] = to_bf16(p);
 }
 
+// Ranking keys of the candidates: log ED distance + network output, and log ED distance
+// alone (ED^2 = m expm1(feature 36)).
+__global__ void k_prop_keys(const float *__restrict__ F, const float *__restrict__ y, int count, int m,
+                            float *__restrict__ key, float *__restrict__ key_ed) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= count) return;
+    const float led = 0.5f * logf((float)m * expm1f(F[(size_t)t * NF + 36]) + 1e-30f);
+    key[t] = led + y[t];
+    key_ed[t] = led;
+}
+
 __global__ void k_fill_f32(float *p, int n, float v) {
     const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (t < n) p[t] = v;
@@ -364,6 +375,13 @@ int launch_mlp_grad(const uint16_t *X, const float *target, const int32_t *idx, 
 int launch_mlp_adam(float *P, const float *G, float *M1, float *M2, float lr, int step, uint16_t *W1b, hipStream_t s) {
     const float bc1 = 1.f - powf(0.9f, (float)step), bc2 = 1.f - powf(0.999f, (float)step);
     hipLaunchKernelGGL(k_mlp_adam, dim3((P_N + 255) / 256), dim3(256), 0, s, P, G, M1, M2, lr, bc1, bc2, W1b);
+    DTWJ_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_prop_keys(const float *F, const float *y, int count, int m, float *key, float *key_ed, hipStream_t s) {
+    if (count <= 0) return 0;
+    hipLaunchKernelGGL(k_prop_keys, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, F, y, count, m, key, key_ed);
     DTWJ_HIP(hipGetLastError());
     return 0;
 }
