@@ -21,7 +21,11 @@
 //     X(i+1,j+1) = X(i,j) + dfa[j] dgv[i] + dfv[i] dga[j], re-anchored exactly (FP64) at the
 //     start of every item; a per-item rounding margin keeps the FP32 test conservative.
 //     Per pair: 4 FMA updates + 2 FMA tests + 1 min; the column data of a lane's four
-//     diagonals rotates through registers (one DPP wave shift per table and row).
+//     diagonals rotates through registers (one DPP wave shift per table and row).  Where the
+//     two-sided test leaves almost nothing (narrow bands), a wave sweeps its next items with
+//     the one-sided test (X^A only: 2 FMA updates + 1 FMA test per pair, three tables) and
+//     returns to two-sided when survivors grow; the choice is made per item from the
+//     previous item's survivor rate.
 //  2. LB_Kim over the 3x3 corners at both ends (cf. CPU_version1/lb_kim.cpp:10-73).
 //  3. two-sided LB_Keogh against the raw-series envelope normalised by the other window's
 //     statistics (cf. lb_keogh_stride_with_nomalise, GPU_version1/src/LB_keogh_warp.cu:74-96),
@@ -185,6 +189,10 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
     unsigned long long c_kelem = 0;  // per lane: LB_Keogh positions evaluated
 
     int cur = -1;  // series of the current item
+    bool two_sided = true;    // sweep test of the current item
+    int prev_series = -1;     // series of the previous item (mode history)
+    int64_t prev_rows = 0;    // rows of the previous item
+    int64_t item_entries = 0; // survivor queue entries of the current item
     const JoinArgs *Ap = series;
     auto flush_counters = [&]() __attribute__((always_inline)) {
         if (cur < 0) return;
@@ -484,6 +492,17 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         }
         const JoinArgs &A = *Ap;
         if (!A.valid) continue;
+        // Sweep test for this item: the one-sided ball bound (X^A only, half the work per pair)
+        // when the previous item of this series left almost no survivors; the two-sided one
+        // otherwise.  Both are valid bounds; the cascade's two-sided LB_Keogh follows either.
+        if (s != prev_series) {
+            two_sided = true;
+            prev_series = s;
+        } else if (prev_rows > 0) {
+            const float rate = (float)item_entries / (64.0f * (float)prev_rows);  // entries per lane-row
+            two_sided = two_sided ? !(rate < 1e-3f) : (rate > 4e-3f);
+        }
+        item_entries = 0;
         const int64_t nw = A.nw, n = A.n;
         const int64_t item = gitem - item_base[s];
         int g;
@@ -493,6 +512,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         const int64_t rows_g = nw - d0;  // rows of diagonal d0
         i_beg = sidx * A.rows_per_item;
         const int64_t i_end = (i_beg + A.rows_per_item < rows_g) ? i_beg + A.rows_per_item : rows_g;
+        prev_rows = i_end - i_beg;
         const int64_t dq0 = d0 + KD * lane;
         // last valid row per owned diagonal: i <= nw - 1 - d
         int64_t lim[KD];
@@ -506,13 +526,15 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         // (m values) and the column span (DG + m + 3 values from i_beg + d0).  Positions past
         // the series end read as 0; they only reach columns past the triangle, never used.
         float XA[KD], XB[KD];
+#pragma unroll
+        for (int q = 0; q < KD; ++q) XB[q] = 0.f;
         {
             double *lrow = reinterpret_cast<double *>(smem);
             double *lcol = lrow + m;
             const int span = DG + m + KD;
             const int64_t jb0 = i_beg + d0;
 #pragma unroll 1
-            for (int pass = 0; pass < 2; ++pass) {
+            for (int pass = 0; pass < (two_sided ? 2 : 1); ++pass) {
                 const global_ptr<double> prow = in_vgprs(pass == 0 ? A.v : A.a);
                 const global_ptr<double> pcol = in_vgprs(pass == 0 ? A.a : A.v);
                 const double c0v = pass == 0 ? A.muv[i_beg] : A.mu[i_beg];
@@ -667,12 +689,17 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 // entering columns are read into separate registers, so that the rotating
                 // column slots end every group in the registers they started in.  FULL: every
                 // pair is inside the triangle.  Test per pair: X^A / sigma_j >= thetaA_i and
-                // X^B / sigma_i >= thetaB_j as signs of single-rounding FMAs; a lane-wide
+                // (TWO) X^B / sigma_i >= thetaB_j as signs of single-rounding FMAs; a lane-wide
                 // min/max reduces the four pairs to one compare per row, and a queue entry
                 // (row | lane | mask of the lane's surviving diagonals) is only formed on rows
-                // with a survivor.
-                auto group = [&](auto full_tag) __attribute__((always_inline)) {
+                // with a survivor.  Without TWO only X^A and its three tables are carried.
+                auto group = [&](auto full_tag, auto two_tag) __attribute__((always_inline)) {
                     constexpr bool FULL = decltype(full_tag)::value;
+                    constexpr bool TWO = decltype(two_tag)::value;
+                    // tables used: columns dfa dga (1/sigma) [dfv dgv thetaB]; rows dfv dgv
+                    // thetaA [dfa dga 1/sigma]
+                    auto colused = [](int r) { return TWO || r == 0 || r == 1 || r == 4; };
+                    auto rowused = [](int r) { return TWO || r == 2 || r == 3 || r == 5; };
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
                         const int r0 = rl + 4 * h;
@@ -681,16 +708,20 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                         float nv[NJARR][4];  // columns entering at lane 63: x = r0 + 256 + u
 #pragma unroll
                         for (int r = 0; r < NJARR; ++r) {
-                            const float4 v4 = *reinterpret_cast<const float4 *>(&rarr[r * RC + r0]);
-                            rv[r][0] = v4.x;
-                            rv[r][1] = v4.y;
-                            rv[r][2] = v4.z;
-                            rv[r][3] = v4.w;
-                            const float4 n4 = *reinterpret_cast<const float4 *>(&jarr[r * JW + r0 + DG]);
-                            nv[r][0] = n4.x;
-                            nv[r][1] = n4.y;
-                            nv[r][2] = n4.z;
-                            nv[r][3] = n4.w;
+                            if (rowused(r)) {
+                                const float4 v4 = *reinterpret_cast<const float4 *>(&rarr[r * RC + r0]);
+                                rv[r][0] = v4.x;
+                                rv[r][1] = v4.y;
+                                rv[r][2] = v4.z;
+                                rv[r][3] = v4.w;
+                            }
+                            if (colused(r)) {
+                                const float4 n4 = *reinterpret_cast<const float4 *>(&jarr[r * JW + r0 + DG]);
+                                nv[r][0] = n4.x;
+                                nv[r][1] = n4.y;
+                                nv[r][2] = n4.z;
+                                nv[r][3] = n4.w;
+                            }
                         }
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
@@ -699,8 +730,12 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
 #pragma unroll
                             for (int q = 0; q < KD; ++q) {
                                 const float sA = fmaf(XA[q], cs[4][q], -rv[5][u]);
-                                const float sB = fmaf(XB[q], rv[4][u], -cs[5][q]);
-                                t[q] = fminf(sA, sB);
+                                if (TWO) {
+                                    const float sB = fmaf(XB[q], rv[4][u], -cs[5][q]);
+                                    t[q] = fminf(sA, sB);
+                                } else {
+                                    t[q] = sA;
+                                }
                                 if (!FULL && r0 + u > limr[q]) t[q] = -1.0f;
                             }
                             const float tmax = fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3]));
@@ -714,16 +749,20 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                                         code0 + ((uint32_t)(r0 + u) << 10) + ((uint32_t)lane << 4) + bits;
                                 }
                                 q1n += __popcll(mk);
+                                item_entries += __popcll(mk);
                             }
 #pragma unroll
                             for (int q = 0; q < KD; ++q) {
                                 XA[q] = fmaf(cs[0][q], rv[3][u], XA[q]);
                                 XA[q] = fmaf(rv[2][u], cs[1][q], XA[q]);
-                                XB[q] = fmaf(rv[0][u], cs[3][q], XB[q]);
-                                XB[q] = fmaf(cs[2][q], rv[1][u], XB[q]);
+                                if (TWO) {
+                                    XB[q] = fmaf(rv[0][u], cs[3][q], XB[q]);
+                                    XB[q] = fmaf(cs[2][q], rv[1][u], XB[q]);
+                                }
                             }
 #pragma unroll
                             for (int r = 0; r < NJARR; ++r) {
+                                if (!colused(r)) continue;
                                 const float in = wave_shift_down1(nv[r][u], cs[r][0]);
                                 cs[r][0] = cs[r][1];
                                 cs[r][1] = cs[r][2];
@@ -733,11 +772,21 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                         }
                     }
                 };
+                using T_ = std::integral_constant<bool, true>;
+                using F_ = std::integral_constant<bool, false>;
                 for (; rl < nrows; rl += 8) {
-                    if (chunk_full && rl + 8 <= nrows)
-                        group(std::integral_constant<bool, true>());
-                    else
-                        group(std::integral_constant<bool, false>());
+                    const bool full8 = chunk_full && rl + 8 <= nrows;
+                    if (two_sided) {
+                        if (full8)
+                            group(T_(), T_());
+                        else
+                            group(F_(), T_());
+                    } else {
+                        if (full8)
+                            group(T_(), F_());
+                        else
+                            group(F_(), F_());
+                    }
                     if (q1n > QLIM) {
                         rl += 8;
                         break;
