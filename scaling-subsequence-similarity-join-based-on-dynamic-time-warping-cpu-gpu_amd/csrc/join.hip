@@ -21,7 +21,7 @@
 //     X(i+1,j+1) = X(i,j) + dfa[j] dgv[i] + dfv[i] dga[j], re-anchored exactly (FP64) at the
 //     start of every item; a per-item rounding margin keeps the FP32 test conservative.
 //     Per pair: 4 FMA updates + 2 FMA tests + 1 min; the column data of a lane's four
-//     diagonals rotates through registers (one DPP wave shift per table and row).  Where the
+//     diagonals rotates through registers (one LDS read per table and 4 rows).  Where the
 //     two-sided test leaves almost nothing (narrow bands), a wave sweeps its next items with
 //     the one-sided test (X^A only: 2 FMA updates + 1 FMA test per pair, three tables) and
 //     returns to two-sided when survivors grow; the choice is made per item from the
@@ -110,13 +110,6 @@ __device__ inline void decode_item(const int64_t *item_start, int groups, int64_
     }
     g = lo;
     s = item - item_start[g];
-}
-
-// Lane L receives src from lane L + 1; lane 63 receives `last` (DPP wave_shl:1 with the
-// out-of-range lane left at the old value).
-__device__ inline float wave_shift_down1(float last, float src) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, last),
-                                                                 __builtin_bit_cast(int, src), 0x130, 0xf, 0xf, false));
 }
 
 // Copies a uniform pointer into vector registers.  The join keeps many uniform values alive;
@@ -674,8 +667,10 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 __syncthreads();
                 // Column slots of this lane: slot q holds the staged column x = r + 4 lane + q
                 // for the current local row r.  Moving to row r + 1 shifts every slot by one
-                // column: slots 1..3 move down, slot 3 takes lane + 1's slot 0 (a DPP wave shift)
-                // and lane 63 takes the new column x = r + 256 (a broadcast LDS read).
+                // column: slots 1..3 move down (register renaming in the unrolled group) and
+                // slot 3 takes column r + 4 lane + 4.  For the 4 rows of a half starting at r0
+                // (a multiple of 4) the entering columns are one aligned float4 per table at
+                // x = r0 + 4 (lane + 1), read from LDS (LDS pipe, no VALU shifts).
                 float cs[NJARR][KD];
 #pragma unroll
                 for (int r = 0; r < NJARR; ++r) {
@@ -705,7 +700,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                         const int r0 = rl + 4 * h;
                         if (!FULL && r0 >= nrows) break;
                         float rv[NJARR][4];  // row data of the 4 rows (same address on every lane)
-                        float nv[NJARR][4];  // columns entering at lane 63: x = r0 + 256 + u
+                        float nv[NJARR][4];  // columns entering slot 3: x = r0 + 4 (lane + 1) + u
 #pragma unroll
                         for (int r = 0; r < NJARR; ++r) {
                             if (rowused(r)) {
@@ -716,41 +711,49 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                                 rv[r][3] = v4.w;
                             }
                             if (colused(r)) {
-                                const float4 n4 = *reinterpret_cast<const float4 *>(&jarr[r * JW + r0 + DG]);
+                                const float4 n4 =
+                                    *reinterpret_cast<const float4 *>(&jarr[r * JW + r0 + 4 * lane + 4]);
                                 nv[r][0] = n4.x;
                                 nv[r][1] = n4.y;
                                 nv[r][2] = n4.z;
                                 nv[r][3] = n4.w;
                             }
                         }
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            if (!FULL && r0 + u >= nrows) break;
-                            float t[KD];
-#pragma unroll
-                            for (int q = 0; q < KD; ++q) {
-                                const float sA = fmaf(XA[q], cs[4][q], -rv[5][u]);
-                                if (TWO) {
-                                    const float sB = fmaf(XB[q], rv[4][u], -cs[5][q]);
-                                    t[q] = fminf(sA, sB);
-                                } else {
-                                    t[q] = sA;
-                                }
-                                if (!FULL && r0 + u > limr[q]) t[q] = -1.0f;
-                            }
-                            const float tmax = fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3]));
+                        // queue entries of row r0 + u: one per lane with a surviving diagonal
+                        float t[4][KD];
+                        auto append = [&](int u) __attribute__((always_inline)) {
+                            const float tmax = fmaxf(fmaxf(t[u][0], t[u][1]), fmaxf(t[u][2], t[u][3]));
                             const bool any = tmax >= 0.0f;
                             const unsigned long long mk = __builtin_amdgcn_ballot_w64(any);
-                            if (mk != 0ull) {
+                            if (__builtin_expect(mk != 0ull, 0)) {
                                 if (any) {
-                                    const uint32_t bits = (t[0] >= 0.0f ? 1u : 0u) | (t[1] >= 0.0f ? 2u : 0u) |
-                                                          (t[2] >= 0.0f ? 4u : 0u) | (t[3] >= 0.0f ? 8u : 0u);
+                                    const uint32_t bits = (t[u][0] >= 0.0f ? 1u : 0u) | (t[u][1] >= 0.0f ? 2u : 0u) |
+                                                          (t[u][2] >= 0.0f ? 4u : 0u) | (t[u][3] >= 0.0f ? 8u : 0u);
                                     q1[q1n + (int)mbcnt64(mk)] =
                                         code0 + ((uint32_t)(r0 + u) << 10) + ((uint32_t)lane << 4) + bits;
                                 }
                                 q1n += __popcll(mk);
                                 item_entries += __popcll(mk);
                             }
+                        };
+                        // one-sided full halves test all 16 pairs with one ballot (survivors are
+                        // rare there by construction) and form entries row by row only on a hit
+                        constexpr bool BATCH = FULL && !TWO;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            if (!FULL && r0 + u >= nrows) break;
+#pragma unroll
+                            for (int q = 0; q < KD; ++q) {
+                                const float sA = fmaf(XA[q], cs[4][q], -rv[5][u]);
+                                if (TWO) {
+                                    const float sB = fmaf(XB[q], rv[4][u], -cs[5][q]);
+                                    t[u][q] = fminf(sA, sB);
+                                } else {
+                                    t[u][q] = sA;
+                                }
+                                if (!FULL && r0 + u > limr[q]) t[u][q] = -1.0f;
+                            }
+                            if (!BATCH) append(u);
 #pragma unroll
                             for (int q = 0; q < KD; ++q) {
                                 XA[q] = fmaf(cs[0][q], rv[3][u], XA[q]);
@@ -763,11 +766,21 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
 #pragma unroll
                             for (int r = 0; r < NJARR; ++r) {
                                 if (!colused(r)) continue;
-                                const float in = wave_shift_down1(nv[r][u], cs[r][0]);
                                 cs[r][0] = cs[r][1];
                                 cs[r][1] = cs[r][2];
                                 cs[r][2] = cs[r][3];
-                                cs[r][3] = in;
+                                cs[r][3] = nv[r][u];
+                            }
+                        }
+                        if (BATCH) {
+                            float hmax = fmaxf(fmaxf(t[0][0], t[0][1]), t[0][2]);
+#pragma unroll
+                            for (int k = 3; k + 1 < 4 * KD; k += 2)  // pairs (3,4) .. (13,14)
+                                hmax = fmaxf(fmaxf(hmax, t[k / KD][k % KD]), t[(k + 1) / KD][(k + 1) % KD]);
+                            hmax = fmaxf(hmax, t[3][KD - 1]);
+                            if (__builtin_expect(__builtin_amdgcn_ballot_w64(hmax >= 0.0f) != 0ull, 0)) {
+#pragma unroll
+                                for (int u = 0; u < 4; ++u) append(u);
                             }
                         }
                     }
