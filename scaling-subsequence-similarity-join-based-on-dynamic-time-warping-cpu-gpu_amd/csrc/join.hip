@@ -117,6 +117,7 @@ __device__ inline void decode_item(const int64_t *item_start, int groups, int64_
 // compiler does not spill (and reload) scalar register tuples around that loop.
 template <typename T>
 using global_ptr = const __attribute__((address_space(1))) T *;
+typedef float vf4 __attribute__((ext_vector_type(4)));  // 16-byte vector load
 template <typename T>
 __device__ inline global_ptr<T> in_vgprs(const T *p) {
     const uint64_t u = (uint64_t)p;
@@ -620,49 +621,75 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
             const uint32_t code0 = (uint32_t)(c0 - i_beg) << 10;
             int rl = 0;  // next local row (multiple of 8)
             while (rl < nrows) {
-                th = load_threshold(A);
-                const float T = th.T;
-                // stage row data for the chunk's rows: dfa dga dfv dgv 1/sigma thetaA
-                for (int x = lane; x < RC; x += 64) {
-                    const int64_t i = c0 + x;
-                    float f0 = 0.f, f1 = 0.f, f2 = 0.f, f3 = 0.f, f4 = 0.f, f5 = 0.f;
-                    if (x < nrows) {
-                        const WinRow r = A.win[i];
+                // Stage the chunk's row data (rows c0 + x, x < RC: dfa dga dfv dgv 1/sigma
+                // thetaA) and column data (columns j = c0 + d0 + x, x < JW: dfa dga dfv dgv
+                // 1/sigma thetaB) into LDS; the one-sided test needs only half of the tables.
+                // Every window load is issued before the first use (indices clamped into the
+                // series, entries past the end zeroed after the load), so that the latencies
+                // of the chunk's loads overlap instead of adding up.
+                {
+                    constexpr int NRL = RC / 64, NCL = (JW + 63) / 64;
+                    static_assert(sizeof(WinRow) == 32, "WinRow is read as two 16-byte vectors");
+                    const global_ptr<vf4> win = (global_ptr<vf4>)A.win;
+                    WinRow rw[NRL], cw[NCL];
+                    auto get = [&](int64_t idx, WinRow &r) __attribute__((always_inline)) {
+                        const vf4 lo = win[2 * idx], hi = win[2 * idx + 1];
+                        r.dfa = lo.x;
+                        r.dga = lo.y;
+                        r.dfv = lo.z;
+                        r.dgv = lo.w;
+                        r.isig = hi.x;
+                        r.mN = hi.y;
+                        r.R = hi.z;
+                        r.sig = hi.w;
+                    };
+#pragma unroll
+                    for (int k = 0; k < NRL; ++k) {
+                        const int64_t i = c0 + lane + 64 * k;
+                        get(i < nw - 1 ? i : nw - 1, rw[k]);
+                    }
+#pragma unroll
+                    for (int k = 0; k < NCL; ++k) {
+                        const int64_t j = c0 + d0 + lane + 64 * k;
+                        get(j < nw - 1 ? j : nw - 1, cw[k]);
+                    }
+                    th = load_threshold(A);
+                    const float T = th.T;
+#pragma unroll
+                    for (int k = 0; k < NRL; ++k) {
+                        const int x = lane + 64 * k;
+                        const WinRow &r = rw[k];
+                        const bool ok = x < nrows;
                         const float tA = T + r.R;
-                        f0 = r.dfa;
-                        f1 = r.dga;
-                        f2 = r.dfv;
-                        f3 = r.dgv;
-                        f4 = r.isig;
-                        f5 = (0.5f * (r.mN - tA * tA) - 0x1p-20f * (r.mN + tA * tA)) * r.sig - EA;
+                        const float thA = (0.5f * (r.mN - tA * tA) - 0x1p-20f * (r.mN + tA * tA)) * r.sig - EA;
+                        rarr[2 * RC + x] = ok ? r.dfv : 0.f;
+                        rarr[3 * RC + x] = ok ? r.dgv : 0.f;
+                        rarr[5 * RC + x] = ok ? thA : 0.f;
+                        if (two_sided) {
+                            rarr[0 * RC + x] = ok ? r.dfa : 0.f;
+                            rarr[1 * RC + x] = ok ? r.dga : 0.f;
+                            rarr[4 * RC + x] = ok ? r.isig : 0.f;
+                        }
                     }
-                    rarr[0 * RC + x] = f0;
-                    rarr[1 * RC + x] = f1;
-                    rarr[2 * RC + x] = f2;
-                    rarr[3 * RC + x] = f3;
-                    rarr[4 * RC + x] = f4;
-                    rarr[5 * RC + x] = f5;
-                }
-                // stage column data for j = c0 + d0 + x, x in [0, JW)
-                for (int x = lane; x < JW; x += 64) {
-                    const int64_t j = c0 + d0 + x;
-                    float f0 = 0.f, f1 = 0.f, f2 = 0.f, f3 = 0.f, f4 = 0.f, f5 = 0.f;
-                    if (j <= nw - 1) {
-                        const WinRow r = A.win[j];
-                        const float tB = T + r.R;
-                        f0 = r.dfa;
-                        f1 = r.dga;
-                        f2 = r.dfv;
-                        f3 = r.dgv;
-                        f4 = r.isig;
-                        f5 = (0.5f * (r.mN - tB * tB) - 0x1p-20f * (r.mN + tB * tB)) * r.sig - EB;
+#pragma unroll
+                    for (int k = 0; k < NCL; ++k) {
+                        const int x = lane + 64 * k;
+                        if (x < JW) {
+                            const WinRow &r = cw[k];
+                            const bool ok = c0 + d0 + x <= nw - 1;
+                            jarr[0 * JW + x] = ok ? r.dfa : 0.f;
+                            jarr[1 * JW + x] = ok ? r.dga : 0.f;
+                            jarr[4 * JW + x] = ok ? r.isig : 0.f;
+                            if (two_sided) {
+                                const float tB = T + r.R;
+                                const float thB =
+                                    (0.5f * (r.mN - tB * tB) - 0x1p-20f * (r.mN + tB * tB)) * r.sig - EB;
+                                jarr[2 * JW + x] = ok ? r.dfv : 0.f;
+                                jarr[3 * JW + x] = ok ? r.dgv : 0.f;
+                                jarr[5 * JW + x] = ok ? thB : 0.f;
+                            }
+                        }
                     }
-                    jarr[0 * JW + x] = f0;
-                    jarr[1 * JW + x] = f1;
-                    jarr[2 * JW + x] = f2;
-                    jarr[3 * JW + x] = f3;
-                    jarr[4 * JW + x] = f4;
-                    jarr[5 * JW + x] = f5;
                 }
                 __syncthreads();
                 // Column slots of this lane: slot q holds the staged column x = r + 4 lane + q
