@@ -8,7 +8,8 @@ the reference's rounding rule gives w = 3, available with --w 3).
 
 Prints one JSON line.  value = mean time-to-exact-top-1 in seconds (lower is better).
 roofline: the fused join kernel is FP32-VALU-bound; achieved = algorithmic FP32 operations
-of the launch (pairs swept x 12, ball survivors x 84, LB_Keogh positions x 20, DTW cells x 5)
+of the launch (pairs swept x 12 two-sided or x 6 one-sided, ball survivors x 84, LB_Keogh
+positions x 20, DTW cells x 5)
 / its HIP-event time, peak 157.3 TFLOP/s.  traffic = HBM bytes per launch from the rocprofv3
 FETCH_SIZE / WRITE_SIZE summary under profiles/ for the same workload (null otherwise).
 A second workload (band ratio 0.05: seed 3, 5 plants, w = 13) is timed beside the headline one
@@ -30,7 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "scaling-subsequence-similarity-join-based-on-dynamic-time-warping-cpu-gpu_amd")
 
 # Algorithmic FP32 operations per unit of work (an FMA counts 2).
-OPS_SWEPT_PAIR = 12      # 4 FMA (two O(1) covariance updates per direction) + 2 mul + 2 compare
+OPS_SWEPT_PAIR = 12      # two-sided: 4 FMA (two O(1) covariance updates per direction) + 2 mul + 2 compare
+OPS_SWEPT_ONE = 6        # one-sided: 2 FMA (one direction) + 1 mul + 1 compare
 OPS_KIM_PAIR = 84        # 12 z values (FMA) + 18 squared differences + 10 min/add chains
 OPS_KEOGH_ELEM = 20      # per position, both directions: 2 FMA envelope, 2 sub, 2 max, 1 FMA
 OPS_DTW_CELL = 5         # sub, mul, add, 2-way min of three
@@ -201,8 +203,8 @@ def main():
     r = results[-1]
     sec = float(np.mean(walls))
     kern_ms = float(np.mean([x.join_kernel_ms for x in results]))
-    ops = (r.pairs_swept * OPS_SWEPT_PAIR + r.pass_ball * OPS_KIM_PAIR + r.keogh_elems *
-           OPS_KEOGH_ELEM + r.dtw_cells * OPS_DTW_CELL)
+    ops = ((r.pairs_swept - r.pairs_swept_one_sided) * OPS_SWEPT_PAIR + r.pairs_swept_one_sided * OPS_SWEPT_ONE +
+           r.pass_ball * OPS_KIM_PAIR + r.keogh_elems * OPS_KEOGH_ELEM + r.dtw_cells * OPS_DTW_CELL)
     achieved = ops / (kern_ms * 1e-3)
     config = {"workload": "top1_dtw_join_plain", "n": args.n, "m": args.m, "w": args.w, "seed": args.seed,
               "planted_pairs": args.plants, "parallelism": "single GPU, persistent grid"}
@@ -231,6 +233,7 @@ def main():
         "dtw_cells_per_s": r.dtw_cells / (kern_ms * 1e-3),
         "pass_rates": {"ball": r.pass_ball / r.pairs_swept, "kim": r.pass_kim / r.pairs_swept,
                        "keogh_to_dtw": r.pass_keogh / r.pairs_swept},
+        "sweep_one_sided_frac": r.pairs_swept_one_sided / r.pairs_swept,
         "seconds": {"prep": r.t_prep, "seed": r.t_seed, "join": r.t_join, "verify": r.t_verify,
                     "join_kernel": kern_ms * 1e-3},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",

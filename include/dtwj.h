@@ -78,7 +78,9 @@ typedef struct dtwj_result {
     int32_t items;       /* sweep work items */
     int32_t rows_per_item; /* sweep tile height used */
     int32_t waves_per_cu;  /* persistent waves per CU used */
-    int32_t reserved[5];
+    int32_t reserved0;
+    int64_t pairs_swept_one_sided; /* of pairs_swept: tested with the one-sided ball bound */
+    int32_t reserved[2];
 } dtwj_result;
 
 /* Fills *opt with defaults.  No reference counterpart (the reference hard-codes

@@ -49,7 +49,8 @@ class Result(ctypes.Structure):
                 ("t_total", ctypes.c_double), ("t_prep", ctypes.c_double), ("t_seed", ctypes.c_double),
                 ("t_join", ctypes.c_double), ("t_verify", ctypes.c_double), ("join_kernel_ms", ctypes.c_double),
                 ("items", ctypes.c_int32), ("rows_per_item", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 5)]
+                ("reserved0", ctypes.c_int32), ("pairs_swept_one_sided", ctypes.c_int64),
+                ("reserved", ctypes.c_int32 * 2)]
 
 
 MLP_IN, MLP_HID = 40, 64
@@ -200,6 +201,7 @@ class Top1:
     items: int
     rows_per_item: int = 0
     waves_per_cu: int = 0
+    pairs_swept_one_sided: int = 0
     extra: dict = field(default_factory=dict)
 
     @property
@@ -213,7 +215,7 @@ class Top1:
 
 
 def _result(r):
-    return Top1(**{k: getattr(r, k) for k, _ in Result._fields_ if k != "reserved"})
+    return Top1(**{k: getattr(r, k) for k, _ in Result._fields_ if not k.startswith("reserved")})
 
 
 def make_options(device=0, seed_items=0, rows_per_item=0, waves_per_cu=0, verbose=False, graph=0, autotune=0):

@@ -419,6 +419,7 @@ int finish_job(Job &J, int64_t *d_vi, int64_t *d_vj, double *d_vcost, hipStream_
     out->dtw_abandoned = (int64_t)counters[4];
     out->dtw_cells = (int64_t)counters[5];
     out->keogh_elems = (int64_t)counters[6];
+    out->pairs_swept_one_sided = (int64_t)counters[7];
     out->candidates = nv;
     out->items = (int32_t)J.n_items;
     out->rows_per_item = J.rows_per_item;

@@ -60,7 +60,8 @@ constexpr int Q2RUN = Q2CAP - 64;      // run LB_Keogh once this many pairs are 
 constexpr int Q3CAP = 128;             // DTW queue (packed pairs)
 constexpr int QLIM = kJoinQueueCap - 8 * 64;  // drain before an 8-row group could overflow
 
-enum { C_SWEPT = 0, C_BALL, C_KIM, C_KEOGH, C_DTW_AB, C_CELLS, C_KEOGH_ELEMS, C_NCOUNTERS };
+enum { C_SWEPT = 0, C_BALL, C_KIM, C_KEOGH, C_DTW_AB, C_CELLS, C_KEOGH_ELEMS, C_SWEPT_ONE, C_NCOUNTERS };
+static_assert(C_NCOUNTERS == 8, "the host reads 8 counters per series");
 
 // LDS floats of the cascade staging for window length m (overlays the column tables):
 // series / upper / lower envelope of the chunk's row windows and column windows, and the
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         if (cur < 0) return;
         __syncthreads();
         if (lane == 0) {
-            for (int c = 0; c < C_KEOGH_ELEMS; ++c)
+            for (int c = 0; c < C_NCOUNTERS; ++c)  // C_KEOGH_ELEMS is counted per lane below
                 if (wcnt[c]) atomicAdd(&Ap->counters[c], wcnt[c]);
         }
         if (c_kelem) atomicAdd(&Ap->counters[C_KEOGH_ELEMS], c_kelem);
@@ -353,31 +354,64 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         const int64_t n = A.n, nw = A.nw;
         const int CI = RC + m, CJ = RC + DG + m;
         __syncthreads();  // queue entries were stored by other lanes; column tables retire
-        for (int x = lane; x < CI; x += 64) {
-            const int64_t p = c0 + x;
-            const bool ok = p < n;
-            sai[x] = ok ? A.ac[p] : 0.f;
-            sui[x] = ok ? A.uc[p] : 0.f;
-            sli[x] = ok ? A.lc[p] : 0.f;
-        }
-        for (int x = lane; x < CJ; x += 64) {
-            const int64_t p = c0 + d0 + x;
-            const bool ok = p < n;
-            saj[x] = ok ? A.ac[p] : 0.f;
-            suj[x] = ok ? A.uc[p] : 0.f;
-            slj[x] = ok ? A.lc[p] : 0.f;
-        }
-        for (int x = lane; x < RC + DG; x += 64) {
-            if (x < RC) {
-                const int64_t r = c0 + x;
-                const bool ok = r < nw;
-                rmu[x] = ok ? A.mu_c[r] : 0.f;
-                ris[x] = ok ? A.win[r].isig : 1.f;
-            }
-            const int64_t cc = c0 + d0 + x;
-            const bool okc = cc < nw;
-            cmu[x] = okc ? A.mu_c[cc] : 0.f;
-            cis[x] = okc ? A.win[cc].isig : 1.f;
+        // Windows of the chunk's rows (CI samples from c0) and columns (CJ samples from
+        // c0 + d0), then mean and 1/sigma of its RC rows and RC + DG columns.  Loads go out in
+        // batches of 8 per lane (indices clamped into the series, entries past the end set
+        // after the load) so that their latencies overlap.
+        {
+            const global_ptr<float> ac = (global_ptr<float>)A.ac, uc = (global_ptr<float>)A.uc,
+                                    lc = (global_ptr<float>)A.lc, muc = (global_ptr<float>)A.mu_c;
+            const global_ptr<float> wf = (global_ptr<float>)A.win;  // WinRow as 8 floats
+            constexpr int WISIG = 4;                                // float index of isig
+            static_assert(sizeof(WinRow) == 32 && offsetof(WinRow, isig) == 4 * WISIG, "WinRow layout");
+            auto stage3 = [&](float *d0p, float *d1p, float *d2p, int64_t base, int cnt) __attribute__((always_inline)) {
+                for (int x0 = 0; x0 < cnt; x0 += 512) {
+                    float v0[8], v1[8], v2[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int64_t p = base + x0 + 64 * u + lane;
+                        const int64_t pc = p < n - 1 ? p : n - 1;
+                        v0[u] = ac[pc];
+                        v1[u] = uc[pc];
+                        v2[u] = lc[pc];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int x = x0 + 64 * u + lane;
+                        if (x < cnt) {
+                            const bool ok = base + x < n;
+                            d0p[x] = ok ? v0[u] : 0.f;
+                            d1p[x] = ok ? v1[u] : 0.f;
+                            d2p[x] = ok ? v2[u] : 0.f;
+                        }
+                    }
+                }
+            };
+            auto stage_stats = [&](float *mup, float *isp, int64_t base, int cnt) __attribute__((always_inline)) {
+                for (int x0 = 0; x0 < cnt; x0 += 512) {
+                    float v0[8], v1[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int64_t r = base + x0 + 64 * u + lane;
+                        const int64_t rc = r < nw - 1 ? r : nw - 1;
+                        v0[u] = muc[rc];
+                        v1[u] = wf[8 * rc + WISIG];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int x = x0 + 64 * u + lane;
+                        if (x < cnt) {
+                            const bool ok = base + x < nw;
+                            mup[x] = ok ? v0[u] : 0.f;
+                            isp[x] = ok ? v1[u] : 1.f;
+                        }
+                    }
+                }
+            };
+            stage3(sai, sui, sli, c0, CI);
+            stage3(saj, suj, slj, c0 + d0, CJ);
+            stage_stats(rmu, ris, c0, RC);
+            stage_stats(cmu, cis, c0 + d0, RC + DG);
         }
         __syncthreads();
         const int row0 = (int)(c0 - i_beg);  // chunk start relative to the item start
@@ -594,6 +628,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
             const int64_t a0 = (i_beg > F - DG + 1) ? i_beg : F - DG + 1, b0 = i_end - 1;
             if (a0 <= b0) cnt += (b0 - a0 + 1) * (2 * F - a0 - b0) / 2;
             count(C_SWEPT, (unsigned long long)cnt);
+            if (!two_sided) count(C_SWEPT_ONE, (unsigned long long)cnt);
         }
         // ---- per-item rounding margins of the FP32 sweep, in the units of X / sigma_j
         // (resp. X / sigma_i): each step rounds two FMAs on values bounded by
