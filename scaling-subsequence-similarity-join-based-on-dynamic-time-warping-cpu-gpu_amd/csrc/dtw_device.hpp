@@ -17,9 +17,21 @@
 // pair per segment; the last lane of a segment owns an out-of-band offset, so the shifts never
 // carry values between pairs.
 //
-// Early abandon: every path crosses anti-diagonal t or t-1, so once the minimum over a
-// segment's registers exceeds ub its result can only exceed ub; the segment is then reported
-// abandoned (the reference abandons with a ballot at the band's middle, DTW_stairs.cu:225-237).
+// Early abandon: every path crosses anti-diagonal t or t+1 (the two held in the registers at a
+// check), so once the minimum over a segment's registers exceeds ub its result can only exceed
+// ub; the segment is then reported abandoned (the reference abandons with a ballot at the
+// band's middle, DTW_stairs.cu:225-237).  Optionally a suffix bound is added to the minimum:
+// after its crossing cell (r, c), r + c <= t + 1 and |r - c| <= w give r, c <= (t + 1 + w) / 2,
+// and the rest of the path visits every later row and every later column, each at a cost of at
+// least its LB_Keogh term (the cumulative-bound early abandon of the UCR suite, applied per
+// anti-diagonal check).  The suffix functor returns that sum for the check at t.
+//
+// FP32 safety of the suffix test: the envelope values are samples (stored with the same
+// rounding), so their z-values bound the band's z-values in FP32 too and each term is at most
+// the cell cost it stands for; the FMA chain of the remaining path loses at most (2m + 2) u of
+// relative value and the FP32 suffix sum is at most (m + 1) u too large (u = 2^-24).  Scaling
+// (minimum + suffix) by (1 - 2^-9) therefore never abandons a pair whose FP32 result would be
+// <= ub, for m up to 8192 (3 (m + 2) u <= 2^-9).
 #pragma once
 
 #include "common.hpp"
@@ -93,6 +105,12 @@ struct PlainZ {
     __device__ T col(int c) const { return zj[c]; }
 };
 
+// No suffix bound: the abandon test uses the partial minimum alone.
+struct NoSuffix {
+    static constexpr bool enabled = false;
+    __device__ float at(int) const { return 0.f; }
+};
+
 // Cells (r, c) with r + c <= t, |r - c| <= w, 0 <= r, c <= m - 1 (w <= m - 1): the cells a
 // DTW evaluated when it stopped after anti-diagonal t.
 __host__ __device__ inline int64_t dtw_cells_upto(int m, int w, int64_t t) {
@@ -126,8 +144,9 @@ inline int dtw_qp_for(int w) {
 // Steps come in pairs (even offsets at t, odd offsets at t+1): both cells of a lane's offset
 // pair lie on the same row r = (t - e)/2, so a pair of steps reads one row value and two
 // column values.
-template <typename T, int QP, typename Z>
-__device__ __forceinline__ T dtw_band(int m, int w, int seg_len, int nseg, T ub, const Z &z, int &t_stop) {
+template <typename T, int QP, typename Z, typename SUF = NoSuffix>
+__device__ __forceinline__ T dtw_band(int m, int w, int seg_len, int nseg, T ub, const Z &z, int &t_stop,
+                                      const SUF &suf = SUF()) {
     const int lane = (int)(threadIdx.x & 63);
     const T INF = dev_inf<T>();
     const int seg = lane / seg_len;
@@ -208,7 +227,12 @@ __device__ __forceinline__ T dtw_band(int m, int w, int seg_len, int nseg, T ub,
                 }
                 v = __shfl(v, seg * seg_len, 64);
             }
-            if (alive && v > ub) {
+            bool over = v > ub;
+            if constexpr (SUF::enabled) {
+                const T sb = (T)suf.at(t);  // every lane takes part (the functor may shuffle)
+                over = over || (v + sb) * (T)(1.0 - 0x1p-9) > ub;
+            }
+            if (alive && over) {
                 alive = false;
                 t_stop = t + 1;
             }

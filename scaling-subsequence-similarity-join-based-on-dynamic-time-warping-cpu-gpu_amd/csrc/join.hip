@@ -158,6 +158,28 @@ struct StagedZ {
     __device__ float col(int c) const { return (xj[c] - mj) * ij; }
 };
 
+// Suffix bound of the DTW of each segment's pair (see dtw_device.hpp): block kappa holds the
+// LB_Keogh terms (both directions) of rows / columns [8 (kappa + 1) + ceil(w/2), + 8), and the
+// check at anti-diagonal t (t = 16 k + 14 or + 15) needs the sum over the blocks kappa >= k = t / 16.
+// Lane x of a segment holds the blocks kappa = x + seg_len q, q < SUF_Q, as reverse prefix sums
+// max(sum A, sum B) over all blocks from kappa on.
+constexpr int SUF_Q = 4;
+struct LaneSuffix {
+    static constexpr bool enabled = true;
+    float reg[SUF_Q];
+    int base, seg_len, nq;  // lane of the segment's first block, segment length, rounds used
+    __device__ float at(int t) const {
+        const int k = t >> 4;
+        const int q = k / seg_len;
+        float v = 0.f;
+#pragma unroll
+        for (int i = 0; i < SUF_Q; ++i)
+            if (i == q) v = reg[i];
+        const float r = __shfl(v, base + (k - q * seg_len), 64);
+        return q < nq ? r : 0.f;
+    }
+};
+
 // Packed (row, column) offsets of a pair inside the staged chunk (both < 2^16).
 __device__ inline uint32_t pack_pair(int li, int lj) { return (uint32_t)li | ((uint32_t)lj << 16); }
 __device__ inline int pair_row(uint32_t p) { return (int)(p & 0xffffu); }
@@ -225,6 +247,11 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         const int per_wave = 64 / seg_len;
         const int seg = lane / seg_len;
         const bool leader = lane == seg * seg_len;
+        // suffix-bound abandon for bands of 16 and more (narrower bands end their DTWs within a
+        // few checks anyway) when the blocks fit SUF_Q rounds and m keeps the FP32 margin valid
+        const int suf_blocks = (m - 8 - ((w + 1) >> 1) + 7) / 8;  // blocks with a row below m
+        const int suf_rounds = suf_blocks > 0 ? (suf_blocks + seg_len - 1) / seg_len : 0;
+        const bool use_suffix = w >= 16 && m <= 8192 && suf_rounds <= SUF_Q;
         for (int e0 = 0; e0 < q3n; e0 += per_wave) {
             const int nseg = (q3n - e0) < per_wave ? (q3n - e0) : per_wave;
             th = load_threshold(A);
@@ -242,10 +269,53 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
             const StagedZ z{sai + li, saj + lj, mi, ii, mj, ij};
             int t_stop = 0;
             float res;
+            // block sums of the LB_Keogh terms of the lane's blocks and their reverse prefix sums
+            // over the segment's blocks (rounds q from last to first, carrying the totals)
+            LaneSuffix suf;
+            suf.base = seg * seg_len;
+            suf.seg_len = seg_len;
+            suf.nq = use_suffix ? suf_rounds : 0;
+            {
+                const int x = lane - seg * seg_len;
+                float cA = 0.f, cB = 0.f;
+#pragma unroll
+                for (int q = SUF_Q - 1; q >= 0; --q) {
+                    suf.reg[q] = 0.f;
+                    if (q < suf.nq) {
+                        float vA = 0.f, vB = 0.f;
+                        const int r0 = 8 * (x + seg_len * q + 1) + ((w + 1) >> 1);
+                        if (seg < nseg && r0 < m) {
+                            const int r1 = r0 + 8 < m ? r0 + 8 : m;
+                            for (int r = r0; r < r1; ++r) {
+                                const float zi = z.row(r), zj = z.col(r);
+                                const float uj = (suj[lj + r] - mj) * ij, lwj = (slj[lj + r] - mj) * ij;
+                                const float ui = (sui[li + r] - mi) * ii, lwi = (sli[li + r] - mi) * ii;
+                                const float ta = fmaxf(fmaxf(zi - uj, lwj - zi), 0.f);
+                                const float tb = fmaxf(fmaxf(zj - ui, lwi - zj), 0.f);
+                                vA = fmaf(ta, ta, vA);
+                                vB = fmaf(tb, tb, vB);
+                            }
+                        }
+#pragma unroll
+                        for (int off = 1; off < 64; off <<= 1) {
+                            const float yA = __shfl_down(vA, off, 64), yB = __shfl_down(vB, off, 64);
+                            if (x + off < seg_len) {
+                                vA += yA;
+                                vB += yB;
+                            }
+                        }
+                        vA += cA;
+                        vB += cB;
+                        cA = __shfl(vA, suf.base, 64);
+                        cB = __shfl(vB, suf.base, 64);
+                        suf.reg[q] = fmaxf(vA, vB);
+                    }
+                }
+            }
             if (w <= 63)
-                res = dtw_band<float, 1>(m, w, seg_len, nseg, th.dtw_ub, z, t_stop);
+                res = dtw_band<float, 1>(m, w, seg_len, nseg, th.dtw_ub, z, t_stop, suf);
             else
-                res = dtw_band<float, Q>(m, w, 64, 1, th.dtw_ub, z, t_stop);
+                res = dtw_band<float, Q>(m, w, 64, 1, th.dtw_ub, z, t_stop, suf);
             const bool lead = leader && seg < nseg;
             const bool ab = lead && !(res <= th.dtw_ub);
             unsigned long long cells = lead ? (unsigned long long)dtw_cells_upto(m, w, t_stop) : 0ull;
