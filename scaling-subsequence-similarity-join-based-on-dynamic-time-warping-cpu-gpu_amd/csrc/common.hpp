@@ -99,11 +99,12 @@ __device__ inline int lane_id() { return (int)__lane_id(); }
 // Entries of each wave's ball-survivor queue (4 bytes each) in global memory.
 constexpr int kJoinQueueCap = 4096;
 
-// Per-window table, 8 floats (32 bytes) so that a row is one scalar 8-dword load.
+// Per-window table, 8 floats (32 bytes, two 16-byte vectors).  The first vector holds what
+// the one-sided sweep reads for a column (dfa, dga, 1/sigma).
 struct WinRow {
     float dfa, dga;  // MPX update terms of the series
-    float dfv, dgv;  // MPX update terms of the envelope mid-line v = (U+L)/2
     float isig;      // 1 / sigma
+    float dfv, dgv;  // MPX update terms of the envelope mid-line v = (U+L)/2
     float mN;        // m + ||(v_i - mu_i)/sigma_i||^2
     float R;         // ball radius ||(U_i - L_i)/2|| / sigma_i
     float sig;       // sigma

@@ -54,6 +54,10 @@ constexpr int RC = 64;                 // rows per LDS chunk
 constexpr int JW = RC + DG + 8;        // staged column entries per chunk
 constexpr int NJARR = 6;               // column arrays: dfa dga dfv dgv 1/sigma thetaB
 constexpr int kSweepFloats = NJARR * JW + NJARR * RC;  // column + row tables
+constexpr int RC1 = 256;               // rows per chunk of the one-sided sweep (3 + 3 tables)
+constexpr int JW1 = RC1 + DG + 8;
+static_assert(3 * JW1 + 3 * RC1 <= kSweepFloats, "one-sided tables fit the two-sided area");
+static_assert(RC1 % RC == 0, "the cascade drains a one-sided chunk in RC-row parts");
 constexpr int PLCAP = 64 * KD;         // expanded pairs of one 64-entry queue batch
 constexpr int Q2CAP = 256;             // LB_Keogh queue (packed pairs)
 constexpr int Q2RUN = Q2CAP - 64;      // run LB_Keogh once this many pairs are queued
@@ -227,6 +231,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
 
     int q1n = 0, q2n = 0, q3n = 0;
     int64_t i_beg = 0, d0 = 0, c0 = 0;
+    int64_t cb = 0;  // first row of the RC rows staged for the cascade
     int m = 0;
     Thr th{};
     float *jarr = smem;               // NJARR x JW column tables of the sweep
@@ -324,7 +329,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
             count(C_CELLS, cells);
             count(C_DTW_AB, __popcll(__builtin_amdgcn_ballot_w64(ab)));
             if (lead && !ab) {
-                const int gi = (int)(c0 + li), gj = (int)(c0 + d0 + lj);
+                const int gi = (int)(cb + li), gj = (int)(cb + d0 + lj);
                 int slot = atomicAdd(A.n_cands, 1);
                 if (slot < A.cand_cap) A.cands[slot] = make_int4(gi, gj, __float_as_int(res), 0);
                 atomicMin(A.bsf, pack_bsf(res, (uint32_t)gi));
@@ -417,22 +422,37 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         }
     };
 
-    // ---- drain: stage the chunk in LDS, expand the queue entries into pairs, LB_Kim (stage 2)
-    // on 64-lane batches, feeding LB_Keogh and DTW; everything is flushed.
+    // ---- drain: the queue entries (sorted by row) are taken in parts of RC rows: stage the
+    // part in LDS, expand its entries into pairs, LB_Kim (stage 2) on 64-lane batches, feeding
+    // LB_Keogh and DTW; everything is flushed before the next part is staged.
     auto drain = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
         const int64_t n = A.n, nw = A.nw;
         const int CI = RC + m, CJ = RC + DG + m;
         __syncthreads();  // queue entries were stored by other lanes; column tables retire
-        // Windows of the chunk's rows (CI samples from c0) and columns (CJ samples from
-        // c0 + d0), then mean and 1/sigma of its RC rows and RC + DG columns.  Loads go out in
+        int h0 = 0;
+        while (h0 < q1n) {
+        // entries [h0, h1) lie in the RC-row part `part` (rows from the item start), which
+        // starts at row cb
+        const int part = (int)(q1[h0] >> 10) / RC;
+        int h1 = h0;
+        for (;;) {
+            const int idx = h1 + lane;
+            const bool in = idx < q1n && (int)(q1[idx] >> 10) / RC == part;
+            const int c = __popcll(__builtin_amdgcn_ballot_w64(in));  // a prefix of the 64
+            h1 += c;
+            if (c < 64) break;
+        }
+        cb = i_beg + (int64_t)RC * part;
+        // Windows of the part's rows (CI samples from cb) and columns (CJ samples from
+        // cb + d0), then mean and 1/sigma of its RC rows and RC + DG columns.  Loads go out in
         // batches of 8 per lane (indices clamped into the series, entries past the end set
         // after the load) so that their latencies overlap.
         {
             const global_ptr<float> ac = (global_ptr<float>)A.ac, uc = (global_ptr<float>)A.uc,
                                     lc = (global_ptr<float>)A.lc, muc = (global_ptr<float>)A.mu_c;
             const global_ptr<float> wf = (global_ptr<float>)A.win;  // WinRow as 8 floats
-            constexpr int WISIG = 4;                                // float index of isig
+            constexpr int WISIG = 2;                                // float index of isig
             static_assert(sizeof(WinRow) == 32 && offsetof(WinRow, isig) == 4 * WISIG, "WinRow layout");
             auto stage3 = [&](float *d0p, float *d1p, float *d2p, int64_t base, int cnt) __attribute__((always_inline)) {
                 for (int x0 = 0; x0 < cnt; x0 += 512) {
@@ -478,17 +498,17 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                     }
                 }
             };
-            stage3(sai, sui, sli, c0, CI);
-            stage3(saj, suj, slj, c0 + d0, CJ);
-            stage_stats(rmu, ris, c0, RC);
-            stage_stats(cmu, cis, c0 + d0, RC + DG);
+            stage3(sai, sui, sli, cb, CI);
+            stage3(saj, suj, slj, cb + d0, CJ);
+            stage_stats(rmu, ris, cb, RC);
+            stage_stats(cmu, cis, cb + d0, RC + DG);
         }
         __syncthreads();
-        const int row0 = (int)(c0 - i_beg);  // chunk start relative to the item start
-        for (int h = 0; h < q1n; h += 64) {
+        const int row0 = (int)(cb - i_beg);  // part start relative to the item start
+        for (int h = h0; h < h1; h += 64) {
             // expand up to 64 entries (row | lane | 4-bit mask) into pairs
             uint32_t e = 0;
-            if (h + lane < q1n) e = q1[h + lane];
+            if (h + lane < h1) e = q1[h + lane];
             const uint32_t mask = e & 15u;
             const int li = (int)(e >> 10) - row0;
             const int lb = li + KD * (int)((e >> 4) & 63u);
@@ -548,7 +568,10 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
             }
             __syncthreads();  // plist is rewritten by the next batch
         }
-        if (q2n > 0) process_q2();
+        if (q2n > 0) process_q2();  // queued pairs refer to this part's staging
+        __syncthreads();
+        h0 = h1;
+        }
         q1n = 0;
         __syncthreads();
     };
@@ -712,87 +735,110 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         const float EA = 0x1p-22f * steps * (float)m * smaxJ * svmaxI * ismaxJ;
         const float EB = 0x1p-22f * steps * (float)m * smaxI * svmaxJ * ismaxI;
 
-        for (c0 = i_beg; c0 < i_end; c0 += RC) {
-            const int nrows = (int)(((c0 + RC < i_end) ? c0 + RC : i_end) - c0);
+        // LDS tables of the sweep.  Two-sided: NJARR column tables of JW entries and NJARR row
+        // tables of RC entries.  One-sided: 3 + 3 tables of chunks RC1 rows high in the same area
+        // (columns dfa dga 1/sigma, rows dfv dgv thetaA), so that its chunks are 4x longer.
+        const int rc = two_sided ? RC : RC1;
+        float *const rarr1 = jarr + 3 * JW1;
+        for (c0 = i_beg; c0 < i_end; c0 += rc) {
+            const int nrows = (int)(((c0 + rc < i_end) ? c0 + rc : i_end) - c0);
             // rows of the chunk whose pair with column q is inside the triangle: r <= limr[q]
             int limr[KD];
 #pragma unroll
             for (int q = 0; q < KD; ++q) {
                 const int64_t l = lim[q] - c0;
-                limr[q] = l < -1 ? -1 : (l > RC ? RC : (int)l);
+                limr[q] = l < -1 ? -1 : (l > rc ? rc : (int)l);
             }
             // the whole chunk is inside the triangle for every lane
             const bool chunk_full = c0 + nrows - 1 + d0 + DG - 1 <= nw - 1;
             const uint32_t code0 = (uint32_t)(c0 - i_beg) << 10;
             int rl = 0;  // next local row (multiple of 8)
             while (rl < nrows) {
-                // Stage the chunk's row data (rows c0 + x, x < RC: dfa dga dfv dgv 1/sigma
-                // thetaA) and column data (columns j = c0 + d0 + x, x < JW: dfa dga dfv dgv
-                // 1/sigma thetaB) into LDS; the one-sided test needs only half of the tables.
-                // Every window load is issued before the first use (indices clamped into the
-                // series, entries past the end zeroed after the load), so that the latencies
-                // of the chunk's loads overlap instead of adding up.
-                {
+                // Stage the chunk's row data (rows c0 + x) and column data (columns c0 + d0 + x)
+                // into LDS: two-sided, dfa dga dfv dgv 1/sigma and thetaA / thetaB; one-sided,
+                // columns dfa dga 1/sigma (the first 16 bytes of a window record) and rows dfv
+                // dgv thetaA.  Every window load is issued before the first use (indices clamped
+                // into the series, entries past the end zeroed after the load), so that the
+                // latencies of the chunk's loads overlap instead of adding up.
+                const global_ptr<vf4> win = (global_ptr<vf4>)A.win;
+                static_assert(sizeof(WinRow) == 32 && offsetof(WinRow, isig) == 8 && offsetof(WinRow, dfv) == 12,
+                              "WinRow is read as two 16-byte vectors");
+                auto theta = [&](const vf4 &lo, const vf4 &hi, float T, float E) __attribute__((always_inline)) {
+                    // (mN, R, sig) = hi.y, hi.z, hi.w
+                    const float t = T + hi.z;
+                    return (0.5f * (hi.y - t * t) - 0x1p-20f * (hi.y + t * t)) * hi.w - E;
+                };
+                if (two_sided) {
                     constexpr int NRL = RC / 64, NCL = (JW + 63) / 64;
-                    static_assert(sizeof(WinRow) == 32, "WinRow is read as two 16-byte vectors");
-                    const global_ptr<vf4> win = (global_ptr<vf4>)A.win;
-                    WinRow rw[NRL], cw[NCL];
-                    auto get = [&](int64_t idx, WinRow &r) __attribute__((always_inline)) {
-                        const vf4 lo = win[2 * idx], hi = win[2 * idx + 1];
-                        r.dfa = lo.x;
-                        r.dga = lo.y;
-                        r.dfv = lo.z;
-                        r.dgv = lo.w;
-                        r.isig = hi.x;
-                        r.mN = hi.y;
-                        r.R = hi.z;
-                        r.sig = hi.w;
-                    };
+                    vf4 rlo[NRL], rhi[NRL], clo[NCL], chi[NCL];
 #pragma unroll
                     for (int k = 0; k < NRL; ++k) {
-                        const int64_t i = c0 + lane + 64 * k;
-                        get(i < nw - 1 ? i : nw - 1, rw[k]);
+                        const int64_t i = c0 + lane + 64 * k, ic = i < nw - 1 ? i : nw - 1;
+                        rlo[k] = win[2 * ic];
+                        rhi[k] = win[2 * ic + 1];
                     }
 #pragma unroll
                     for (int k = 0; k < NCL; ++k) {
-                        const int64_t j = c0 + d0 + lane + 64 * k;
-                        get(j < nw - 1 ? j : nw - 1, cw[k]);
+                        const int64_t j = c0 + d0 + lane + 64 * k, jc = j < nw - 1 ? j : nw - 1;
+                        clo[k] = win[2 * jc];
+                        chi[k] = win[2 * jc + 1];
                     }
                     th = load_threshold(A);
-                    const float T = th.T;
 #pragma unroll
                     for (int k = 0; k < NRL; ++k) {
                         const int x = lane + 64 * k;
-                        const WinRow &r = rw[k];
                         const bool ok = x < nrows;
-                        const float tA = T + r.R;
-                        const float thA = (0.5f * (r.mN - tA * tA) - 0x1p-20f * (r.mN + tA * tA)) * r.sig - EA;
-                        rarr[2 * RC + x] = ok ? r.dfv : 0.f;
-                        rarr[3 * RC + x] = ok ? r.dgv : 0.f;
-                        rarr[5 * RC + x] = ok ? thA : 0.f;
-                        if (two_sided) {
-                            rarr[0 * RC + x] = ok ? r.dfa : 0.f;
-                            rarr[1 * RC + x] = ok ? r.dga : 0.f;
-                            rarr[4 * RC + x] = ok ? r.isig : 0.f;
-                        }
+                        rarr[0 * RC + x] = ok ? rlo[k].x : 0.f;        // dfa
+                        rarr[1 * RC + x] = ok ? rlo[k].y : 0.f;        // dga
+                        rarr[2 * RC + x] = ok ? rlo[k].w : 0.f;        // dfv
+                        rarr[3 * RC + x] = ok ? rhi[k].x : 0.f;        // dgv
+                        rarr[4 * RC + x] = ok ? rlo[k].z : 0.f;        // 1/sigma
+                        rarr[5 * RC + x] = ok ? theta(rlo[k], rhi[k], th.T, EA) : 0.f;
                     }
 #pragma unroll
                     for (int k = 0; k < NCL; ++k) {
                         const int x = lane + 64 * k;
                         if (x < JW) {
-                            const WinRow &r = cw[k];
                             const bool ok = c0 + d0 + x <= nw - 1;
-                            jarr[0 * JW + x] = ok ? r.dfa : 0.f;
-                            jarr[1 * JW + x] = ok ? r.dga : 0.f;
-                            jarr[4 * JW + x] = ok ? r.isig : 0.f;
-                            if (two_sided) {
-                                const float tB = T + r.R;
-                                const float thB =
-                                    (0.5f * (r.mN - tB * tB) - 0x1p-20f * (r.mN + tB * tB)) * r.sig - EB;
-                                jarr[2 * JW + x] = ok ? r.dfv : 0.f;
-                                jarr[3 * JW + x] = ok ? r.dgv : 0.f;
-                                jarr[5 * JW + x] = ok ? thB : 0.f;
-                            }
+                            jarr[0 * JW + x] = ok ? clo[k].x : 0.f;
+                            jarr[1 * JW + x] = ok ? clo[k].y : 0.f;
+                            jarr[2 * JW + x] = ok ? clo[k].w : 0.f;
+                            jarr[3 * JW + x] = ok ? chi[k].x : 0.f;
+                            jarr[4 * JW + x] = ok ? clo[k].z : 0.f;
+                            jarr[5 * JW + x] = ok ? theta(clo[k], chi[k], th.T, EB) : 0.f;
+                        }
+                    }
+                } else {
+                    constexpr int NRL = RC1 / 64, NCL = (JW1 + 63) / 64;
+                    vf4 rlo[NRL], rhi[NRL], clo[NCL];
+#pragma unroll
+                    for (int k = 0; k < NRL; ++k) {
+                        const int64_t i = c0 + lane + 64 * k, ic = i < nw - 1 ? i : nw - 1;
+                        rlo[k] = win[2 * ic];
+                        rhi[k] = win[2 * ic + 1];
+                    }
+#pragma unroll
+                    for (int k = 0; k < NCL; ++k) {
+                        const int64_t j = c0 + d0 + lane + 64 * k, jc = j < nw - 1 ? j : nw - 1;
+                        clo[k] = win[2 * jc];
+                    }
+                    th = load_threshold(A);
+#pragma unroll
+                    for (int k = 0; k < NRL; ++k) {
+                        const int x = lane + 64 * k;
+                        const bool ok = x < nrows;
+                        rarr1[0 * RC1 + x] = ok ? rlo[k].w : 0.f;  // dfv
+                        rarr1[1 * RC1 + x] = ok ? rhi[k].x : 0.f;  // dgv
+                        rarr1[2 * RC1 + x] = ok ? theta(rlo[k], rhi[k], th.T, EA) : 0.f;
+                    }
+#pragma unroll
+                    for (int k = 0; k < NCL; ++k) {
+                        const int x = lane + 64 * k;
+                        if (x < JW1) {
+                            const bool ok = c0 + d0 + x <= nw - 1;
+                            jarr[0 * JW1 + x] = ok ? clo[k].x : 0.f;  // dfa
+                            jarr[1 * JW1 + x] = ok ? clo[k].y : 0.f;  // dga
+                            jarr[2 * JW1 + x] = ok ? clo[k].z : 0.f;  // 1/sigma
                         }
                     }
                 }
@@ -803,15 +849,14 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 // slot 3 takes column r + 4 lane + 4.  For the 4 rows of a half starting at r0
                 // (a multiple of 4) the entering columns are one aligned float4 per table at
                 // x = r0 + 4 (lane + 1), read from LDS (LDS pipe, no VALU shifts).
+                // Table r (dfa dga dfv dgv 1/sigma theta) of the mode's layout.
+                auto colp = [&](auto two_tag, int r) __attribute__((always_inline)) {
+                    return decltype(two_tag)::value ? jarr + r * JW : jarr + (r == 4 ? 2 : r) * JW1;
+                };
+                auto rowp = [&](auto two_tag, int r) __attribute__((always_inline)) {
+                    return decltype(two_tag)::value ? rarr + r * RC : rarr1 + (r == 5 ? 2 : r - 2) * RC1;
+                };
                 float cs[NJARR][KD];
-#pragma unroll
-                for (int r = 0; r < NJARR; ++r) {
-                    const float4 v4 = *reinterpret_cast<const float4 *>(&jarr[r * JW + rl + 4 * lane]);
-                    cs[r][0] = v4.x;
-                    cs[r][1] = v4.y;
-                    cs[r][2] = v4.z;
-                    cs[r][3] = v4.w;
-                }
                 // One 8-row group: rows rl..rl+7 of the chunk, in two halves of 4 rows whose
                 // entering columns are read into separate registers, so that the rotating
                 // column slots end every group in the registers they started in.  FULL: every
@@ -827,6 +872,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                     // thetaA [dfa dga 1/sigma]
                     auto colused = [](int r) { return TWO || r == 0 || r == 1 || r == 4; };
                     auto rowused = [](int r) { return TWO || r == 2 || r == 3 || r == 5; };
+                    (void)full_tag;
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
                         const int r0 = rl + 4 * h;
@@ -836,7 +882,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
 #pragma unroll
                         for (int r = 0; r < NJARR; ++r) {
                             if (rowused(r)) {
-                                const float4 v4 = *reinterpret_cast<const float4 *>(&rarr[r * RC + r0]);
+                                const float4 v4 = *reinterpret_cast<const float4 *>(rowp(two_tag, r) + r0);
                                 rv[r][0] = v4.x;
                                 rv[r][1] = v4.y;
                                 rv[r][2] = v4.z;
@@ -844,7 +890,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                             }
                             if (colused(r)) {
                                 const float4 n4 =
-                                    *reinterpret_cast<const float4 *>(&jarr[r * JW + r0 + 4 * lane + 4]);
+                                    *reinterpret_cast<const float4 *>(colp(two_tag, r) + r0 + 4 * lane + 4);
                                 nv[r][0] = n4.x;
                                 nv[r][1] = n4.y;
                                 nv[r][2] = n4.z;
@@ -919,24 +965,32 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 };
                 using T_ = std::integral_constant<bool, true>;
                 using F_ = std::integral_constant<bool, false>;
-                for (; rl < nrows; rl += 8) {
-                    const bool full8 = chunk_full && rl + 8 <= nrows;
-                    if (two_sided) {
-                        if (full8)
-                            group(T_(), T_());
-                        else
-                            group(F_(), T_());
-                    } else {
-                        if (full8)
-                            group(T_(), F_());
-                        else
-                            group(F_(), F_());
+                auto sweep = [&](auto two_tag) __attribute__((always_inline)) {
+                    constexpr bool TWO = decltype(two_tag)::value;
+#pragma unroll
+                    for (int r = 0; r < NJARR; ++r) {
+                        if (!(TWO || r == 0 || r == 1 || r == 4)) continue;
+                        const float4 v4 = *reinterpret_cast<const float4 *>(colp(two_tag, r) + rl + 4 * lane);
+                        cs[r][0] = v4.x;
+                        cs[r][1] = v4.y;
+                        cs[r][2] = v4.z;
+                        cs[r][3] = v4.w;
                     }
-                    if (q1n > QLIM) {
-                        rl += 8;
-                        break;
+                    for (; rl < nrows; rl += 8) {
+                        if (chunk_full && rl + 8 <= nrows)
+                            group(T_(), two_tag);
+                        else
+                            group(F_(), two_tag);
+                        if (q1n > QLIM) {
+                            rl += 8;
+                            break;
+                        }
                     }
-                }
+                };
+                if (two_sided)
+                    sweep(T_());
+                else
+                    sweep(F_());
                 // single drain point (end of chunk, or before the queue could overflow)
                 if (q1n > 0) drain();
                 __syncthreads();
