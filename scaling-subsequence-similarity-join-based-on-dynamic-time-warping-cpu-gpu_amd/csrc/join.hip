@@ -317,7 +317,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                     }
                 }
             }
-            if (w <= 63)
+            if (Q == 1 || w <= 63)  // Q = 1 launches hold bands up to 63 only (2w + 1 <= 128)
                 res = dtw_band<float, 1>(m, w, seg_len, nseg, th.dtw_ub, z, t_stop, suf);
             else
                 res = dtw_band<float, Q>(m, w, 64, 1, th.dtw_ub, z, t_stop, suf);
