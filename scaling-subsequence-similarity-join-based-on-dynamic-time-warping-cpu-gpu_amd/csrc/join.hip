@@ -90,8 +90,12 @@ __device__ inline float dtw_err(float D, const JoinArgs &A) {
     return A.rel_dtw * D + 4.0f * e * sqrtf(2.0f * A.m * D) + 8.0f * A.m * e * e;
 }
 
-__device__ inline Thr load_threshold(const JoinArgs &A) {
-    unsigned long long key = __hip_atomic_load(A.bsf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ inline unsigned long long load_bsf(const JoinArgs &A) {
+    return __hip_atomic_load(A.bsf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Thresholds of every stage from a best-so-far word (an older word is larger: still valid).
+__device__ inline Thr threshold_of(unsigned long long key, const JoinArgs &A) {
     const float cost = bsf_cost(key);
     Thr t;
     t.T2 = cost + dtw_err(cost, A);
@@ -101,6 +105,8 @@ __device__ inline Thr load_threshold(const JoinArgs &A) {
     t.dtw_ub = t.T2 + dtw_err(t.T2, A);
     return t;
 }
+
+__device__ inline Thr load_threshold(const JoinArgs &A) { return threshold_of(load_bsf(A), A); }
 
 __device__ inline float sq(float x) { return x * x; }
 
@@ -343,7 +349,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
     auto process_q2 = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
         __syncthreads();  // q2 entries were written by other lanes
-        th = load_threshold(A);
+        // th: the drain batch's thresholds (at most one batch old)
         const float sm = sqrtf((float)m);
         int taken = 0;     // queue entries handed out (uniform)
         bool act = false;  // this lane holds a pair
@@ -505,10 +511,16 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         }
         __syncthreads();
         const int row0 = (int)(cb - i_beg);  // part start relative to the item start
+        // the next batch's entries and best-so-far word are loaded one batch ahead, so that
+        // their latency overlaps the current batch (an older word is only larger: still valid)
+        uint32_t e_next = h0 + lane < h1 ? q1[h0 + lane] : 0u;
+        unsigned long long key_next = load_bsf(A);
         for (int h = h0; h < h1; h += 64) {
             // expand up to 64 entries (row | lane | 4-bit mask) into pairs
-            uint32_t e = 0;
-            if (h + lane < h1) e = q1[h + lane];
+            const uint32_t e = e_next;
+            const unsigned long long key = key_next;
+            e_next = h + 64 + lane < h1 ? q1[h + 64 + lane] : 0u;
+            key_next = load_bsf(A);
             const uint32_t mask = e & 15u;
             const int li = (int)(e >> 10) - row0;
             const int lb = li + KD * (int)((e >> 4) & 63u);
@@ -521,7 +533,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 if ((mask >> q) & 1u) plist[pos++] = pack_pair(li, lb + q);
             count(C_BALL, (unsigned long long)total);
             __syncthreads();
-            th = load_threshold(A);
+            th = threshold_of(key, A);
             for (int p = 0; p < total; p += 64) {
                 const bool act = p + lane < total;
                 int pi = 0, pj = 0;
