@@ -196,22 +196,51 @@ __device__ __forceinline__ T dtw_band(int m, int w, int seg_len, int nseg, T ub,
         odd_step(r, zr);
         t = 1;
     }
+    // z-values are read one iteration ahead (their latency overlaps a whole iteration): the
+    // row value of the pair of steps, and the two columns r + e and r + e + 1; the odd step's
+    // column is the next iteration's even-step column.
+    T zce[QP], zco[QP];
 #pragma unroll
-    for (int k = 0; k < QP; ++k) r[k] = (t - e_off[k]) >> 1;
+    for (int k = 0; k < QP; ++k) {
+        r[k] = (t - e_off[k]) >> 1;
+        zr[k] = z.row(clampi(r[k]));
+        zce[k] = z.col(clampi(r[k] + e_off[k]));
+        zco[k] = z.col(clampi(r[k] + e_off[k] + 1));
+    }
     for (; t <= t_end; t += 2) {
+        T zr_n[QP], zco_n[QP];
+#pragma unroll
+        for (int k = 0; k < QP; ++k) {
+            zr_n[k] = z.row(clampi(r[k] + 1));
+            zco_n[k] = z.col(clampi(r[k] + e_off[k] + 2));
+        }
         // even offsets at t: up = own odd (t-1), left = previous odd (t-1), diag = own even (t-2)
         const T left_in = from_left<T>(INF, O[QP - 1]);
 #pragma unroll
         for (int k = 0; k < QP; ++k) {
-            zr[k] = z.row(clampi(r[k]));
             const T left = k > 0 ? O[k - 1] : left_in;
-            const T d = zr[k] - z.col(clampi(r[k] + e_off[k]));
+            const T d = zr[k] - zce[k];
             const T v = fma(d, d, min3(E[k], O[k], left));
             E[k] = inE[k] ? v : INF;
         }
-        if (t + 1 <= t_end) odd_step(r, zr);
+        if (t + 1 <= t_end) {
+            // odd offsets: up = next even (t), left = own even (t), diag = own odd (t-1)
+            const T right_in = from_right<T>(INF, E[0]);
 #pragma unroll
-        for (int k = 0; k < QP; ++k) ++r[k];
+            for (int k = 0; k < QP; ++k) {
+                const T up = k + 1 < QP ? E[k + 1] : right_in;
+                const T d = zr[k] - zco[k];
+                const T v = fma(d, d, min3(O[k], up, E[k]));
+                O[k] = inO[k] ? v : INF;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < QP; ++k) {
+            ++r[k];
+            zr[k] = zr_n[k];
+            zce[k] = zco[k];
+            zco[k] = zco_n[k];
+        }
         if ((t & 15) >= 14 && t + 1 < t_end) {
             // segment minimum over both anti-diagonals held in the registers
             T v = INF;

@@ -355,6 +355,10 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         bool act = false;  // this lane holds a pair
         int li = 0, lj = 0, k = 0;
         float accA = 0.f, accB = 0.f, thrA = 0.f, thrB = 0.f, rA = 0.f, bA = 0.f, rB = 0.f, bB = 0.f;
+        // the DTW queue is run from one place (below) whenever it fills up and at the end, so
+        // that the DTW code is inlined once
+        bool more = true;
+        while (more) {
         for (;;) {
             // hand out queued pairs to idle lanes
             const bool need = !act;
@@ -383,7 +387,10 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
             }
             taken += __popcll(nk);
             if (taken > q2n) taken = q2n;
-            if (__builtin_amdgcn_ballot_w64(act) == 0ull) break;
+            if (__builtin_amdgcn_ballot_w64(act) == 0ull) {
+                more = false;
+                break;
+            }
             if (act) {
                 auto term = [&](int kk) __attribute__((always_inline)) {
                     const float x = saj[lj + kk];
@@ -415,17 +422,15 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 q3n += __popcll(dk);
                 count(C_KEOGH, __popcll(dk));
                 if (done) act = false;
-                if (q3n > Q3CAP - 64) {
-                    __syncthreads();
-                    process_q3();
-                }
+                if (q3n > Q3CAP - 64) break;
             }
         }
-        q2n = 0;
         if (q3n > 0) {
             __syncthreads();
             process_q3();
         }
+        }
+        q2n = 0;
     };
 
     // ---- drain: the queue entries (sorted by row) are taken in parts of RC rows: stage the
@@ -515,12 +520,15 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         // their latency overlaps the current batch (an older word is only larger: still valid)
         uint32_t e_next = h0 + lane < h1 ? q1[h0 + lane] : 0u;
         unsigned long long key_next = load_bsf(A);
-        for (int h = h0; h < h1; h += 64) {
+        // Batches of 64 entries; one extra pass after the last batch flushes the LB_Keogh queue,
+        // so that process_q2 (and the DTW inside it) is inlined at one place.
+        for (int h = h0;; h += 64) {
+            const bool last = h >= h1;
             // expand up to 64 entries (row | lane | 4-bit mask) into pairs
-            const uint32_t e = e_next;
+            const uint32_t e = last ? 0u : e_next;
             const unsigned long long key = key_next;
             e_next = h + 64 + lane < h1 ? q1[h + 64 + lane] : 0u;
-            key_next = load_bsf(A);
+            if (!last) key_next = load_bsf(A);
             const uint32_t mask = e & 15u;
             const int li = (int)(e >> 10) - row0;
             const int lb = li + KD * (int)((e >> 4) & 63u);
@@ -534,7 +542,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
             count(C_BALL, (unsigned long long)total);
             __syncthreads();
             th = threshold_of(key, A);
-            for (int p = 0; p < total; p += 64) {
+            for (int p = 0;; p += 64) {
                 const bool act = p + lane < total;
                 int pi = 0, pj = 0;
                 bool alive = false;
@@ -575,13 +583,14 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                     if (alive) q2[q2n + (int)mbcnt64(msk)] = pack_pair(pi, pj);
                     q2n += __popcll(msk);
                     count(C_KIM, __popcll(msk));
-                    if (q2n >= Q2RUN) process_q2();
                 }
+                // queued pairs refer to this part's staging: all of them run before the next part
+                if (q2n >= Q2RUN || (last && q2n > 0)) process_q2();
+                if (p + 64 >= total) break;
             }
             __syncthreads();  // plist is rewritten by the next batch
+            if (last) break;
         }
-        if (q2n > 0) process_q2();  // queued pairs refer to this part's staging
-        __syncthreads();
         h0 = h1;
         }
         q1n = 0;
