@@ -296,15 +296,20 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                         float vA = 0.f, vB = 0.f;
                         const int r0 = 8 * (x + seg_len * q + 1) + ((w + 1) >> 1);
                         if (seg < nseg && r0 < m) {
-                            const int r1 = r0 + 8 < m ? r0 + 8 : m;
-                            for (int r = r0; r < r1; ++r) {
-                                const float zi = z.row(r), zj = z.col(r);
-                                const float uj = (suj[lj + r] - mj) * ij, lwj = (slj[lj + r] - mj) * ij;
-                                const float ui = (sui[li + r] - mi) * ii, lwi = (sli[li + r] - mi) * ii;
-                                const float ta = fmaxf(fmaxf(zi - uj, lwj - zi), 0.f);
-                                const float tb = fmaxf(fmaxf(zj - ui, lwi - zj), 0.f);
-                                vA = fmaf(ta, ta, vA);
-                                vB = fmaf(tb, tb, vB);
+                            // the lane's 8 rows in an order skewed by the lane index: at each
+                            // step the lanes' addresses differ by 9 (odd), so no bank conflicts
+#pragma unroll
+                            for (int rr = 0; rr < 8; ++rr) {
+                                const int r = r0 + ((rr + x) & 7);
+                                if (r < m) {
+                                    const float zi = z.row(r), zj = z.col(r);
+                                    const float uj = (suj[lj + r] - mj) * ij, lwj = (slj[lj + r] - mj) * ij;
+                                    const float ui = (sui[li + r] - mi) * ii, lwi = (sli[li + r] - mi) * ii;
+                                    const float ta = fmaxf(fmaxf(zi - uj, lwj - zi), 0.f);
+                                    const float tb = fmaxf(fmaxf(zj - ui, lwi - zj), 0.f);
+                                    vA = fmaf(ta, ta, vA);
+                                    vB = fmaf(tb, tb, vB);
+                                }
                             }
                         }
 #pragma unroll
