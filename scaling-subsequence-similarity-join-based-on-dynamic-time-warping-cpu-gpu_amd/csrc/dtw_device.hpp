@@ -49,16 +49,6 @@ __device__ inline double dev_inf<double>() {
     return __builtin_huge_val();
 }
 
-template <typename T>
-__device__ inline T wave_min(T x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        T y = __shfl_xor(x, off, 64);
-        x = y < x ? y : x;
-    }
-    return x;
-}
-
 // Lane L receives src of lane L - 1 (lane 0: edge) / of lane L + 1 (lane 63: edge): DPP wave
 // shifts, the out-of-range lane keeping `edge`.
 __device__ inline uint32_t dpp_from_left_u32(uint32_t edge, uint32_t src) {
@@ -152,6 +142,9 @@ __device__ __forceinline__ T dtw_band(int m, int w, int seg_len, int nseg, T ub,
     const int seg = lane / seg_len;
     const int x = lane - seg * seg_len;
     const bool valid = seg < nseg;
+    // lanes of this lane's segment
+    const unsigned long long segmask =
+        valid ? (seg_len >= 64 ? ~0ull : ((1ull << seg_len) - 1ull) << (seg * seg_len)) : 0ull;
     T E[QP], O[QP];
     int e_off[QP];
     bool inE[QP], inO[QP];
@@ -242,26 +235,19 @@ __device__ __forceinline__ T dtw_band(int m, int w, int seg_len, int nseg, T ub,
             zco[k] = zco_n[k];
         }
         if ((t & 15) >= 14 && t + 1 < t_end) {
-            // segment minimum over both anti-diagonals held in the registers
+            // The segment is over ub when its minimum over both live anti-diagonals is; both
+            // tests are monotone in the value, so that holds exactly when every lane of the
+            // segment is over on its own cells: one ballot instead of a cross-lane minimum.
             T v = INF;
 #pragma unroll
             for (int k = 0; k < QP; ++k) v = min3(v, E[k], O[k]);
-            if (nseg == 1) {
-                v = wave_min(v);
-            } else {
-#pragma unroll
-                for (int off = 1; off < 64; off <<= 1) {
-                    const T y = __shfl_down(v, off, 64);
-                    if (x + off < seg_len) v = y < v ? y : v;
-                }
-                v = __shfl(v, seg * seg_len, 64);
-            }
             bool over = v > ub;
             if constexpr (SUF::enabled) {
                 const T sb = (T)suf.at(t);  // every lane takes part (the functor may shuffle)
                 over = over || (v + sb) * (T)(1.0 - 0x1p-9) > ub;
             }
-            if (alive && over) {
+            const unsigned long long keep = __builtin_amdgcn_ballot_w64(!over);
+            if (alive && (keep & segmask) == 0ull) {
                 alive = false;
                 t_stop = t + 1;
             }

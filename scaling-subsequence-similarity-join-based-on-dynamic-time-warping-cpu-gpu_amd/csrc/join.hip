@@ -263,9 +263,15 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         const int suf_blocks = (m - 8 - ((w + 1) >> 1) + 7) / 8;  // blocks with a row below m
         const int suf_rounds = suf_blocks > 0 ? (suf_blocks + seg_len - 1) / seg_len : 0;
         const bool use_suffix = w >= 16 && m <= 8192 && suf_rounds <= SUF_Q;
+        // the first batch uses the current thresholds; each later one the best-so-far word
+        // loaded one batch earlier (an older word is only larger: still valid)
+        unsigned long long key = load_bsf(A);
         for (int e0 = 0; e0 < q3n; e0 += per_wave) {
             const int nseg = (q3n - e0) < per_wave ? (q3n - e0) : per_wave;
-            th = load_threshold(A);
+            if (e0 > 0) {
+                th = threshold_of(key, A);
+                key = load_bsf(A);
+            }
             int li = 0, lj = 0;
             float mi = 0.f, ii = 0.f, mj = 0.f, ij = 0.f;
             if (seg < nseg) {
