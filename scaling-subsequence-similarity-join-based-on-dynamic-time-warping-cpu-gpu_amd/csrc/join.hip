@@ -142,17 +142,6 @@ __device__ inline unsigned int mbcnt64(unsigned long long msk) {
     return __builtin_amdgcn_mbcnt_hi((unsigned int)(msk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)msk, 0u));
 }
 
-// Inclusive prefix sum of x over the 64 lanes.
-__device__ inline int wave_scan_incl(int x) {
-    const int lane = (int)threadIdx.x & 63;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    return x;
-}
-
 // Maximum of blkmax[which] over the 256-window blocks covering windows [lo, hi].
 __device__ inline float range_max(const JoinArgs &A, int which, int64_t lo, int64_t hi) {
     float r = 0.f;
@@ -214,6 +203,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         if (lane == 0) wcnt[which] += v;
     };
     unsigned long long c_kelem = 0;  // per lane: LB_Keogh positions evaluated
+    uint32_t c_cells = 0;            // per lane (segment leaders): DTW cells, flushed below 2^31
 
     int cur = -1;  // series of the current item
     bool two_sided = true;    // sweep test of the current item
@@ -229,7 +219,9 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 if (wcnt[c]) atomicAdd(&Ap->counters[c], wcnt[c]);
         }
         if (c_kelem) atomicAdd(&Ap->counters[C_KEOGH_ELEMS], c_kelem);
+        if (c_cells) atomicAdd(&Ap->counters[C_CELLS], (unsigned long long)c_cells);
         c_kelem = 0;
+        c_cells = 0;
         __syncthreads();
         if (lane < 8) wcnt[lane] = 0ull;
         __syncthreads();
@@ -340,10 +332,15 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 res = dtw_band<float, Q>(m, w, 64, 1, th.dtw_ub, z, t_stop, suf);
             const bool lead = leader && seg < nseg;
             const bool ab = lead && !(res <= th.dtw_ub);
-            unsigned long long cells = lead ? (unsigned long long)dtw_cells_upto(m, w, t_stop) : 0ull;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) cells += __shfl_xor(cells, off, 64);
-            count(C_CELLS, cells);
+            if (lead) {
+                const unsigned long long cells = (unsigned long long)dtw_cells_upto(m, w, t_stop);
+                if (cells + c_cells >= (1ull << 31)) {
+                    atomicAdd(&A.counters[C_CELLS], cells + c_cells);
+                    c_cells = 0;
+                } else {
+                    c_cells += (uint32_t)cells;
+                }
+            }
             count(C_DTW_AB, __popcll(__builtin_amdgcn_ballot_w64(ab)));
             if (lead && !ab) {
                 const int gi = (int)(cb + li), gj = (int)(cb + d0 + lj);
@@ -543,13 +540,16 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
             const uint32_t mask = e & 15u;
             const int li = (int)(e >> 10) - row0;
             const int lb = li + KD * (int)((e >> 4) & 63u);
-            const int cnt = __popc(mask);
-            const int incl = wave_scan_incl(cnt);
-            const int total = __shfl(incl, 63, 64);
-            int pos = incl - cnt;
+            // pairs listed diagonal-bit-major: the pairs of bit q follow those of bits < q, in
+            // lane order (one ballot per bit instead of a cross-lane prefix sum)
+            int total = 0;
 #pragma unroll
-            for (int q = 0; q < KD; ++q)
-                if ((mask >> q) & 1u) plist[pos++] = pack_pair(li, lb + q);
+            for (int q = 0; q < KD; ++q) {
+                const bool has = (mask >> q) & 1u;
+                const unsigned long long bq = __builtin_amdgcn_ballot_w64(has);
+                if (has) plist[total + (int)mbcnt64(bq)] = pack_pair(li, lb + q);
+                total += __popcll(bq);
+            }
             count(C_BALL, (unsigned long long)total);
             __syncthreads();
             th = threshold_of(key, A);
