@@ -400,14 +400,15 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 break;
             }
             if (act) {
+                // x - (U rA + bA) = (x - bA) - U rA and (L rA + bA) - x = L rA - (x - bA): one
+                // subtraction and two FMAs per direction (rounding error <= 2u (|x| + |U rA| +
+                // |bA|), inside the 4u of eA / eB)
                 auto term = [&](int kk) __attribute__((always_inline)) {
-                    const float x = saj[lj + kk];
-                    const float hi = fmaf(sui[li + kk], rA, bA), lo = fmaf(sli[li + kk], rA, bA);
-                    const float t = fmaxf(fmaxf(x - hi, lo - x), 0.f);
+                    const float cx = saj[lj + kk] - bA;
+                    const float t = fmaxf(fmaxf(fmaf(-sui[li + kk], rA, cx), fmaf(sli[li + kk], rA, -cx)), 0.f);
                     accA = fmaf(t, t, accA);
-                    const float y = sai[li + kk];
-                    const float hj = fmaf(suj[lj + kk], rB, bB), lj_ = fmaf(slj[lj + kk], rB, bB);
-                    const float u = fmaxf(fmaxf(y - hj, lj_ - y), 0.f);
+                    const float cy = sai[li + kk] - bB;
+                    const float u = fmaxf(fmaxf(fmaf(-suj[lj + kk], rB, cy), fmaf(slj[lj + kk], rB, -cy)), 0.f);
                     accB = fmaf(u, u, accB);
                 };
                 if (k + 16 <= m) {
