@@ -13,7 +13,7 @@ positions x 20, DTW cells x 5)
 / its HIP-event time, peak 157.3 TFLOP/s.  traffic = HBM bytes per launch from the rocprofv3
 FETCH_SIZE / WRITE_SIZE summary under profiles/ for the same workload (null otherwise).
 A second workload (band ratio 0.05: seed 3, 5 plants, w = 13) is timed beside the headline one
-and reported under "wide_band"; the same workload with the learned proposal against the plain
+and reported under "band_ratio_0_05"; the same workload with the learned proposal against the plain
 join under "learned"; the 512-series batched form under "batched".
 cpu_baseline: the repository's FP64 CPU restatement of the reference join (oracle/, the
 "port"), single-threaded, on a bounded prefix of the same series, extrapolated by pair count.
@@ -72,9 +72,10 @@ def run_workload(d, n, m, w, seed, plants, steps, warmup, opts):
     return series, results, walls, planted
 
 
-def learned_leg(d, n, m, steps, warmup):
+def learned_leg(d, n, m, steps, warmup, cpu_budget=0.0):
     """Learned proposal vs the plain GPU join on the band-ratio-0.05 workload (seed 3, 5 planted
-    motifs, w = 13), both from host memory, with the committed trained network."""
+    motifs, w = 13), both from host memory, with the committed trained network; with a CPU
+    budget, the oracle's time on the same workload beside them (bounded prefix, extrapolated)."""
     w = 13
     series, _, _ = d.generate(n, 3, n_plants=5, plant_len=m)
     model = d.load_mlp(os.path.join(PKG_DIR, "models", "proposal_m256_w13.json"))
@@ -106,6 +107,7 @@ def learned_leg(d, n, m, steps, warmup):
                     "join_kernel_s": rl.join_kernel_ms * 1e-3, "propose_s": rep["t_propose"],
                     "pass_rates": rates(rl), "candidates_scored": rep["candidates"], "verified": rep["verified"],
                     "regret": rep["regret"], "regret_ed_order": rep["regret_ed"]},
+        "cpu_baseline": cpu_baseline(series, m, w, cpu_budget) if cpu_budget > 0 else None,
     }
 
 
@@ -189,6 +191,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--wide-1024", action="store_true",
+                    help="also run the wide-band workload once (n=4M, m=1024, w=102; about 2 minutes)")
     ap.add_argument("--rows-per-item", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     args = ap.parse_args()
@@ -244,7 +248,7 @@ def main():
         _, res5, walls5, planted5 = run_workload(d, args.n, args.m, 13, 3, 5, args.steps, args.warmup, opts)
         r5 = res5[-1]
         k5 = float(np.mean([x.join_kernel_ms for x in res5]))
-        line["wide_band"] = {
+        line["band_ratio_0_05"] = {
             "config": {"workload": "top1_dtw_join_plain", "n": args.n, "m": args.m, "w": 13, "seed": 3,
                        "planted_pairs": 5},
             "time_to_exact_top1_s": float(np.mean(walls5)), "join_kernel_s": k5 * 1e-3,
@@ -256,9 +260,25 @@ def main():
                            "keogh_to_dtw": r5.pass_keogh / r5.pairs_swept},
         }
     if not args.no_learned:
-        line["learned"] = learned_leg(d, args.n, args.m, args.steps, args.warmup)
+        line["learned"] = learned_leg(d, args.n, args.m, args.steps, args.warmup,
+                                      0.0 if args.no_cpu_baseline else args.cpu_budget)
     if not args.no_batched:
         line["batched"] = batched_leg(d, args.batch_count)
+    if args.wide_1024:
+        # random walk n = 4M, seed 2, step std alternating 0.5 / 2.0 every 100k samples
+        wa, _, _ = d.generate(4_000_000, 2, step_std=0.5, step_std_alt=2.0, regime_len=100_000)
+        t0 = time.perf_counter()
+        rw = d.top1(wa, 1024, 102)
+        tw = time.perf_counter() - t0
+        line["wide_band_m1024"] = {
+            "config": {"workload": "top1_dtw_join_plain", "n": 4_000_000, "m": 1024, "w": 102, "seed": 2,
+                       "regimes": "step std 0.5 / 2.0, 100k samples each"},
+            "time_to_exact_top1_s": tw, "join_kernel_s": rw.join_kernel_ms * 1e-3,
+            "top1": {"i": rw.i, "j": rw.j, "dist": rw.dist, "dist_fp32": rw.dist_fp32},
+            "dtw_cells": rw.dtw_cells, "dtw_cells_per_s": rw.dtw_cells / (rw.join_kernel_ms * 1e-3),
+            "pass_rates": {"ball": rw.pass_ball / rw.pairs_swept, "kim": rw.pass_kim / rw.pairs_swept,
+                           "keogh_to_dtw": rw.pass_keogh / rw.pairs_swept},
+        }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(series, args.m, args.w, args.cpu_budget)
     print(json.dumps(line), flush=True)
