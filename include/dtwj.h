@@ -50,7 +50,9 @@ typedef struct dtwj_options {
                               -1 = launch kernels directly) */
     int32_t autotune;      /* batch: 1 = pick rows_per_item / waves_per_cu by timing probe joins
                               on a sample of the batch (0 = use the fields above or defaults) */
-    int32_t reserved[5];
+    int32_t sweep;         /* diagonal sweep test: 0 = adaptive per work item (default), 1 = always
+                              the two-sided ball bound, 2 = always the one-sided one (both exact) */
+    int32_t reserved[4];
 } dtwj_options;
 
 typedef struct dtwj_result {

@@ -37,7 +37,7 @@ class DTWJError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("seed_items", ctypes.c_int32), ("rows_per_item", ctypes.c_int32),
                 ("waves_per_cu", ctypes.c_int32), ("verbose", ctypes.c_int32), ("graph", ctypes.c_int32),
-                ("autotune", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("autotune", ctypes.c_int32), ("sweep", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
 
 
 class Result(ctypes.Structure):
@@ -218,12 +218,13 @@ def _result(r):
     return Top1(**{k: getattr(r, k) for k, _ in Result._fields_ if not k.startswith("reserved")})
 
 
-def make_options(device=0, seed_items=0, rows_per_item=0, waves_per_cu=0, verbose=False, graph=0, autotune=0):
+def make_options(device=0, seed_items=0, rows_per_item=0, waves_per_cu=0, verbose=False, graph=0, autotune=0,
+                 sweep=0):
     o = Options()
     lib().dtwj_default_options(ctypes.byref(o))
     o.device, o.seed_items, o.rows_per_item, o.waves_per_cu, o.verbose = (
         device, seed_items, rows_per_item, waves_per_cu, int(bool(verbose)))
-    o.graph, o.autotune = int(graph), int(autotune)
+    o.graph, o.autotune, o.sweep = int(graph), int(autotune), int(sweep)
     return o
 
 

@@ -146,6 +146,7 @@ struct JoinArgs {
     float eps_z;       // bound on |z32 - z| for z computed from ac/mu_c/isig
     float rel_dtw;     // relative inflation of the best-so-far cost used as pruning threshold
     int32_t valid;     // 0: the preprocessing flagged the series; the join skips it
+    int32_t sweep_mode;  // 0 adaptive, 1 two-sided, 2 one-sided ball test in the sweep
 };
 
 // seed scan: per-item best ED pair

@@ -173,6 +173,7 @@ struct Job {
     int64_t n = 0, nw = 0, K = 0;
     int32_t m = 0, w = 0;  // w already clamped to m - 1
     int32_t groups = 0, rows_per_item = 0, n_seed = 0, cand_cap = 0;
+    int32_t sweep_mode = 0;  // dtwj_options::sweep
     int64_t n_items = 0;
     std::vector<int64_t> item_start;
     std::vector<int2> extra;  // explicit seed pairs (besides the scan's and (0, m))
@@ -307,6 +308,7 @@ void take_job(Arena &ar, Job &J) {
     A.n_cands = J.d_ncands;
     A.cand_cap = J.cand_cap;
     A.counters = J.d_counters;
+    A.sweep_mode = J.sweep_mode;
 }
 
 // Host-to-device copies a job needs before its kernels run (item offsets, explicit seeds).
@@ -643,9 +645,11 @@ struct Seeds {
 
 int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w_in, const dtwj_options *opt,
              const Seeds &seeds, dtwj_result *out, clk::time_point t_entry) {
+    if (opt && (opt->sweep < 0 || opt->sweep > 2)) return fail(DTWJ_EINVAL, "options.sweep must be 0, 1 or 2");
     Job J;
     plan_job(J, d_a, n, m, w_in, opt ? opt->rows_per_item : 0, opt ? opt->seed_items : 0, seeds.gpu_scan, seeds.pairs,
              kCandCap);
+    J.sweep_mode = opt ? opt->sweep : 0;
     const int32_t wpc = (opt && opt->waves_per_cu > 0) ? opt->waves_per_cu : 8;
     const int32_t grid = grid_for(c, wpc, J.n_items);
     JoinArgs *d_args = nullptr;
@@ -732,6 +736,7 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
 // the series is left out; *join_ms receives the device time of the captured work.
 int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const dtwj_options *opt, int32_t rows_req,
               int32_t wpc, dtwj_result *out, int32_t *status, float *join_ms) {
+    if (opt && (opt->sweep < 0 || opt->sweep > 2)) return fail(DTWJ_EINVAL, "options.sweep must be 0, 1 or 2");
     std::vector<Job> jobs;
     std::vector<int32_t> owner;  // job -> batch index
     jobs.reserve(count);
@@ -749,6 +754,7 @@ int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const d
         if (status[k] != DTWJ_OK) continue;
         jobs.emplace_back();
         plan_job(jobs.back(), nullptr, S.n, S.m, S.w, rows_req, opt ? opt->seed_items : 0, true, {}, kBatchCandCap);
+        jobs.back().sweep_mode = opt ? opt->sweep : 0;
         owner.push_back(k);
     }
     const int32_t nj = (int32_t)jobs.size();

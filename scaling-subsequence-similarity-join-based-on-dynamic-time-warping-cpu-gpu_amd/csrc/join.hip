@@ -649,7 +649,10 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         // Sweep test for this item: the one-sided ball bound (X^A only, half the work per pair)
         // when the previous item of this series left almost no survivors; the two-sided one
         // otherwise.  Both are valid bounds; the cascade's two-sided LB_Keogh follows either.
-        if (s != prev_series) {
+        if (A.sweep_mode != 0) {  // fixed by the caller
+            two_sided = A.sweep_mode == 1;
+            prev_series = s;
+        } else if (s != prev_series) {
             two_sided = true;
             prev_series = s;
         } else if (prev_rows > 0) {
