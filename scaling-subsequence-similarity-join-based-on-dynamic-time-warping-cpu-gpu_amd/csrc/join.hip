@@ -362,6 +362,15 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         int taken = 0;     // queue entries handed out (uniform)
         bool act = false;  // this lane holds a pair
         int li = 0, lj = 0, k = 0;
+        // Windows of 256 and more are taken in blocks of 16 positions spread over the window
+        // (stride 16 within superblocks of 256 positions, the superblocks interleaved), so that
+        // a pair's bound grows with samples of the whole window and abandons sooner than in
+        // window order; positions past the last whole superblock (and shorter windows) follow
+        // in order.  kb, kc: block and superblock of the lane's next strided block.
+        constexpr int KS = 16;
+        const int nsup = m / (16 * KS);      // superblocks
+        const int nstr = 16 * KS * nsup;     // positions taken strided
+        int kb = 0, kc = 0;
         float accA = 0.f, accB = 0.f, thrA = 0.f, thrB = 0.f, rA = 0.f, bA = 0.f, rB = 0.f, bB = 0.f;
         // the DTW queue is run from one place (below) whenever it fills up and at the end, so
         // that the DTW code is inlined once
@@ -391,6 +400,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 thrB = (T2 + 2.f * eB * sqrtf(m * T2) + m * eB * eB + m * 0x1p-22f * T2) * si2 * (1.f + 0x1p-20f);
                 accA = accB = 0.f;
                 k = 0;
+                kb = kc = 0;
                 act = true;
             }
             taken += __popcll(nk);
@@ -411,8 +421,32 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                     const float u = fmaxf(fmaxf(fmaf(-suj[lj + kk], rB, cy), fmaf(slj[lj + kk], rB, -cy)), 0.f);
                     accB = fmaf(u, u, accB);
                 };
-                if (k + 16 <= m) {
-                    // fixed block: the 96 LDS reads are independent and issue back to back
+                auto strided = [&]() __attribute__((always_inline)) {
+                    constexpr int S = KS;
+                    const int base = 16 * S * kc + kb;
+                    // block base pointers, so that the 16 reads per array use immediate offsets
+                    const float *xaj = saj + lj + base, *xui = sui + li + base, *xli = sli + li + base;
+                    const float *xai = sai + li + base, *xuj = suj + lj + base, *xlj = slj + lj + base;
+#pragma unroll
+                    for (int t = 0; t < 16; ++t) {
+                        const float cx = xaj[S * t] - bA;
+                        const float ta = fmaxf(fmaxf(fmaf(-xui[S * t], rA, cx), fmaf(xli[S * t], rA, -cx)), 0.f);
+                        accA = fmaf(ta, ta, accA);
+                        const float cy = xai[S * t] - bB;
+                        const float tb = fmaxf(fmaxf(fmaf(-xuj[S * t], rB, cy), fmaf(xlj[S * t], rB, -cy)), 0.f);
+                        accB = fmaf(tb, tb, accB);
+                    }
+                    if (++kc == nsup) {
+                        kc = 0;
+                        ++kb;
+                    }
+                };
+                if (k < nstr) {
+                    // fixed blocks: the 96 LDS reads are independent and issue back to back
+                    strided();
+                    k += 16;
+                    c_kelem += 16ull;
+                } else if (k + 16 <= m) {
 #pragma unroll
                     for (int kk = 0; kk < 16; ++kk) term(k + kk);
                     k += 16;
