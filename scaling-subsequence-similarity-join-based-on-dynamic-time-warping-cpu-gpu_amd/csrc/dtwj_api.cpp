@@ -188,9 +188,11 @@ struct Job {
 };
 
 // Sweep tile height: a multiple of 128 rows, at most 2^20 (queue entries carry the row
-// offset inside the item in 22 bits).
-int32_t normalise_rows(int32_t r) {
-    r = r > 0 ? std::min(r, 1 << 20) : 2048;
+// offset inside the item in 22 bits).  Automatic: 3072 rows for bands up to 3 (survivors are
+// rare there, so a taller item amortises its FP64 anchor at little cost in the FP32 drift
+// margin), 2048 otherwise.
+int32_t normalise_rows(int32_t r, int32_t w) {
+    r = r > 0 ? std::min(r, 1 << 20) : (w <= 3 ? 3072 : 2048);
     return std::max(128, (r + 127) / 128 * 128);
 }
 
@@ -203,7 +205,7 @@ void plan_job(Job &J, const double *d_a, int64_t n, int32_t m, int32_t w_in, int
     J.nw = n - m + 1;
     J.K = n - 2 * (int64_t)m + 1;  // diagonal offsets m .. n-m
     J.groups = (int32_t)((J.K + kDiagGroup - 1) / kDiagGroup);
-    J.rows_per_item = normalise_rows(rows_per_item);
+    J.rows_per_item = normalise_rows(rows_per_item, J.w);
     J.item_start.assign(J.groups + 1, 0);
     for (int32_t g = 0; g < J.groups; ++g) {
         const int64_t rows_g = J.nw - ((int64_t)m + (int64_t)kDiagGroup * g);
