@@ -179,6 +179,29 @@ struct LaneSuffix {
     }
 };
 
+// LDS layout of the cascade's staging (valid while draining) for window length m: windows of
+// the part's RC rows (CI samples: series, upper and lower envelope) and of its RC + DG columns
+// (CJ samples), then mean and 1/sigma of the rows and of the columns.  Built where it is used
+// from an opaque copy of m, so that the ten addresses are not kept live (in scalar registers
+// that would spill) through the sweep.
+struct CascadeLDS {
+    float *sai, *sui, *sli, *saj, *suj, *slj, *rmu, *ris, *cmu, *cis;
+    __device__ CascadeLDS(float *base, int m) {
+        asm volatile("" : "+s"(m));
+        const int CI = RC + m, CJ = RC + DG + m;
+        sai = base;
+        sui = sai + CI;
+        sli = sui + CI;
+        saj = sli + CI;
+        suj = saj + CJ;
+        slj = suj + CJ;
+        rmu = slj + CJ;
+        ris = rmu + RC;
+        cmu = ris + RC;
+        cis = cmu + (RC + DG);
+    }
+};
+
 // Packed (row, column) offsets of a pair inside the staged chunk (both < 2^16).
 __device__ inline uint32_t pack_pair(int li, int lj) { return (uint32_t)li | ((uint32_t)lj << 16); }
 __device__ inline int pair_row(uint32_t p) { return (int)(p & 0xffffu); }
@@ -235,8 +258,6 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
     float *jarr = smem;               // NJARR x JW column tables of the sweep
     float *rarr = jarr + NJARR * JW;  // NJARR x RC row tables of the sweep
     // cascade staging (valid only while draining), laid out per series from its window length
-    float *sai = smem, *sui = smem, *sli = smem, *saj = smem, *suj = smem, *slj = smem;
-    float *rmu = smem, *ris = smem, *cmu = smem, *cis = smem;
     uint32_t *plist = reinterpret_cast<uint32_t *>(smem + uni);  // expanded pairs of a batch
     uint32_t *q2 = plist + PLCAP;                                // LB_Keogh queue
     uint32_t *q3 = q2 + Q2CAP;                                   // DTW queue
@@ -245,6 +266,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
     // (segments of w + 1 lanes); wider bands one pair per wave with QP offset pairs per lane.
     auto process_q3 = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
+        const CascadeLDS L(smem, m);
         const int w = A.w;
         const int seg_len = w <= 63 ? w + 1 : 64;
         const int per_wave = 64 / seg_len;
@@ -270,12 +292,12 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 const uint32_t pr = q3[e0 + seg];
                 li = pair_row(pr);
                 lj = pair_col(pr);
-                mi = rmu[li];
-                ii = ris[li];
-                mj = cmu[lj];
-                ij = cis[lj];
+                mi = L.rmu[li];
+                ii = L.ris[li];
+                mj = L.cmu[lj];
+                ij = L.cis[lj];
             }
-            const StagedZ z{sai + li, saj + lj, mi, ii, mj, ij};
+            const StagedZ z{L.sai + li, L.saj + lj, mi, ii, mj, ij};
             int t_stop = 0;
             float res;
             // block sums of the LB_Keogh terms of the lane's blocks and their reverse prefix sums
@@ -301,8 +323,8 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                                 const int r = r0 + ((rr + x) & 7);
                                 if (r < m) {
                                     const float zi = z.row(r), zj = z.col(r);
-                                    const float uj = (suj[lj + r] - mj) * ij, lwj = (slj[lj + r] - mj) * ij;
-                                    const float ui = (sui[li + r] - mi) * ii, lwi = (sli[li + r] - mi) * ii;
+                                    const float uj = (L.suj[lj + r] - mj) * ij, lwj = (L.slj[lj + r] - mj) * ij;
+                                    const float ui = (L.sui[li + r] - mi) * ii, lwi = (L.sli[li + r] - mi) * ii;
                                     const float ta = fmaxf(fmaxf(zi - uj, lwj - zi), 0.f);
                                     const float tb = fmaxf(fmaxf(zj - ui, lwi - zj), 0.f);
                                     vA = fmaf(ta, ta, vA);
@@ -356,6 +378,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
     // the next pair as soon as its pair is abandoned or complete.  Consumes the whole queue.
     auto process_q2 = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
+        const CascadeLDS L(smem, m);
         __syncthreads();  // q2 entries were written by other lanes
         // th: the drain batch's thresholds (at most one batch old)
         const float sm = sqrtf((float)m);
@@ -385,8 +408,8 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 const uint32_t pr = q2[idx];
                 li = pair_row(pr);
                 lj = pair_col(pr);
-                const float isi = ris[li], isj = cis[lj];
-                const float mci = rmu[li], mcj = cmu[lj];
+                const float isi = L.ris[li], isj = L.cis[lj];
+                const float mci = L.rmu[li], mcj = L.cmu[lj];
                 // raw envelope of window i mapped into the units of window j, and the reverse
                 rA = isi / isj;  // sigma_j / sigma_i
                 bA = mcj - mci * rA;
@@ -414,19 +437,19 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 // subtraction and two FMAs per direction (rounding error <= 2u (|x| + |U rA| +
                 // |bA|), inside the 4u of eA / eB)
                 auto term = [&](int kk) __attribute__((always_inline)) {
-                    const float cx = saj[lj + kk] - bA;
-                    const float t = fmaxf(fmaxf(fmaf(-sui[li + kk], rA, cx), fmaf(sli[li + kk], rA, -cx)), 0.f);
+                    const float cx = L.saj[lj + kk] - bA;
+                    const float t = fmaxf(fmaxf(fmaf(-L.sui[li + kk], rA, cx), fmaf(L.sli[li + kk], rA, -cx)), 0.f);
                     accA = fmaf(t, t, accA);
-                    const float cy = sai[li + kk] - bB;
-                    const float u = fmaxf(fmaxf(fmaf(-suj[lj + kk], rB, cy), fmaf(slj[lj + kk], rB, -cy)), 0.f);
+                    const float cy = L.sai[li + kk] - bB;
+                    const float u = fmaxf(fmaxf(fmaf(-L.suj[lj + kk], rB, cy), fmaf(L.slj[lj + kk], rB, -cy)), 0.f);
                     accB = fmaf(u, u, accB);
                 };
                 auto strided = [&]() __attribute__((always_inline)) {
                     constexpr int S = KS;
                     const int base = 16 * S * kc + kb;
                     // block base pointers, so that the 16 reads per array use immediate offsets
-                    const float *xaj = saj + lj + base, *xui = sui + li + base, *xli = sli + li + base;
-                    const float *xai = sai + li + base, *xuj = suj + lj + base, *xlj = slj + lj + base;
+                    const float *xaj = L.saj + lj + base, *xui = L.sui + li + base, *xli = L.sli + li + base;
+                    const float *xai = L.sai + li + base, *xuj = L.suj + lj + base, *xlj = L.slj + lj + base;
 #pragma unroll
                     for (int t = 0; t < 16; ++t) {
                         const float cx = xaj[S * t] - bA;
@@ -481,6 +504,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
     // LB_Keogh and DTW; everything is flushed before the next part is staged.
     auto drain = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
+        const CascadeLDS L(smem, m);
         const int64_t n = A.n, nw = A.nw;
         const int CI = RC + m, CJ = RC + DG + m;
         __syncthreads();  // queue entries were stored by other lanes; column tables retire
@@ -552,10 +576,10 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                     }
                 }
             };
-            stage3(sai, sui, sli, cb, CI);
-            stage3(saj, suj, slj, cb + d0, CJ);
-            stage_stats(rmu, ris, cb, RC);
-            stage_stats(cmu, cis, cb + d0, RC + DG);
+            stage3(L.sai, L.sui, L.sli, cb, CI);
+            stage3(L.saj, L.suj, L.slj, cb + d0, CJ);
+            stage_stats(L.rmu, L.ris, cb, RC);
+            stage_stats(L.cmu, L.cis, cb + d0, RC + DG);
         }
         __syncthreads();
         const int row0 = (int)(cb - i_beg);  // part start relative to the item start
@@ -596,13 +620,18 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                     const uint32_t pr = plist[p + lane];
                     pi = pair_row(pr);
                     pj = pair_col(pr);
-                    const float mi = rmu[pi], ii = ris[pi], mj = cmu[pj], ij = cis[pj];
-                    const float *ai = sai + pi, *aj = saj + pj;
+                    const float *ai = L.sai + pi, *aj = L.saj + pj;
+                    // the 16 LDS reads of the pair go out together (one wait), then the arithmetic
+                    const float mi = L.rmu[pi], ii = L.ris[pi], mj = L.cmu[pj], ij = L.cis[pj];
+                    const float a0 = ai[0], a1 = ai[1], a2 = ai[2], b0 = aj[0], b1 = aj[1], b2 = aj[2];
+                    const float ae0 = ai[m - 1], ae1 = ai[m - 2], ae2 = ai[m - 3];
+                    const float be0 = aj[m - 1], be1 = aj[m - 2], be2 = aj[m - 3];
+                    __builtin_amdgcn_sched_barrier(0);
                     // x: window i (columns), y: window j (rows)
-                    const float x0 = (ai[0] - mi) * ii, x1 = (ai[1] - mi) * ii, x2 = (ai[2] - mi) * ii;
-                    const float y0 = (aj[0] - mj) * ij, y1 = (aj[1] - mj) * ij, y2 = (aj[2] - mj) * ij;
-                    const float xe0 = (ai[m - 1] - mi) * ii, xe1 = (ai[m - 2] - mi) * ii, xe2 = (ai[m - 3] - mi) * ii;
-                    const float ye0 = (aj[m - 1] - mj) * ij, ye1 = (aj[m - 2] - mj) * ij, ye2 = (aj[m - 3] - mj) * ij;
+                    const float x0 = (a0 - mi) * ii, x1 = (a1 - mi) * ii, x2 = (a2 - mi) * ii;
+                    const float y0 = (b0 - mj) * ij, y1 = (b1 - mj) * ij, y2 = (b2 - mj) * ij;
+                    const float xe0 = (ae0 - mi) * ii, xe1 = (ae1 - mi) * ii, xe2 = (ae2 - mi) * ii;
+                    const float ye0 = (be0 - mj) * ij, ye1 = (be1 - mj) * ij, ye2 = (be2 - mj) * ij;
                     // Any warping path from (0,0) first leaves {0,1}^2 at (2,0),(2,1),(2,2),(1,2) or
                     // (0,2), entered from an adjacent cell of {0,1}^2.  Same at the far corner.
                     const float c00 = sq(y0 - x0), c10 = sq(y1 - x0), c01 = sq(y0 - x1), c11 = sq(y1 - x1);
@@ -666,17 +695,6 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
             cur = s;
             Ap = series + s;
             m = Ap->m;
-            const int CI = RC + m, CJ = RC + DG + m;
-            sai = smem;
-            sui = sai + CI;
-            sli = sui + CI;
-            saj = sli + CI;
-            suj = saj + CJ;
-            slj = suj + CJ;
-            rmu = slj + CJ;
-            ris = rmu + RC;
-            cmu = ris + RC;
-            cis = cmu + (RC + DG);
         }
         const JoinArgs &A = *Ap;
         if (!A.valid) continue;
