@@ -87,9 +87,10 @@ __device__ inline double from_right<double>(double edge, double src) {
 __device__ inline float min3(float a, float b, float c) { return fminf(fminf(a, b), c); }
 __device__ inline double min3(double a, double b, double c) { return fmin(fmin(a, b), c); }
 
-// z-values of a DTW pair already normalised in memory.
+// z-values of a DTW pair already normalised in memory (indices must lie in [0, m-1]).
 template <typename T>
 struct PlainZ {
+    static constexpr bool kPadded = false;
     const T *zi, *zj;
     __device__ T row(int r) const { return zi[r]; }
     __device__ T col(int c) const { return zj[c]; }
@@ -126,8 +127,11 @@ inline int dtw_qp_for(int w) {
 }
 
 // Banded DTW of up to `nseg` pairs at once (QP = 1: segments of seg_len lanes, seg_len >= w+1;
-// QP > 1: nseg = 1, seg_len = 64).  z.row(r) / z.col(c) give the z-values of the lane's pair
-// (0 <= r, c <= m-1; they are only called with clamped indices).  Returns, on every lane, the
+// QP > 1: nseg = 1, seg_len = 64).  z.row(r) / z.col(c) give the z-values of the lane's pair.
+// Without Z::kPadded they are only called with indices clamped into [0, m-1]; with it, with
+// indices in [-(w+3)/2 - 1, m + (w+3)/2 + 1], where they must return finite values (cells
+// outside the matrix then take arbitrary finite costs: those before it stay +inf, as all their
+// predecessors are, and those after it feed no cell inside it).  Returns, on every lane, the
 // squared cost D(m-1, m-1) of the lane's pair, or +inf when the pair was abandoned against ub
 // (then t_stop is the last anti-diagonal evaluated, else 2m-2).
 //
@@ -162,14 +166,24 @@ __device__ __forceinline__ T dtw_band(int m, int w, int seg_len, int nseg, T ub,
     }
     const int mm1 = m - 1;
     const int t_end = 2 * m - 2;
-    auto clampi = [mm1](int v) { return min(max(v, 0), mm1); };
+    auto clampi = [mm1](int v) {
+        if constexpr (Z::kPadded)
+            return v;
+        else
+            return min(max(v, 0), mm1);
+    };
+    // offsets used for the z-value indices: lanes past the band (their cells are forced to
+    // +inf) read at offset w + 1, so that the indices stay within the padded range
+    int e_ix[QP];
+#pragma unroll
+    for (int k = 0; k < QP; ++k) e_ix[k] = min(e_off[k], w + 1);
     auto odd_step = [&](int r[QP], T zr[QP]) __attribute__((always_inline)) {
         // odd offsets: up = next even (t-1), left = own even (t-1), diag = own odd (t-2)
         const T right_in = from_right<T>(INF, E[0]);
 #pragma unroll
         for (int k = 0; k < QP; ++k) {
             const T up = k + 1 < QP ? E[k + 1] : right_in;
-            const T d = zr[k] - z.col(clampi(r[k] + e_off[k] + 1));
+            const T d = zr[k] - z.col(clampi(r[k] + e_ix[k] + 1));
             const T v = fma(d, d, min3(O[k], up, E[k]));
             O[k] = inO[k] ? v : INF;
         }
@@ -183,7 +197,7 @@ __device__ __forceinline__ T dtw_band(int m, int w, int seg_len, int nseg, T ub,
         // t = 0 has odd parity: only the odd offsets, on row (0 - e - 1) / 2
 #pragma unroll
         for (int k = 0; k < QP; ++k) {
-            r[k] = (0 - e_off[k] - 1) >> 1;
+            r[k] = (0 - e_ix[k] - 1) >> 1;
             zr[k] = z.row(clampi(r[k]));
         }
         odd_step(r, zr);
@@ -195,17 +209,17 @@ __device__ __forceinline__ T dtw_band(int m, int w, int seg_len, int nseg, T ub,
     T zce[QP], zco[QP];
 #pragma unroll
     for (int k = 0; k < QP; ++k) {
-        r[k] = (t - e_off[k]) >> 1;
+        r[k] = (t - e_ix[k]) >> 1;
         zr[k] = z.row(clampi(r[k]));
-        zce[k] = z.col(clampi(r[k] + e_off[k]));
-        zco[k] = z.col(clampi(r[k] + e_off[k] + 1));
+        zce[k] = z.col(clampi(r[k] + e_ix[k]));
+        zco[k] = z.col(clampi(r[k] + e_ix[k] + 1));
     }
     for (; t <= t_end; t += 2) {
         T zr_n[QP], zco_n[QP];
 #pragma unroll
         for (int k = 0; k < QP; ++k) {
             zr_n[k] = z.row(clampi(r[k] + 1));
-            zco_n[k] = z.col(clampi(r[k] + e_off[k] + 2));
+            zco_n[k] = z.col(clampi(r[k] + e_ix[k] + 2));
         }
         // even offsets at t: up = own odd (t-1), left = previous odd (t-1), diag = own even (t-2)
         const T left_in = from_left<T>(INF, O[QP - 1]);

@@ -149,12 +149,15 @@ __device__ inline float range_max(const JoinArgs &A, int which, int64_t lo, int6
     return r;
 }
 
-// z-values of a DTW pair normalised on the fly from the staged chunk (x - mean) / sigma.
+// z-values of a DTW pair normalised on the fly from the staged chunk, x / sigma - mean / sigma
+// in one FMA (nmi = -mean / sigma).  Indices may leave the window by up to (w + 3) / 2 + 1 on
+// either side: the cascade layout keeps finite staged values there (see CascadeLDS).
 struct StagedZ {
+    static constexpr bool kPadded = true;
     const float *xi, *xj;
-    float mi, ii, mj, ij;
-    __device__ float row(int r) const { return (xi[r] - mi) * ii; }
-    __device__ float col(int c) const { return (xj[c] - mj) * ij; }
+    float ii, nmi, ij, nmj;
+    __device__ float row(int r) const { return fmaf(xi[r], ii, nmi); }
+    __device__ float col(int c) const { return fmaf(xj[c], ij, nmj); }
 };
 
 // Suffix bound of the DTW of each segment's pair (see dtw_device.hpp): block kappa holds the
@@ -179,9 +182,12 @@ struct LaneSuffix {
     }
 };
 
-// LDS layout of the cascade's staging (valid while draining) for window length m: windows of
-// the part's RC rows (CI samples: series, upper and lower envelope) and of its RC + DG columns
-// (CJ samples), then mean and 1/sigma of the rows and of the columns.  Built where it is used
+// LDS layout of the cascade's staging (valid while draining) for window length m: mean and
+// 1/sigma of the part's RC rows and of its RC + DG columns, then the windows of the rows (CI
+// samples: series, upper and lower envelope) and of the columns (CJ samples).  The DTW reads
+// up to (w + 3) / 2 + 1 <= 2 RC + 2 (RC + DG) values before and after a window without
+// clamping: before the row series lie the statistics, before the column series the row lower
+// envelope, after each series its upper envelope (all finite, w < m).  Built where it is used
 // from an opaque copy of m, so that the ten addresses are not kept live (in scalar registers
 // that would spill) through the sweep.
 struct CascadeLDS {
@@ -189,18 +195,20 @@ struct CascadeLDS {
     __device__ CascadeLDS(float *base, int m) {
         asm volatile("" : "+s"(m));
         const int CI = RC + m, CJ = RC + DG + m;
-        sai = base;
+        rmu = base;
+        ris = rmu + RC;
+        cmu = ris + RC;
+        cis = cmu + (RC + DG);
+        sai = cis + (RC + DG);
         sui = sai + CI;
         sli = sui + CI;
         saj = sli + CI;
         suj = saj + CJ;
         slj = suj + CJ;
-        rmu = slj + CJ;
-        ris = rmu + RC;
-        cmu = ris + RC;
-        cis = cmu + (RC + DG);
     }
 };
+static_assert((511 + 3) / 2 + 1 <= 2 * RC + 2 * (RC + DG),
+              "DTW reads before a row window stay in the statistics for the widest band, w = 511");
 
 // Packed (row, column) offsets of a pair inside the staged chunk (both < 2^16).
 __device__ inline uint32_t pack_pair(int li, int lj) { return (uint32_t)li | ((uint32_t)lj << 16); }
@@ -287,17 +295,17 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 key = load_bsf(A);
             }
             int li = 0, lj = 0;
-            float mi = 0.f, ii = 0.f, mj = 0.f, ij = 0.f;
+            float ii = 0.f, nmi = 0.f, ij = 0.f, nmj = 0.f;
             if (seg < nseg) {
                 const uint32_t pr = q3[e0 + seg];
                 li = pair_row(pr);
                 lj = pair_col(pr);
-                mi = L.rmu[li];
                 ii = L.ris[li];
-                mj = L.cmu[lj];
+                nmi = -L.rmu[li] * ii;
                 ij = L.cis[lj];
+                nmj = -L.cmu[lj] * ij;
             }
-            const StagedZ z{L.sai + li, L.saj + lj, mi, ii, mj, ij};
+            const StagedZ z{L.sai + li, L.saj + lj, ii, nmi, ij, nmj};
             int t_stop = 0;
             float res;
             // block sums of the LB_Keogh terms of the lane's blocks and their reverse prefix sums
@@ -322,9 +330,10 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                             for (int rr = 0; rr < 8; ++rr) {
                                 const int r = r0 + ((rr + x) & 7);
                                 if (r < m) {
+                                    // envelope z-values by the same map as the series'
                                     const float zi = z.row(r), zj = z.col(r);
-                                    const float uj = (L.suj[lj + r] - mj) * ij, lwj = (L.slj[lj + r] - mj) * ij;
-                                    const float ui = (L.sui[li + r] - mi) * ii, lwi = (L.sli[li + r] - mi) * ii;
+                                    const float uj = fmaf(L.suj[lj + r], ij, nmj), lwj = fmaf(L.slj[lj + r], ij, nmj);
+                                    const float ui = fmaf(L.sui[li + r], ii, nmi), lwi = fmaf(L.sli[li + r], ii, nmi);
                                     const float ta = fmaxf(fmaxf(zi - uj, lwj - zi), 0.f);
                                     const float tb = fmaxf(fmaxf(zj - ui, lwi - zj), 0.f);
                                     vA = fmaf(ta, ta, vA);
