@@ -214,7 +214,8 @@ __device__ __forceinline__ T dtw_band(int m, int w, int seg_len, int nseg, T ub,
         zce[k] = z.col(clampi(r[k] + e_ix[k]));
         zco[k] = z.col(clampi(r[k] + e_ix[k] + 1));
     }
-    for (; t <= t_end; t += 2) {
+    // One pair of steps: the even offsets at t and, when `odd`, the odd offsets at t + 1.
+    auto step_pair = [&](bool odd) __attribute__((always_inline)) {
         T zr_n[QP], zco_n[QP];
 #pragma unroll
         for (int k = 0; k < QP; ++k) {
@@ -230,7 +231,7 @@ __device__ __forceinline__ T dtw_band(int m, int w, int seg_len, int nseg, T ub,
             const T v = fma(d, d, min3(E[k], O[k], left));
             E[k] = inE[k] ? v : INF;
         }
-        if (t + 1 <= t_end) {
+        if (odd) {
             // odd offsets: up = next even (t), left = own even (t), diag = own odd (t-1)
             const T right_in = from_right<T>(INF, E[0]);
 #pragma unroll
@@ -248,26 +249,42 @@ __device__ __forceinline__ T dtw_band(int m, int w, int seg_len, int nseg, T ub,
             zce[k] = zco[k];
             zco[k] = zco_n[k];
         }
-        if ((t & 15) >= 14 && t + 1 < t_end) {
-            // The segment is over ub when its minimum over both live anti-diagonals is; both
-            // tests are monotone in the value, so that holds exactly when every lane of the
-            // segment is over on its own cells: one ballot instead of a cross-lane minimum.
-            T v = INF;
+    };
+    // Abandon test after the pair of steps at tc (tc = 14 or 15 mod 16); true when no segment
+    // of the wave is left.  The segment is over ub when its minimum over both live
+    // anti-diagonals is; both tests are monotone in the value, so that holds exactly when every
+    // lane of the segment is over on its own cells: one ballot instead of a cross-lane minimum.
+    auto check = [&](int tc) __attribute__((always_inline)) {
+        T v = INF;
 #pragma unroll
-            for (int k = 0; k < QP; ++k) v = min3(v, E[k], O[k]);
-            bool over = v > ub;
-            if constexpr (SUF::enabled) {
-                const T sb = (T)suf.at(t);  // every lane takes part (the functor may shuffle)
-                over = over || (v + sb) * (T)(1.0 - 0x1p-9) > ub;
-            }
-            const unsigned long long keep = __builtin_amdgcn_ballot_w64(!over);
-            if (alive && (keep & segmask) == 0ull) {
-                alive = false;
-                t_stop = t + 1;
-            }
-            if (__builtin_amdgcn_ballot_w64(alive) == 0ull) break;
+        for (int k = 0; k < QP; ++k) v = min3(v, E[k], O[k]);
+        bool over = v > ub;
+        if constexpr (SUF::enabled) {
+            const T sb = (T)suf.at(tc);  // every lane takes part (the functor may shuffle)
+            over = over || (v + sb) * (T)(1.0 - 0x1p-9) > ub;
+        }
+        const unsigned long long keep = __builtin_amdgcn_ballot_w64(!over);
+        if (alive && (keep & segmask) == 0ull) {
+            alive = false;
+            t_stop = tc + 1;
+        }
+        return __builtin_amdgcn_ballot_w64(alive) == 0ull;
+    };
+    // Blocks of 8 pairs of steps (t = t0 + 16 k .. + 15, t0 = t mod 2 here) end in a test, as
+    // long as the block's last odd step t + 15 comes before the last anti-diagonal; in the
+    // remaining tail (t + 15 >= t_end) no test falls, since a test needs tc + 1 < t_end.
+    bool done = false;
+    while (t + 15 < t_end) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) step_pair(true);
+        t += 16;
+        if (check(t - 2)) {
+            done = true;
+            break;
         }
     }
+    if (!done)
+        for (; t <= t_end; t += 2) step_pair(t + 1 <= t_end);
     // result: offset 0 of the lane's segment
     const int src_lane = valid ? seg * seg_len + idx0 / QP : lane;
     const int k0 = idx0 % QP;

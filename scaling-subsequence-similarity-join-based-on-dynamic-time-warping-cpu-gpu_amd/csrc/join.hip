@@ -1073,16 +1073,28 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                         cs[r][2] = v4.z;
                         cs[r][3] = v4.w;
                     }
-                    for (; rl < nrows; rl += 8) {
-                        if (chunk_full && rl + 8 <= nrows)
+                    // whole 8-row groups of a chunk inside the triangle take the unchecked form;
+                    // a queue close to full ends the sweep (the chunk resumes after the drain)
+                    bool stop = false;
+                    if (chunk_full) {
+                        const int nfull = nrows & ~7;
+                        for (; rl < nfull; rl += 8) {
                             group(T_(), two_tag);
-                        else
-                            group(F_(), two_tag);
-                        if (q1n > QLIM) {
-                            rl += 8;
-                            break;
+                            if (q1n > QLIM) {
+                                rl += 8;
+                                stop = true;
+                                break;
+                            }
                         }
                     }
+                    if (!stop)
+                        for (; rl < nrows; rl += 8) {
+                            group(F_(), two_tag);
+                            if (q1n > QLIM) {
+                                rl += 8;
+                                break;
+                            }
+                        }
                 };
                 if (two_sided)
                     sweep(T_());
