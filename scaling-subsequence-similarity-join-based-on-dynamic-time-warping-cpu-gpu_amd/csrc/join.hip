@@ -391,6 +391,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         __syncthreads();  // q2 entries were written by other lanes
         // th: the drain batch's thresholds (at most one batch old)
         const float sm = sqrtf((float)m);
+        float T2 = th.T2, sqmT2 = sqrtf(m * T2);  // updated after each DTW run
         int taken = 0;     // queue entries handed out (uniform)
         bool act = false;  // this lane holds a pair
         int li = 0, lj = 0, k = 0;
@@ -426,10 +427,11 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 bB = mci - mcj * rB;
                 const float eA = 0x1p-22f * ((A.amax * (1.f + rA) + fabsf(bA)) * isj + sm);
                 const float eB = 0x1p-22f * ((A.amax * (1.f + rB) + fabsf(bB)) * isi + sm);
-                const float T2 = th.T2;
-                const float sj2 = 1.f / (isj * isj), si2 = 1.f / (isi * isi);
-                thrA = (T2 + 2.f * eA * sqrtf(m * T2) + m * eA * eA + m * 0x1p-22f * T2) * sj2 * (1.f + 0x1p-20f);
-                thrB = (T2 + 2.f * eB * sqrtf(m * T2) + m * eB * eB + m * 0x1p-22f * T2) * si2 * (1.f + 0x1p-20f);
+                // sigma^2 from the hardware reciprocal (1 ulp): the 2^-20 inflation covers it
+                const float rsj = __builtin_amdgcn_rcpf(isj), rsi = __builtin_amdgcn_rcpf(isi);
+                const float sj2 = rsj * rsj, si2 = rsi * rsi;
+                thrA = (T2 + 2.f * eA * sqmT2 + m * eA * eA + m * 0x1p-22f * T2) * sj2 * (1.f + 0x1p-20f);
+                thrB = (T2 + 2.f * eB * sqmT2 + m * eB * eB + m * 0x1p-22f * T2) * si2 * (1.f + 0x1p-20f);
                 accA = accB = 0.f;
                 k = 0;
                 kb = kc = 0;
@@ -503,6 +505,8 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         if (q3n > 0) {
             __syncthreads();
             process_q3();
+            T2 = th.T2;
+            sqmT2 = sqrtf(m * T2);
         }
         }
         q2n = 0;
@@ -637,10 +641,12 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                     const float be0 = aj[m - 1], be1 = aj[m - 2], be2 = aj[m - 3];
                     __builtin_amdgcn_sched_barrier(0);
                     // x: window i (columns), y: window j (rows)
-                    const float x0 = (a0 - mi) * ii, x1 = (a1 - mi) * ii, x2 = (a2 - mi) * ii;
-                    const float y0 = (b0 - mj) * ij, y1 = (b1 - mj) * ij, y2 = (b2 - mj) * ij;
-                    const float xe0 = (ae0 - mi) * ii, xe1 = (ae1 - mi) * ii, xe2 = (ae2 - mi) * ii;
-                    const float ye0 = (be0 - mj) * ij, ye1 = (be1 - mj) * ij, ye2 = (be2 - mj) * ij;
+                    // z-values as one FMA each, x / sigma - mean / sigma (within eps_z)
+                    const float nmi = -mi * ii, nmj = -mj * ij;
+                    const float x0 = fmaf(a0, ii, nmi), x1 = fmaf(a1, ii, nmi), x2 = fmaf(a2, ii, nmi);
+                    const float y0 = fmaf(b0, ij, nmj), y1 = fmaf(b1, ij, nmj), y2 = fmaf(b2, ij, nmj);
+                    const float xe0 = fmaf(ae0, ii, nmi), xe1 = fmaf(ae1, ii, nmi), xe2 = fmaf(ae2, ii, nmi);
+                    const float ye0 = fmaf(be0, ij, nmj), ye1 = fmaf(be1, ij, nmj), ye2 = fmaf(be2, ij, nmj);
                     // Any warping path from (0,0) first leaves {0,1}^2 at (2,0),(2,1),(2,2),(1,2) or
                     // (0,2), entered from an adjacent cell of {0,1}^2.  Same at the far corner.
                     const float c00 = sq(y0 - x0), c10 = sq(y1 - x0), c01 = sq(y0 - x1), c11 = sq(y1 - x1);
