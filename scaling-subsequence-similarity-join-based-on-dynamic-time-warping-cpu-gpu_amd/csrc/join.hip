@@ -461,14 +461,30 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                     // block base pointers, so that the 16 reads per array use immediate offsets
                     const float *xaj = L.saj + lj + base, *xui = L.sui + li + base, *xli = L.sli + li + base;
                     const float *xai = L.sai + li + base, *xuj = L.suj + lj + base, *xlj = L.slj + lj + base;
+                    // the reads of HB positions go out together (one wait), then their arithmetic
+                    constexpr int HB = 8;
 #pragma unroll
-                    for (int t = 0; t < 16; ++t) {
-                        const float cx = xaj[S * t] - bA;
-                        const float ta = fmaxf(fmaxf(fmaf(-xui[S * t], rA, cx), fmaf(xli[S * t], rA, -cx)), 0.f);
-                        accA = fmaf(ta, ta, accA);
-                        const float cy = xai[S * t] - bB;
-                        const float tb = fmaxf(fmaxf(fmaf(-xuj[S * t], rB, cy), fmaf(xlj[S * t], rB, -cy)), 0.f);
-                        accB = fmaf(tb, tb, accB);
+                    for (int h = 0; h < 16; h += HB) {
+                        float va[HB], vu[HB], vl[HB], wa[HB], wu[HB], wl[HB];
+#pragma unroll
+                        for (int t = 0; t < HB; ++t) {
+                            va[t] = xaj[S * (h + t)];
+                            vu[t] = xui[S * (h + t)];
+                            vl[t] = xli[S * (h + t)];
+                            wa[t] = xai[S * (h + t)];
+                            wu[t] = xuj[S * (h + t)];
+                            wl[t] = xlj[S * (h + t)];
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int t = 0; t < HB; ++t) {
+                            const float cx = va[t] - bA;
+                            const float ta = fmaxf(fmaxf(fmaf(-vu[t], rA, cx), fmaf(vl[t], rA, -cx)), 0.f);
+                            accA = fmaf(ta, ta, accA);
+                            const float cy = wa[t] - bB;
+                            const float tb = fmaxf(fmaxf(fmaf(-wu[t], rB, cy), fmaf(wl[t], rB, -cy)), 0.f);
+                            accB = fmaf(tb, tb, accB);
+                        }
                     }
                     if (++kc == nsup) {
                         kc = 0;
@@ -476,7 +492,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                     }
                 };
                 if (k < nstr) {
-                    // fixed blocks: the 96 LDS reads are independent and issue back to back
+                    // fixed blocks: 96 independent LDS reads, issued in two groups of 48
                     strided();
                     k += 16;
                     c_kelem += 16ull;
@@ -1157,35 +1173,68 @@ __global__ __launch_bounds__(64) void k_seed(SeedArgs S, int2 *pairs) {
     int64_t i_end = i_beg + S.rows;
     if (i_end > rows_g) i_end = rows_g;
     const int64_t dq0 = d0 + KD * lane;
+    // FP64 anchor of the lane's four diagonals at row i_beg: the column values slide through
+    // registers (one new load per k, reads past the series end only feed diagonals outside
+    // the triangle, which are discarded); loads run 8 positions ahead of their use.
     float C[KD];
     {
+        const int64_t n = S.n;
+        const global_ptr<double> pa = (global_ptr<double>)S.a;
+        const int64_t jb = i_beg + dq0;
+        auto col = [&](int64_t p) { return pa[p < n - 1 ? p : n - 1]; };
         double c[KD] = {0.0, 0.0, 0.0, 0.0};
         const double m0 = S.mu[i_beg];
-        for (int k = 0; k < m; ++k) {
-            const double cc = S.a[i_beg + k] - m0;
+        double win[KD];
 #pragma unroll
-            for (int q = 0; q < KD; ++q)
-                if (i_beg + dq0 + q <= nw - 1) c[q] = fma(cc, S.a[i_beg + dq0 + q + k], c[q]);
+        for (int q = 0; q < KD - 1; ++q) win[q + 1] = col(jb + q);
+#pragma unroll 8
+        for (int k = 0; k < m; ++k) {
+            const double cc = pa[i_beg + k] - m0;
+#pragma unroll
+            for (int q = 0; q < KD - 1; ++q) win[q] = win[q + 1];
+            win[KD - 1] = col(jb + k + KD - 1);
+#pragma unroll
+            for (int q = 0; q < KD; ++q) c[q] = fma(cc, win[q], c[q]);
         }
 #pragma unroll
-        for (int q = 0; q < KD; ++q) C[q] = (float)c[q];
+        for (int q = 0; q < KD; ++q) C[q] = i_beg + dq0 + q <= nw - 1 ? (float)c[q] : 0.f;
     }
+    // rows in batches of 4 whose window records (the 16-byte halves holding dfa, dga, 1/sigma)
+    // are all loaded before the first use; indices clamped into the table, out-of-range pairs
+    // skipped exactly as in a row-by-row scan
+    const global_ptr<vf4> win4 = (global_ptr<vf4>)S.win;
+    static_assert(sizeof(WinRow) == 32 && offsetof(WinRow, isig) == 8, "WinRow is read as two 16-byte vectors");
     float best = __builtin_huge_valf();
     int bi = -1, bj = -1;
-    for (int64_t i = i_beg; i < i_end; ++i) {
-        const WinRow ri = S.win[i];
+    for (int64_t i0 = i_beg; i0 < i_end; i0 += 4) {
+        vf4 ri[4], rj[4][KD];
 #pragma unroll
-        for (int q = 0; q < KD; ++q) {
-            const int64_t j = i + dq0 + q;
-            if (j <= nw - 1) {
-                const WinRow rj = S.win[j];
-                const float ed2 = 2.0f * m - 2.0f * C[q] * ri.isig * rj.isig;
-                if (ed2 < best) {
-                    best = ed2;
-                    bi = (int)i;
-                    bj = (int)j;
+        for (int u = 0; u < 4; ++u) {
+            const int64_t i = i0 + u < nw - 1 ? i0 + u : nw - 1;
+            ri[u] = win4[2 * i];
+#pragma unroll
+            for (int q = 0; q < KD; ++q) {
+                const int64_t j = i0 + u + dq0 + q, jc = j < nw - 1 ? j : nw - 1;
+                rj[u][q] = win4[2 * jc];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t i = i0 + u;
+            if (i >= i_end) break;
+#pragma unroll
+            for (int q = 0; q < KD; ++q) {
+                const int64_t j = i + dq0 + q;
+                if (j <= nw - 1) {
+                    // (dfa, dga, isig) = .x, .y, .z
+                    const float ed2 = 2.0f * m - 2.0f * C[q] * ri[u].z * rj[u][q].z;
+                    if (ed2 < best) {
+                        best = ed2;
+                        bi = (int)i;
+                        bj = (int)j;
+                    }
+                    C[q] = C[q] + rj[u][q].x * ri[u].y + ri[u].x * rj[u][q].y;
                 }
-                C[q] = C[q] + rj.dfa * ri.dga + ri.dfa * rj.dga;
             }
         }
     }
