@@ -243,11 +243,11 @@ __global__ __launch_bounds__(256) void k_window_stats(const double *__restrict__
                                                       int m, double *mu, double *sig, double *muv, float *mu_c,
                                                       WinRow *win, float *blkmax, int nblk, float *gstat,
                                                       int32_t *flags) {
-    __shared__ float r0[256], r1[256], r2[256];
+    __shared__ float r0[256], r1[256], r2[256], r3[256], r4[256];
     const double c0 = *c0p;
     const int64_t nw = n - m + 1;
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    float fs = 0.f, fsv = 0.f, fis = 0.f;
+    float fs = 0.f, fsv = 0.f, fis = 0.f, fmc = 0.f, fsmin = __builtin_huge_valf();
     if (i < nw) {
         WinStats s = window_stats(P, n, m, c0, i);
         if (s.degenerate) atomicOr(&flags[1], 1);
@@ -276,28 +276,38 @@ __global__ __launch_bounds__(256) void k_window_stats(const double *__restrict__
         fs = (float)s.sig;
         fsv = (float)s.sv;
         fis = (float)is;
-        atomic_max_pos(&gstat[1], fabsf(mc));
-        atomic_max_pos(&gstat[2], fis);
-        atomic_max_pos(&gstat[3], fs);
-        atomic_min_pos(&gstat[4], fs);
+        fmc = fabsf(mc);
+        fsmin = fs;
     }
+    // block maxima (and the minimum of sigma), then one set of atomics per block for the
+    // series-wide statistics
     r0[threadIdx.x] = fs;
     r1[threadIdx.x] = fsv;
     r2[threadIdx.x] = fis;
+    r3[threadIdx.x] = fmc;
+    r4[threadIdx.x] = fsmin;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
             r0[threadIdx.x] = fmaxf(r0[threadIdx.x], r0[threadIdx.x + s]);
             r1[threadIdx.x] = fmaxf(r1[threadIdx.x], r1[threadIdx.x + s]);
             r2[threadIdx.x] = fmaxf(r2[threadIdx.x], r2[threadIdx.x + s]);
+            r3[threadIdx.x] = fmaxf(r3[threadIdx.x], r3[threadIdx.x + s]);
+            r4[threadIdx.x] = fminf(r4[threadIdx.x], r4[threadIdx.x + s]);
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0 && (int)blockIdx.x < nblk) {
-        // round maxima up by one part in 2^20 so the FP32 conversion never underestimates
-        blkmax[0 * nblk + blockIdx.x] = r0[0] * (1.0f + 0x1p-20f);
-        blkmax[1 * nblk + blockIdx.x] = r1[0] * (1.0f + 0x1p-20f);
-        blkmax[2 * nblk + blockIdx.x] = r2[0] * (1.0f + 0x1p-20f);
+    if (threadIdx.x == 0) {
+        if ((int)blockIdx.x < nblk) {
+            // round maxima up by one part in 2^20 so the FP32 conversion never underestimates
+            blkmax[0 * nblk + blockIdx.x] = r0[0] * (1.0f + 0x1p-20f);
+            blkmax[1 * nblk + blockIdx.x] = r1[0] * (1.0f + 0x1p-20f);
+            blkmax[2 * nblk + blockIdx.x] = r2[0] * (1.0f + 0x1p-20f);
+        }
+        atomic_max_pos(&gstat[1], r3[0]);
+        atomic_max_pos(&gstat[2], r2[0]);
+        atomic_max_pos(&gstat[3], r0[0]);
+        atomic_min_pos(&gstat[4], r4[0]);  // every block holds at least one window
     }
 }
 
