@@ -476,7 +476,9 @@ struct Proposal {
     float *d_norm = nullptr; // feature mean, scale
     int64_t *d_vi = nullptr, *d_vj = nullptr;
     double *d_vcost = nullptr;
-    std::vector<int2> cands;  // de-duplicated candidate pairs (host)
+    uint64_t *d_k0 = nullptr, *d_k1 = nullptr;  // candidate keys before / after the sort
+    void *d_sort_tmp = nullptr;
+    std::vector<int2> cands;  // candidate pairs in (i, j) order (host)
 };
 
 // Statistics, candidate scan and raw features of one series (device-resident series d_a).
@@ -488,6 +490,8 @@ int propose(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w,
     const size_t rows16 = (ncand + 15) / 16 * 16;
     const int np = mlp_param_count(), pad = mlp_feature_pad();
     const int nv = std::max(n_verify, 1) * 2;
+    const size_t sort_bytes = pair_sort_tmp_bytes((int64_t)ncand);
+    if (ncand > 0 && sort_bytes == 0) return fail(DTWJ_EDEVICE, "radix sort set-up failed");
     auto carve = [&](Arena &ar) {
         take_job(ar, J);
         Pr.d_cand = ar.take<int2>(ncand + 1);
@@ -501,6 +505,9 @@ int propose(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w,
         Pr.d_vi = ar.take<int64_t>(nv);
         Pr.d_vj = ar.take<int64_t>(nv);
         Pr.d_vcost = ar.take<double>(nv);
+        Pr.d_k0 = ar.take<uint64_t>(ncand + 1);
+        Pr.d_k1 = ar.take<uint64_t>(ncand + 1);
+        Pr.d_sort_tmp = ar.take<char>(sort_bytes + 1);
     };
     Arena dry;
     carve(dry);
@@ -531,17 +538,16 @@ int propose(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w,
     S.rows = std::min(J.rows_per_item, 512);
     S.n_seed = J.n_seed;
     if (int rc = launch_seed_lanes(S, Pr.d_cand, s)) return rc;
-    std::vector<int2> raw(ncand);
-    DTWJ_HIP(hipMemcpyAsync(raw.data(), Pr.d_cand, sizeof(int2) * ncand, hipMemcpyDeviceToHost, s));
+    // the lanes' pairs in (i, j) order, sorted on the device; duplicates are dropped (none
+    // arise: every (item, lane) of the scan owns its own rows and diagonals)
+    if (int rc = launch_pair_sort(Pr.d_cand, (int64_t)ncand, Pr.d_k0, Pr.d_k1, Pr.d_sort_tmp, sort_bytes, s)) return rc;
+    std::vector<uint64_t> keys(ncand);
+    DTWJ_HIP(hipMemcpyAsync(keys.data(), Pr.d_k1, sizeof(uint64_t) * ncand, hipMemcpyDeviceToHost, s));
     DTWJ_HIP(hipStreamSynchronize(s));
     Pr.cands.clear();
-    for (const int2 &p : raw)
-        if (p.x >= 0) Pr.cands.push_back(p);
-    std::sort(Pr.cands.begin(), Pr.cands.end(),
-              [](const int2 &a, const int2 &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
-    Pr.cands.erase(std::unique(Pr.cands.begin(), Pr.cands.end(),
-                               [](const int2 &a, const int2 &b) { return a.x == b.x && a.y == b.y; }),
-                   Pr.cands.end());
+    Pr.cands.reserve(ncand);
+    for (size_t k = 0; k < ncand && keys[k] != ~0ull; ++k)
+        if (k == 0 || keys[k] != keys[k - 1]) Pr.cands.push_back(make_int2((int)(keys[k] >> 32), (int)(uint32_t)keys[k]));
     const int cnt = (int)Pr.cands.size();
     if (cnt > 0) {
         DTWJ_HIP(hipMemcpyAsync(Pr.d_cand, Pr.cands.data(), sizeof(int2) * cnt, hipMemcpyHostToDevice, s));

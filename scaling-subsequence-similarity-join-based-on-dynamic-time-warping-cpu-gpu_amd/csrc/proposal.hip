@@ -10,6 +10,8 @@
 // Forward: M = 16 samples, K = 48 padded features (3 steps), N = 64 hidden (4 tiles).
 // Weight gradient dW1 = X^T dH: M = 48 features (3 tiles), K = 16 samples, N = 64 hidden; the
 // hidden gradient dH comes out of the forward in the D layout, which is exactly the B layout.
+#include <hipcub/hipcub.hpp>
+
 #include "common.hpp"
 #include "../../include/dtwj.h"
 #include "internal.hpp"
@@ -336,6 +338,33 @@ void params_to_mlp(const float *P, dtwj_mlp &M) {
         M.w2[h] = P[P_W2 + h];
     }
     M.b2 = P[P_B2];
+}
+
+// Candidate pairs as 64-bit keys (i << 32 | j, both non-negative and below 2^31), entries
+// without a pair as the largest key, so that an ascending sort orders pairs by (i, j) and puts
+// the empty entries last.
+__global__ __launch_bounds__(256) void k_pair_keys(const int2 *__restrict__ cand, int64_t count, uint64_t *key) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= count) return;
+    const int2 p = cand[t];
+    key[t] = p.x >= 0 ? ((uint64_t)(uint32_t)p.x << 32) | (uint32_t)p.y : ~0ull;
+}
+
+size_t pair_sort_tmp_bytes(int64_t count) {
+    size_t bytes = 0;
+    if (hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, (const uint64_t *)nullptr, (uint64_t *)nullptr, (int)count, 0,
+                                          64) != hipSuccess)
+        return 0;
+    return bytes;
+}
+
+int launch_pair_sort(const int2 *cand, int64_t count, uint64_t *key_in, uint64_t *key_out, void *tmp, size_t tmp_bytes,
+                     hipStream_t s) {
+    if (count <= 0) return 0;
+    hipLaunchKernelGGL(k_pair_keys, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, cand, count, key_in);
+    DTWJ_HIP(hipGetLastError());
+    DTWJ_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, (const uint64_t *)key_in, key_out, (int)count, 0, 64, s));
+    return 0;
 }
 
 int launch_prop_features(const JoinArgs &A, const int2 *cand, int count, float *F, hipStream_t s) {
