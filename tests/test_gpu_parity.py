@@ -83,6 +83,21 @@ def test_top1_matches_brute_force(gpu, oracle, n, m, w, seed):
     assert abs(r.dist - bd) <= 1e-9 * bd
 
 
+@pytest.mark.parametrize("n,m,w,seed", [(3000, 300, 150, 11), (2500, 200, 180, 11), (2000, 300, 260, 12),
+                                        (1600, 300, 280, 13)])
+def test_top1_wide_bands_match_oracle(gpu, oracle, n, m, w, seed):
+    """Bands of 2w + 1 > 256 offsets: the join's DTW runs with 4 (w = 150, 180) or 8 (w = 260,
+    280) offset pairs per lane, in every sweep mode and with small tiles (several items and
+    drains per wave)."""
+    a = rw(n, seed)
+    o = oracle.motif_discovery(a, m, w)
+    for opts in ({}, {"sweep": 1, "rows_per_item": 512, "waves_per_cu": 1}, {"sweep": 2}):
+        r = gpu.top1(a, m, w, **opts)
+        assert (r.i, r.j) == (o.i, o.j), (opts, r.i, r.j, o.i, o.j)
+        assert abs(r.dist - o.bsf) <= 1e-9 * o.bsf
+        assert r.pairs_swept == r.pairs_total
+
+
 def test_single_pair_and_extreme_bands(gpu, oracle):
     a = rw(256, 21)
     r = gpu.top1(a, 128, 5)                     # n = 2m: exactly one admissible pair
