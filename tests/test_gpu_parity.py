@@ -3,7 +3,7 @@ CPU oracle or the reference's golden vectors.
 
 Tolerances: integer outputs (indices) bit-exact; FP64 stages (statistics, verified distance)
 relative 1e-9; FP32 DTW cells relative 1e-4 of the cost (the FP32-vs-FP64 error is reported);
-north-star bar for the final distance: relative 1e-5.
+required bar for the final distance: relative 1e-5.
 """
 import os
 import subprocess
