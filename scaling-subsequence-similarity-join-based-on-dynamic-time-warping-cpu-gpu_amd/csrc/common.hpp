@@ -163,3 +163,17 @@ struct SeedArgs {
     int32_t rows;           // rows scanned per seed item
     int32_t n_seed;         // number of seed items (spread over the item space)
 };
+
+// One series of a batched seed scan: its scan arguments and seed-pair buffer; the blocks of all
+// series are numbered consecutively (base[t] .. base[t+1] - 1 belong to series t).
+struct SeedJob {
+    SeedArgs S;
+    int2 *pairs;
+};
+
+// One series of a batched seed-pair DTW launch: its join descriptor index and seed pairs.
+struct SeedDtwJob {
+    const int2 *pairs;
+    int32_t series;
+    int32_t npairs;
+};

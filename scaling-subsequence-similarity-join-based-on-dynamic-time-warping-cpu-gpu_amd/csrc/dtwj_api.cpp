@@ -328,23 +328,27 @@ int upload_job(Job &J, hipStream_t s) {
 }
 
 // Seed scan (ED over sampled items) + DTW of every seed pair.  d_args: the job's descriptor.
+// Seed-scan arguments of a job whose device buffers are placed.
+SeedArgs seed_args(const Job &J) {
+    SeedArgs S;
+    S.a = J.d_a;
+    S.mu = J.P.mu;
+    S.win = J.P.win;
+    S.n = J.n;
+    S.nw = J.nw;
+    S.m = J.m;
+    S.item_start = J.d_item_start;
+    S.groups = J.groups;
+    S.n_items = J.n_items;
+    S.rows_per_item = J.rows_per_item;
+    S.rows = std::min(J.rows_per_item, 512);
+    S.n_seed = J.n_seed;
+    return S;
+}
+
 int enqueue_seeds(Job &J, const JoinArgs *d_args, hipStream_t s) {
-    if (J.n_seed > 0) {
-        SeedArgs S;
-        S.a = J.d_a;
-        S.mu = J.P.mu;
-        S.win = J.P.win;
-        S.n = J.n;
-        S.nw = J.nw;
-        S.m = J.m;
-        S.item_start = J.d_item_start;
-        S.groups = J.groups;
-        S.n_items = J.n_items;
-        S.rows_per_item = J.rows_per_item;
-        S.rows = std::min(J.rows_per_item, 512);
-        S.n_seed = J.n_seed;
-        if (int rc = launch_seed(S, J.d_seed_pairs, s)) return rc;
-    }
+    if (J.n_seed > 0)
+        if (int rc = launch_seed(seed_args(J), J.d_seed_pairs, s)) return rc;
     return launch_seed_dtw(d_args, J.m, J.w, J.d_seed_pairs, J.n_seed_pairs, s);
 }
 
@@ -781,6 +785,9 @@ int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const d
     int32_t *d_work = nullptr;
     uint32_t *d_queue = nullptr;
     double *d_vcost = nullptr;
+    SeedJob *d_sjobs = nullptr;
+    SeedDtwJob *d_djobs = nullptr;
+    int32_t *d_sbase = nullptr, *d_dbase = nullptr;
     auto carve = [&](Arena &ar) {
         for (int32_t t = 0; t < nj; ++t) {
             d_data[t] = ar.take<double>(jobs[t].n);
@@ -794,6 +801,10 @@ int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const d
         d_vi = ar.take<int64_t>(kBatchCandCap);
         d_vj = ar.take<int64_t>(kBatchCandCap);
         d_vcost = ar.take<double>(kBatchCandCap);
+        d_sjobs = ar.take<SeedJob>(nj);
+        d_sbase = ar.take<int32_t>(nj + 1);
+        d_djobs = ar.take<SeedDtwJob>(nj);
+        d_dbase = ar.take<int32_t>(nj + 4);
     };
     Arena dry;
     carve(dry);
@@ -803,7 +814,56 @@ int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const d
     ar.cap = c->mem_cap;
     carve(ar);
 
+    // Seeds of all series in few launches: one seed scan over every series' blocks, and one
+    // seed-pair DTW launch per band class (offset pairs per lane), each block finding its
+    // series from prefix sums of the blocks.
+    std::vector<SeedJob> hsj;
+    std::vector<int32_t> hsb{0};
+    for (int32_t t = 0; t < nj; ++t) {
+        if (jobs[t].n_seed <= 0) continue;
+        SeedJob q;
+        q.S = seed_args(jobs[t]);
+        q.pairs = jobs[t].d_seed_pairs;
+        hsj.push_back(q);
+        hsb.push_back(hsb.back() + jobs[t].n_seed + 1);
+    }
+    struct DtwClass {
+        int32_t job0, cnt, base0, blocks, max_m, w;
+    };
+    std::vector<DtwClass> dcls;
+    std::vector<SeedDtwJob> hdj;
+    std::vector<int32_t> hdb;
+    auto qp_of = [](int32_t w) {  // offset pairs per lane of the DTW kernels: 128 QP >= 2w + 1
+        const int32_t band = 2 * w + 1;
+        return band <= 128 ? 1 : band <= 256 ? 2 : band <= 512 ? 4 : 8;
+    };
+    for (int qp : {1, 2, 4, 8}) {
+        DtwClass k{(int32_t)hdj.size(), 0, (int32_t)hdb.size(), 0, 0, 0};
+        hdb.push_back(0);
+        for (int32_t t = 0; t < nj; ++t) {
+            if (qp_of(jobs[t].w) != qp || jobs[t].n_seed_pairs <= 0) continue;
+            hdj.push_back(SeedDtwJob{jobs[t].d_seed_pairs, t, jobs[t].n_seed_pairs});
+            k.blocks += jobs[t].n_seed_pairs;
+            hdb.push_back(k.blocks);
+            k.max_m = std::max(k.max_m, jobs[t].m);
+            k.w = jobs[t].w;
+            ++k.cnt;
+        }
+        if (k.cnt > 0)
+            dcls.push_back(k);
+        else
+            hdb.pop_back();
+    }
+
     hipStream_t s = c->stream;
+    if (!hsj.empty()) {
+        DTWJ_HIP(hipMemcpyAsync(d_sjobs, hsj.data(), sizeof(SeedJob) * hsj.size(), hipMemcpyHostToDevice, s));
+        DTWJ_HIP(hipMemcpyAsync(d_sbase, hsb.data(), sizeof(int32_t) * hsb.size(), hipMemcpyHostToDevice, s));
+    }
+    if (!hdj.empty()) {
+        DTWJ_HIP(hipMemcpyAsync(d_djobs, hdj.data(), sizeof(SeedDtwJob) * hdj.size(), hipMemcpyHostToDevice, s));
+        DTWJ_HIP(hipMemcpyAsync(d_dbase, hdb.data(), sizeof(int32_t) * hdb.size(), hipMemcpyHostToDevice, s));
+    }
     std::vector<JoinArgs> hargs(nj);
     for (int32_t t = 0; t < nj; ++t) {
         const dtwj_series &S = series[owner[t]];
@@ -819,8 +879,11 @@ int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const d
         for (int32_t t = 0; t < nj; ++t)
             if (int rc = prep_series(jobs[t].P, s)) return rc;
         if (int rc = launch_join_setup(d_args, nj, d_work, s)) return rc;
-        for (int32_t t = 0; t < nj; ++t)
-            if (int rc = enqueue_seeds(jobs[t], d_args + t, s)) return rc;
+        if (int rc = launch_seed_batch(d_sjobs, d_sbase, (int32_t)hsj.size(), hsb.back(), s)) return rc;
+        for (const DtwClass &k : dcls)
+            if (int rc = launch_seed_dtw_batch(d_args, k.max_m, k.w, d_djobs + k.job0, d_dbase + k.base0, k.cnt,
+                                               k.blocks, s))
+                return rc;
         return launch_join(d_args, d_item_base, nj, max_m, max_w, d_work, d_queue, grid, s);
     };
     const bool use_graph = !(opt && opt->graph < 0);

@@ -42,6 +42,12 @@ int launch_seed_lanes(const SeedArgs &args, int2 *pairs, hipStream_t s);
 // DTW of a pair list (seeds) of the series d_args (device); atomicMin into its best-so-far and
 // pushes candidates like the join.
 int launch_seed_dtw(const JoinArgs *d_args, int32_t m, int32_t w, const int2 *pairs, int32_t npairs, hipStream_t s);
+// Batched forms: the seed scans of nj series (nblocks = sum of n_seed + 1, base = their prefix
+// sums) in one launch, and the seed-pair DTWs of nj series of one band class (QP of w; LDS
+// for the longest window max_m).
+int launch_seed_batch(const SeedJob *jobs, const int32_t *base, int32_t nj, int32_t nblocks, hipStream_t s);
+int launch_seed_dtw_batch(const JoinArgs *d_args, int32_t max_m, int32_t w, const SeedDtwJob *jobs,
+                          const int32_t *base, int32_t nj, int32_t nblocks, hipStream_t s);
 // Error model, validity and state resets of nseries device join descriptors; zeroes *d_work.
 int launch_join_setup(JoinArgs *d_series, int32_t nseries, int32_t *d_work, hipStream_t s);
 // The fused persistent join over nseries series (device descriptors, cumulative item counts).

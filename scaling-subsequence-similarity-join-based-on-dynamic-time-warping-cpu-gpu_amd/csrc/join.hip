@@ -1155,15 +1155,15 @@ __global__ void k_join_setup(JoinArgs *series, int32_t nseries, int32_t *work_co
 // closest pair of its own four diagonals, the candidate source of the learned proposal).
 // Block n_seed writes the fixed seed pair (0, m).
 template <bool PER_LANE>
-__global__ __launch_bounds__(64) void k_seed(SeedArgs S, int2 *pairs) {
+__device__ __forceinline__ void seed_block(const SeedArgs &S, const int blk, int2 *pairs) {
     const int lane = (int)threadIdx.x;
     const int m = S.m;
-    if ((int)blockIdx.x == S.n_seed) {
-        if (lane == 0 && !PER_LANE) pairs[blockIdx.x] = make_int2(0, m);
+    if (blk == S.n_seed) {
+        if (lane == 0 && !PER_LANE) pairs[blk] = make_int2(0, m);
         return;
     }
     const int64_t nw = S.nw;
-    const int64_t item = ((int64_t)blockIdx.x * S.n_items) / S.n_seed;
+    const int64_t item = ((int64_t)blk * S.n_items) / S.n_seed;
     int g;
     int64_t sidx;
     decode_item(S.item_start, S.groups, item, g, sidx);
@@ -1239,7 +1239,7 @@ __global__ __launch_bounds__(64) void k_seed(SeedArgs S, int2 *pairs) {
         }
     }
     if (PER_LANE) {
-        pairs[(size_t)blockIdx.x * 64 + lane] = make_int2(bi, bj);
+        pairs[(size_t)blk * 64 + lane] = make_int2(bi, bj);
         return;
     }
     // wave argmin
@@ -1251,19 +1251,43 @@ __global__ __launch_bounds__(64) void k_seed(SeedArgs S, int2 *pairs) {
     const unsigned long long msk = __ballot(best == v && bi >= 0);
     const int src = msk ? (int)__ffsll((long long)msk) - 1 : 0;
     const int wi = __shfl(bi, src, 64), wj = __shfl(bj, src, 64);
-    if (lane == 0) pairs[blockIdx.x] = msk ? make_int2(wi, wj) : make_int2(-1, -1);
+    if (lane == 0) pairs[blk] = msk ? make_int2(wi, wj) : make_int2(-1, -1);
+}
+
+template <bool PER_LANE>
+__global__ __launch_bounds__(64) void k_seed(SeedArgs S, int2 *pairs) {
+    seed_block<PER_LANE>(S, (int)blockIdx.x, pairs);
+}
+
+// Series t of a batch owns blocks base[t] .. base[t+1] - 1 (first t with base[t+1] > b).
+__device__ inline int batch_owner(const int32_t *base, int32_t nj, int32_t b) {
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (base[mid + 1] > b)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// The seed scans of every series of a batch in one launch.
+__global__ __launch_bounds__(64) void k_seed_batch(const SeedJob *__restrict__ jobs, const int32_t *__restrict__ base,
+                                                   int32_t nj) {
+    const int b = (int)blockIdx.x;
+    const int t = batch_owner(base, nj, b);
+    seed_block<false>(jobs[t].S, b - base[t], jobs[t].pairs);
 }
 
 // ---- DTW of a list of candidate pairs (seeds), one pair per wave; pushes results like the
 // join does.
 template <int QP>
-__global__ __launch_bounds__(64) void k_seed_dtw(const JoinArgs *__restrict__ Ap, const int2 *pairs, int npairs) {
-    extern __shared__ float zs[];
-    const JoinArgs &A = *Ap;
+__device__ __forceinline__ void seed_dtw_block(const JoinArgs &A, const int2 *pairs, int npairs, int p, float *zs) {
     const int lane = (int)threadIdx.x;
     const int m = A.m;
-    if (blockIdx.x >= (unsigned)npairs || !A.valid) return;
-    const int2 pr = pairs[blockIdx.x];
+    if (p >= npairs || !A.valid) return;
+    const int2 pr = pairs[p];
     if (pr.x < 0) return;
     const int i = pr.x, j = pr.y;
     const Thr th = load_threshold(A);
@@ -1281,6 +1305,23 @@ __global__ __launch_bounds__(64) void k_seed_dtw(const JoinArgs *__restrict__ Ap
         if (slot < A.cand_cap) A.cands[slot] = make_int4(i, j, __float_as_int(res), 0);
         atomicMin(A.bsf, pack_bsf(res, (uint32_t)i));
     }
+}
+
+template <int QP>
+__global__ __launch_bounds__(64) void k_seed_dtw(const JoinArgs *__restrict__ Ap, const int2 *pairs, int npairs) {
+    extern __shared__ float zs[];
+    seed_dtw_block<QP>(*Ap, pairs, npairs, (int)blockIdx.x, zs);
+}
+
+// The seed pairs of several series of a batch (one band class) in one launch.
+template <int QP>
+__global__ __launch_bounds__(64) void k_seed_dtw_batch(const JoinArgs *__restrict__ args,
+                                                       const SeedDtwJob *__restrict__ jobs,
+                                                       const int32_t *__restrict__ base, int32_t nj) {
+    extern __shared__ float zs[];
+    const int b = (int)blockIdx.x;
+    const int k = batch_owner(base, nj, b);
+    seed_dtw_block<QP>(args[jobs[k].series], jobs[k].pairs, jobs[k].npairs, b - base[k], zs);
 }
 
 }  // namespace
@@ -1321,6 +1362,29 @@ int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t ns
 
 int launch_seed(const SeedArgs &S, int2 *pairs, hipStream_t s) {
     hipLaunchKernelGGL(k_seed<false>, dim3(S.n_seed + 1), dim3(64), 0, s, S, pairs);
+    DTWJ_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_seed_batch(const SeedJob *jobs, const int32_t *base, int32_t nj, int32_t nblocks, hipStream_t s) {
+    if (nj <= 0 || nblocks <= 0) return 0;
+    hipLaunchKernelGGL(k_seed_batch, dim3(nblocks), dim3(64), 0, s, jobs, base, nj);
+    DTWJ_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_seed_dtw_batch(const JoinArgs *d_args, int32_t max_m, int32_t w, const SeedDtwJob *jobs,
+                          const int32_t *base, int32_t nj, int32_t nblocks, hipStream_t s) {
+    if (nj <= 0 || nblocks <= 0) return 0;
+    const int QP = dtw_qp_for(w);
+    const size_t lds = sizeof(float) * 2 * (size_t)max_m;
+    switch (QP) {
+        case 1: hipLaunchKernelGGL(k_seed_dtw_batch<1>, dim3(nblocks), dim3(64), lds, s, d_args, jobs, base, nj); break;
+        case 2: hipLaunchKernelGGL(k_seed_dtw_batch<2>, dim3(nblocks), dim3(64), lds, s, d_args, jobs, base, nj); break;
+        case 4: hipLaunchKernelGGL(k_seed_dtw_batch<4>, dim3(nblocks), dim3(64), lds, s, d_args, jobs, base, nj); break;
+        case 8: hipLaunchKernelGGL(k_seed_dtw_batch<8>, dim3(nblocks), dim3(64), lds, s, d_args, jobs, base, nj); break;
+        default: set_error("band too wide: 2w+1 must be <= 1024"); return DTWJ_EINVAL;
+    }
     DTWJ_HIP(hipGetLastError());
     return 0;
 }
