@@ -42,7 +42,8 @@ extern "C" {
 /* ---- options and results --------------------------------------------------------- */
 typedef struct dtwj_options {
     int32_t device;        /* HIP device ordinal (default 0) */
-    int32_t seed_items;    /* ED-scan seed work items before the join (default 0 = automatic) */
+    int32_t seed_items;    /* ED-scan seed work items before the join (0 = automatic: 1024, or 4096
+                              per series of a batch) */
     int32_t rows_per_item; /* rows per sweep work item, multiple of 128 (default 0 = automatic) */
     int32_t waves_per_cu;  /* persistent waves per CU (default 0 = automatic) */
     int32_t verbose;       /* print per-stage counters to stderr */
