@@ -41,6 +41,21 @@ def test_batch_matches_oracle_brute_force(gpu, oracle):
         assert abs(res[t].dist - bd) <= 1e-9 * bd
 
 
+def test_batch_with_several_band_classes_matches_oracle(gpu, oracle):
+    """Series whose bands need 1, 2, 4 and 8 offset pairs per lane in one batch: the seed DTWs
+    run as one launch per class, the join with the widest class's kernel."""
+    mix = [(3000, 300, 150, 11), (1800, 128, 6, 130), (2500, 200, 80, 131), (1600, 300, 280, 13)]
+    series = [np.cumsum(np.random.default_rng(seed).standard_normal(n)) for n, _, _, seed in mix]
+    ms, ws = [m for _, m, _, _ in mix], [w for _, _, w, _ in mix]
+    refs = [oracle.motif_discovery(a, m, w) for a, m, w in zip(series, ms, ws)]
+    for opts in ({}, {"graph": -1}):
+        res = gpu.top1_batch(series, ms, ws, **opts)
+        for t, (o, m, w) in enumerate(zip(refs, ms, ws)):
+            assert not isinstance(res[t], Exception), (t, res[t])
+            assert (res[t].i, res[t].j) == (o.i, o.j), (opts, t, m, w)
+            assert abs(res[t].dist - o.bsf) <= 1e-9 * o.bsf
+
+
 def test_graph_and_direct_launch_agree_and_repeat(gpu):
     series, ms, ws = make_batch(gpu, MIX[:6])
     ctx = gpu.Context()
