@@ -122,6 +122,7 @@ struct JoinArgs {
     const float *ac;       // n, a - c0 (FP32)
     const float *uc;       // n, U - c0
     const float *lc;       // n, L - c0
+    const uint32_t *wc;    // n, envelope widths (U - a, a - L) as FP16 rounded up: lo, hi half
     const float *mu_c;     // nw, mu - c0
     const float *blkmax;   // 3 x nblk: max sigma, max sv, max 1/sigma over 256-window blocks
     const float *gstat;    // preprocessing statistics (value range, see Prep::gstat)

@@ -234,6 +234,7 @@ size_t job_bytes(const Job &J) {
     add(8 * nw * 3);      // mu sig muv
     add(8);               // c0
     add(4 * n * 3);       // ac uc lc
+    add(4 * n);           // wc
     add(4 * nw);          // mu_c
     add(sizeof(WinRow) * nw);
     add(4 * 3 * nblk);
@@ -268,6 +269,7 @@ void take_job(Arena &ar, Job &J) {
     P.ac = ar.take<float>(n);
     P.uc = ar.take<float>(n);
     P.lc = ar.take<float>(n);
+    P.wc = ar.take<uint32_t>(n);
     P.mu_c = ar.take<float>(nw);
     P.win = ar.take<WinRow>(nw);
     P.nblk = (int32_t)((nw + 255) / 256);
@@ -294,6 +296,7 @@ void take_job(Arena &ar, Job &J) {
     A.ac = P.ac;
     A.uc = P.uc;
     A.lc = P.lc;
+    A.wc = P.wc;
     A.mu_c = P.mu_c;
     A.blkmax = P.blkmax;
     A.gstat = P.gstat;

@@ -18,6 +18,7 @@ struct Prep {
     double *c0 = nullptr;                             // 1: centring constant (min+max)/2
     // FP32 outputs
     float *ac = nullptr, *uc = nullptr, *lc = nullptr;  // n
+    uint32_t *wc = nullptr;                             // n: FP16 widths uc - ac (lo), ac - lc (hi)
     float *mu_c = nullptr;                              // nw
     WinRow *win = nullptr;                              // nw
     float *blkmax = nullptr;                            // 3 * nblk
@@ -53,7 +54,7 @@ int launch_join_setup(JoinArgs *d_series, int32_t nseries, int32_t *d_work, hipS
 // The fused persistent join over nseries series (device descriptors, cumulative item counts).
 int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t nseries, int32_t max_m, int32_t max_w,
                 int32_t *d_work, uint32_t *d_queue, int32_t grid, hipStream_t s);
-size_t join_lds_bytes(int32_t m);
+size_t join_lds_bytes(int32_t m, int32_t w);
 
 // DTW of an explicit pair list (dtw.hip).  prec 32: FP32 cells from ac/mu_c/isig;
 // prec 64: FP64 cells from a/mu/sig.  cost: npairs doubles (device).  Reads only the

@@ -67,16 +67,21 @@ constexpr int QLIM = kJoinQueueCap - 8 * 64;  // drain before an 8-row group cou
 enum { C_SWEPT = 0, C_BALL, C_KIM, C_KEOGH, C_DTW_AB, C_CELLS, C_KEOGH_ELEMS, C_SWEPT_ONE, C_NCOUNTERS };
 static_assert(C_NCOUNTERS == 8, "the host reads 8 counters per series");
 
-// LDS floats of the cascade staging for window length m (overlays the column tables):
-// series / upper / lower envelope of the chunk's row windows and column windows, and the
-// centred mean and 1/sigma of those windows.
-__host__ __device__ constexpr int cascade_floats(int m) {
-    return 3 * (RC + m) + 3 * (RC + DG + m) + 2 * RC + 2 * (RC + DG);
+// LDS words of the cascade staging for window length m (overlays the column tables): series
+// and envelope of the chunk's row windows and column windows (upper and lower envelope in FP32,
+// or, with pw, one word of packed FP16 widths), and the centred mean and 1/sigma of those
+// windows.
+__host__ __device__ constexpr int cascade_floats(int m, bool pw) {
+    return (pw ? 2 : 3) * (RC + m) + (pw ? 2 : 3) * (RC + DG + m) + 2 * RC + 2 * (RC + DG);
 }
 
-__host__ __device__ constexpr int union_floats(int m) {
-    return cascade_floats(m) > kSweepFloats ? cascade_floats(m) : kSweepFloats;
+__host__ __device__ constexpr int union_floats(int m, bool pw) {
+    return cascade_floats(m, pw) > kSweepFloats ? cascade_floats(m, pw) : kSweepFloats;
 }
+
+// Join kernels whose cascade stages packed envelope widths: those of bands of 129..255 offsets
+// (two offset pairs per lane), which come with long windows and run two waves per SIMD.
+__host__ __device__ constexpr bool packed_envelope(int Q) { return Q == 2; }
 
 struct Thr {
     float T2;      // squared-cost threshold (best-so-far inflated by the FP32 DTW error)
@@ -183,15 +188,25 @@ struct LaneSuffix {
 };
 
 // LDS layout of the cascade's staging (valid while draining) for window length m: mean and
+// LDS layout of the cascade's staging (valid while draining) for window length m: mean and
 // 1/sigma of the part's RC rows and of its RC + DG columns, then the windows of the rows (CI
-// samples: series, upper and lower envelope) and of the columns (CJ samples).  The DTW reads
-// up to (w + 3) / 2 + 1 <= 2 RC + 2 (RC + DG) values before and after a window without
-// clamping: before the row series lie the statistics, before the column series the row lower
-// envelope, after each series its upper envelope (all finite, w < m).  Built where it is used
-// from an opaque copy of m, so that the ten addresses are not kept live (in scalar registers
-// that would spill) through the sweep.
+// samples) and of the columns (CJ samples).  Without PW: series, upper and lower envelope of
+// the rows, then the same of the columns, all FP32.  With PW: the two series, then the envelope
+// widths of the same windows, two FP16 values per word (lo: U - a, hi: a - L, both rounded up,
+// so a + lo >= U and a - hi <= L in FP32: a widened envelope, under which LB_Keogh and the DTW
+// suffix bound stay lower bounds) in 2/3 of the space.  The DTW reads up to
+// (w + 3) / 2 + 1 <= 2 RC + 2 (RC + DG) values before and after a series window without
+// clamping; those reads land on finite values: before the row series the statistics, after it
+// the row upper envelope (PW: the column series), before the column series the row lower
+// envelope (PW: the row series), after it the column upper envelope (PW: the row widths, whose
+// words read as floats are finite: sign 0 and exponent field <= 248, both halves being
+// non-negative FP16 values up to +inf).  Built where it is used from an opaque copy of m, so that
+// the addresses are not kept live (in scalar registers that would spill) through the sweep.
+template <bool PW>
 struct CascadeLDS {
-    float *sai, *sui, *sli, *saj, *suj, *slj, *rmu, *ris, *cmu, *cis;
+    float *sai, *saj, *rmu, *ris, *cmu, *cis;
+    float *sui = nullptr, *sli = nullptr, *suj = nullptr, *slj = nullptr;  // without PW
+    uint32_t *swi = nullptr, *swj = nullptr;                              // with PW
     __device__ CascadeLDS(float *base, int m) {
         asm volatile("" : "+s"(m));
         const int CI = RC + m, CJ = RC + DG + m;
@@ -200,11 +215,39 @@ struct CascadeLDS {
         cmu = ris + RC;
         cis = cmu + (RC + DG);
         sai = cis + (RC + DG);
-        sui = sai + CI;
-        sli = sui + CI;
-        saj = sli + CI;
-        suj = saj + CJ;
-        slj = suj + CJ;
+        if constexpr (PW) {
+            saj = sai + CI;
+            swi = reinterpret_cast<uint32_t *>(saj + CJ);
+            swj = swi + CI;
+        } else {
+            sui = sai + CI;
+            sli = sui + CI;
+            saj = sli + CI;
+            suj = saj + CJ;
+            slj = suj + CJ;
+        }
+    }
+    // widened envelope (upper, lower) at a sample x with packed widths wb
+    __device__ static void env(float x, uint32_t wb, float &up, float &lo) {
+        up = x + (float)__builtin_bit_cast(_Float16, (uint16_t)(wb & 0xffffu));
+        lo = x - (float)__builtin_bit_cast(_Float16, (uint16_t)(wb >> 16));
+    }
+    // envelope of the row window at position k (series value x = sai[k]), of the column window
+    __device__ void env_i(int k, float x, float &up, float &lo) const {
+        if constexpr (PW)
+            env(x, swi[k], up, lo);
+        else {
+            up = sui[k];
+            lo = sli[k];
+        }
+    }
+    __device__ void env_j(int k, float x, float &up, float &lo) const {
+        if constexpr (PW)
+            env(x, swj[k], up, lo);
+        else {
+            up = suj[k];
+            lo = slj[k];
+        }
     }
 };
 static_assert((511 + 3) / 2 + 1 <= 2 * RC + 2 * (RC + DG),
@@ -219,8 +262,9 @@ __device__ inline int pair_col(uint32_t p) { return (int)(p >> 16); }
 // device work queue is the range [0, item_base[nseries]) claimed through work_counter);
 // uni: floats of the LDS union area, sized for the longest window of the launch.
 template <int Q>
-__global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base,
-                                             int32_t nseries, int32_t uni, int32_t *work_counter, uint32_t *queue) {
+__device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base,
+                                          int32_t nseries, int32_t uni, int32_t *work_counter, uint32_t *queue) {
+    constexpr bool PW = packed_envelope(Q);
     extern __shared__ __align__(16) float smem[];  // 16-byte aligned: the sweep reads float4 rows
     const int lane = (int)threadIdx.x;
     uint32_t *q1 = queue + (size_t)blockIdx.x * kJoinQueueCap;  // this wave's survivor queue
@@ -274,7 +318,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
     // (segments of w + 1 lanes); wider bands one pair per wave with QP offset pairs per lane.
     auto process_q3 = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
-        const CascadeLDS L(smem, m);
+        const CascadeLDS<PW> L(smem, m);
         const int w = A.w;
         const int seg_len = w <= 63 ? w + 1 : 64;
         const int per_wave = 64 / seg_len;
@@ -331,9 +375,13 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                                 const int r = r0 + ((rr + x) & 7);
                                 if (r < m) {
                                     // envelope z-values by the same map as the series'
+                                    const float xi = L.sai[li + r], xj = L.saj[lj + r];
+                                    float Ui, Li, Uj, Lj;
+                                    L.env_i(li + r, xi, Ui, Li);
+                                    L.env_j(lj + r, xj, Uj, Lj);
                                     const float zi = z.row(r), zj = z.col(r);
-                                    const float uj = fmaf(L.suj[lj + r], ij, nmj), lwj = fmaf(L.slj[lj + r], ij, nmj);
-                                    const float ui = fmaf(L.sui[li + r], ii, nmi), lwi = fmaf(L.sli[li + r], ii, nmi);
+                                    const float uj = fmaf(Uj, ij, nmj), lwj = fmaf(Lj, ij, nmj);
+                                    const float ui = fmaf(Ui, ii, nmi), lwi = fmaf(Li, ii, nmi);
                                     const float ta = fmaxf(fmaxf(zi - uj, lwj - zi), 0.f);
                                     const float tb = fmaxf(fmaxf(zj - ui, lwi - zj), 0.f);
                                     vA = fmaf(ta, ta, vA);
@@ -387,7 +435,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
     // the next pair as soon as its pair is abandoned or complete.  Consumes the whole queue.
     auto process_q2 = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
-        const CascadeLDS L(smem, m);
+        const CascadeLDS<PW> L(smem, m);
         __syncthreads();  // q2 entries were written by other lanes
         // th: the drain batch's thresholds (at most one batch old)
         const float sm = sqrtf((float)m);
@@ -448,42 +496,76 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                 // subtraction and two FMAs per direction (rounding error <= 2u (|x| + |U rA| +
                 // |bA|), inside the 4u of eA / eB)
                 auto term = [&](int kk) __attribute__((always_inline)) {
-                    const float cx = L.saj[lj + kk] - bA;
-                    const float t = fmaxf(fmaxf(fmaf(-L.sui[li + kk], rA, cx), fmaf(L.sli[li + kk], rA, -cx)), 0.f);
+                    const float xj = L.saj[lj + kk], xi = L.sai[li + kk];
+                    float Ui, Li, Uj, Lj;
+                    L.env_i(li + kk, xi, Ui, Li);
+                    L.env_j(lj + kk, xj, Uj, Lj);
+                    const float cx = xj - bA;
+                    const float t = fmaxf(fmaxf(fmaf(-Ui, rA, cx), fmaf(Li, rA, -cx)), 0.f);
                     accA = fmaf(t, t, accA);
-                    const float cy = L.sai[li + kk] - bB;
-                    const float u = fmaxf(fmaxf(fmaf(-L.suj[lj + kk], rB, cy), fmaf(L.slj[lj + kk], rB, -cy)), 0.f);
+                    const float cy = xi - bB;
+                    const float u = fmaxf(fmaxf(fmaf(-Uj, rB, cy), fmaf(Lj, rB, -cy)), 0.f);
                     accB = fmaf(u, u, accB);
                 };
                 auto strided = [&]() __attribute__((always_inline)) {
                     constexpr int S = KS;
                     const int base = 16 * S * kc + kb;
                     // block base pointers, so that the 16 reads per array use immediate offsets
-                    const float *xaj = L.saj + lj + base, *xui = L.sui + li + base, *xli = L.sli + li + base;
-                    const float *xai = L.sai + li + base, *xuj = L.suj + lj + base, *xlj = L.slj + lj + base;
                     // the reads of HB positions go out together (one wait), then their arithmetic
                     constexpr int HB = 8;
+                    const float *xaj = L.saj + lj + base, *xai = L.sai + li + base;
+                    if constexpr (PW) {
+                        const uint32_t *xwi = L.swi + li + base, *xwj = L.swj + lj + base;
 #pragma unroll
-                    for (int h = 0; h < 16; h += HB) {
-                        float va[HB], vu[HB], vl[HB], wa[HB], wu[HB], wl[HB];
+                        for (int h = 0; h < 16; h += HB) {
+                            float va[HB], wa[HB];
+                            uint32_t vw[HB], ww[HB];
 #pragma unroll
-                        for (int t = 0; t < HB; ++t) {
-                            va[t] = xaj[S * (h + t)];
-                            vu[t] = xui[S * (h + t)];
-                            vl[t] = xli[S * (h + t)];
-                            wa[t] = xai[S * (h + t)];
-                            wu[t] = xuj[S * (h + t)];
-                            wl[t] = xlj[S * (h + t)];
+                            for (int t = 0; t < HB; ++t) {
+                                va[t] = xaj[S * (h + t)];
+                                wa[t] = xai[S * (h + t)];
+                                vw[t] = xwi[S * (h + t)];
+                                ww[t] = xwj[S * (h + t)];
+                            }
+                            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                            for (int t = 0; t < HB; ++t) {
+                                float Ui, Li, Uj, Lj;
+                                L.env(wa[t], vw[t], Ui, Li);
+                                L.env(va[t], ww[t], Uj, Lj);
+                                const float cx = va[t] - bA;
+                                const float ta = fmaxf(fmaxf(fmaf(-Ui, rA, cx), fmaf(Li, rA, -cx)), 0.f);
+                                accA = fmaf(ta, ta, accA);
+                                const float cy = wa[t] - bB;
+                                const float tb = fmaxf(fmaxf(fmaf(-Uj, rB, cy), fmaf(Lj, rB, -cy)), 0.f);
+                                accB = fmaf(tb, tb, accB);
+                            }
                         }
-                        __builtin_amdgcn_sched_barrier(0);
+                    } else {
+                        const float *xui = L.sui + li + base, *xli = L.sli + li + base;
+                        const float *xuj = L.suj + lj + base, *xlj = L.slj + lj + base;
 #pragma unroll
-                        for (int t = 0; t < HB; ++t) {
-                            const float cx = va[t] - bA;
-                            const float ta = fmaxf(fmaxf(fmaf(-vu[t], rA, cx), fmaf(vl[t], rA, -cx)), 0.f);
-                            accA = fmaf(ta, ta, accA);
-                            const float cy = wa[t] - bB;
-                            const float tb = fmaxf(fmaxf(fmaf(-wu[t], rB, cy), fmaf(wl[t], rB, -cy)), 0.f);
-                            accB = fmaf(tb, tb, accB);
+                        for (int h = 0; h < 16; h += HB) {
+                            float va[HB], vu[HB], vl[HB], wa[HB], wu[HB], wl[HB];
+#pragma unroll
+                            for (int t = 0; t < HB; ++t) {
+                                va[t] = xaj[S * (h + t)];
+                                vu[t] = xui[S * (h + t)];
+                                vl[t] = xli[S * (h + t)];
+                                wa[t] = xai[S * (h + t)];
+                                wu[t] = xuj[S * (h + t)];
+                                wl[t] = xlj[S * (h + t)];
+                            }
+                            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                            for (int t = 0; t < HB; ++t) {
+                                const float cx = va[t] - bA;
+                                const float ta = fmaxf(fmaxf(fmaf(-vu[t], rA, cx), fmaf(vl[t], rA, -cx)), 0.f);
+                                accA = fmaf(ta, ta, accA);
+                                const float cy = wa[t] - bB;
+                                const float tb = fmaxf(fmaxf(fmaf(-wu[t], rB, cy), fmaf(wl[t], rB, -cy)), 0.f);
+                                accB = fmaf(tb, tb, accB);
+                            }
                         }
                     }
                     if (++kc == nsup) {
@@ -533,7 +615,7 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
     // LB_Keogh and DTW; everything is flushed before the next part is staged.
     auto drain = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
-        const CascadeLDS L(smem, m);
+        const CascadeLDS<PW> L(smem, m);
         const int64_t n = A.n, nw = A.nw;
         const int CI = RC + m, CJ = RC + DG + m;
         __syncthreads();  // queue entries were stored by other lanes; column tables retire
@@ -558,9 +640,32 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
         {
             const global_ptr<float> ac = (global_ptr<float>)A.ac, uc = (global_ptr<float>)A.uc,
                                     lc = (global_ptr<float>)A.lc, muc = (global_ptr<float>)A.mu_c;
+            const global_ptr<uint32_t> wc = (global_ptr<uint32_t>)A.wc;
             const global_ptr<float> wf = (global_ptr<float>)A.win;  // WinRow as 8 floats
             constexpr int WISIG = 2;                                // float index of isig
             static_assert(sizeof(WinRow) == 32 && offsetof(WinRow, isig) == 4 * WISIG, "WinRow layout");
+            auto stage2 = [&](float *dap, uint32_t *dwp, int64_t base, int cnt) __attribute__((always_inline)) {
+                for (int x0 = 0; x0 < cnt; x0 += 512) {
+                    float v0[8];
+                    uint32_t v1[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int64_t p = base + x0 + 64 * u + lane;
+                        const int64_t pc = p < n - 1 ? p : n - 1;
+                        v0[u] = ac[pc];
+                        v1[u] = wc[pc];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int x = x0 + 64 * u + lane;
+                        if (x < cnt) {
+                            const bool ok = base + x < n;
+                            dap[x] = ok ? v0[u] : 0.f;
+                            dwp[x] = ok ? v1[u] : 0u;
+                        }
+                    }
+                }
+            };
             auto stage3 = [&](float *d0p, float *d1p, float *d2p, int64_t base, int cnt) __attribute__((always_inline)) {
                 for (int x0 = 0; x0 < cnt; x0 += 512) {
                     float v0[8], v1[8], v2[8];
@@ -605,8 +710,13 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
                     }
                 }
             };
-            stage3(L.sai, L.sui, L.sli, cb, CI);
-            stage3(L.saj, L.suj, L.slj, cb + d0, CJ);
+            if constexpr (PW) {
+                stage2(L.sai, L.swi, cb, CI);
+                stage2(L.saj, L.swj, cb + d0, CJ);
+            } else {
+                stage3(L.sai, L.sui, L.sli, cb, CI);
+                stage3(L.saj, L.suj, L.slj, cb + d0, CJ);
+            }
             stage_stats(L.rmu, L.ris, cb, RC);
             stage_stats(L.cmu, L.cis, cb + d0, RC + DG);
         }
@@ -1131,6 +1241,22 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
     flush_counters();
 }
 
+template <int Q>
+__global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base,
+                                             int32_t nseries, int32_t uni, int32_t *work_counter, uint32_t *queue) {
+    join_body<Q>(series, item_base, nseries, uni, work_counter, queue);
+}
+
+// The same join held to the register budget of two waves per SIMD (a few spills to scratch):
+// used for bands of two offset pairs per lane, whose packed-envelope staging lets two waves
+// per SIMD fit the LDS up to m = 1024.
+template <int Q>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_join_occ2(
+    const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base, int32_t nseries, int32_t uni,
+    int32_t *work_counter, uint32_t *queue) {
+    join_body<Q>(series, item_base, nseries, uni, work_counter, queue);
+}
+
 // ---- set-up of every series of a join: the FP32 error model from the preprocessing's value
 // range, the validity flag, and the best-so-far / candidate / counter resets (a kernel, so
 // that a captured graph holds no memset node).  One thread per series.
@@ -1326,8 +1452,9 @@ __global__ __launch_bounds__(64) void k_seed_dtw_batch(const JoinArgs *__restric
 
 }  // namespace
 
-size_t join_lds_bytes(int32_t m) {
-    return sizeof(float) * (size_t)union_floats(m) + sizeof(uint32_t) * (PLCAP + Q2CAP + Q3CAP);
+size_t join_lds_bytes(int32_t m, int32_t w) {
+    return sizeof(float) * (size_t)union_floats(m, packed_envelope(dtw_qp_for(w))) +
+           sizeof(uint32_t) * (PLCAP + Q2CAP + Q3CAP);
 }
 
 int launch_join_setup(JoinArgs *d_series, int32_t nseries, int32_t *d_work, hipStream_t s) {
@@ -1339,8 +1466,8 @@ int launch_join_setup(JoinArgs *d_series, int32_t nseries, int32_t *d_work, hipS
 int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t nseries, int32_t max_m, int32_t max_w,
                 int32_t *d_work, uint32_t *d_queue, int32_t grid, hipStream_t s) {
     const int Q = dtw_qp_for(max_w);
-    const size_t lds = join_lds_bytes(max_m);
-    const int32_t uni = union_floats(max_m);
+    const size_t lds = join_lds_bytes(max_m, max_w);
+    const int32_t uni = union_floats(max_m, packed_envelope(Q));
     if (lds + 64 > 160 * 1024) {  // + the kernel's static stage counters
         set_error("window too long for the join's LDS staging");
         return DTWJ_EINVAL;
@@ -1350,7 +1477,10 @@ int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t ns
                        d_queue)
     switch (Q) {
         case 1: DTWJ_LAUNCH_JOIN(1); break;
-        case 2: DTWJ_LAUNCH_JOIN(2); break;
+        case 2:  // two waves per SIMD (packed envelope staging)
+            hipLaunchKernelGGL(k_join_occ2<2>, dim3(grid), dim3(64), lds, s, d_series, d_item_base, nseries, uni,
+                               d_work, d_queue);
+            break;
         case 4: DTWJ_LAUNCH_JOIN(4); break;
         case 8: DTWJ_LAUNCH_JOIN(8); break;
         default: set_error("band too wide: 2w+1 must be <= 1024"); return DTWJ_EINVAL;

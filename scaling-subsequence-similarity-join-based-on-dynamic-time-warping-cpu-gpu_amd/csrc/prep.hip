@@ -234,6 +234,16 @@ __device__ inline WinStats window_stats(const dd_t *P, int64_t n, int m, double 
     return r;
 }
 
+// Bits of the smallest FP16 value >= v (v >= 0; +inf beyond the FP16 range).
+__device__ inline uint32_t f16_up(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, f) + 1u);
+    const _Float16 h = (_Float16)f;
+    uint32_t b = (uint32_t)__builtin_bit_cast(uint16_t, h);
+    if ((float)h < f) ++b;  // h is finite here (an infinite h is above f)
+    return b;
+}
+
 __device__ inline void atomic_max_pos(float *p, float x) { atomicMax((unsigned int *)p, fbits(x)); }
 __device__ inline void atomic_min_pos(float *p, float x) { atomicMin((unsigned int *)p, fbits(x)); }
 
@@ -313,7 +323,7 @@ __global__ __launch_bounds__(256) void k_window_stats(const double *__restrict__
 
 __global__ __launch_bounds__(256) void k_positions(const double *__restrict__ a, const double *__restrict__ U,
                                                    const double *__restrict__ L, const double *c0p, int64_t n,
-                                                   float *ac, float *uc, float *lc, float *gstat) {
+                                                   float *ac, float *uc, float *lc, uint32_t *wc, float *gstat) {
     __shared__ float red[256];
     const double c0 = *c0p;
     int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -323,6 +333,9 @@ __global__ __launch_bounds__(256) void k_positions(const double *__restrict__ a,
         ac[t] = x;
         uc[t] = u;
         lc[t] = l;
+        // widths of the FP32 envelope around the FP32 sample, exact in FP64, as the smallest
+        // FP16 values not below them: x + wU >= u and x - wL <= l hold after any FP32 rounding
+        wc[t] = f16_up((double)u - (double)x) | (f16_up((double)x - (double)l) << 16);
         mx = fmaxf(fabsf(x), fmaxf(fabsf(u), fabsf(l)));
     }
     red[threadIdx.x] = mx;
@@ -356,7 +369,7 @@ int prep_series(Prep &P, hipStream_t s) {
     hipLaunchKernelGGL(k_window_stats, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, P.a, P.v, P.P, P.c0, n, m,
                        P.mu, P.sig, P.muv, P.mu_c, P.win, P.blkmax, P.nblk, P.gstat, P.flags);
     hipLaunchKernelGGL(k_positions, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P.a, P.U, P.L, P.c0, n, P.ac,
-                       P.uc, P.lc, P.gstat);
+                       P.uc, P.lc, P.wc, P.gstat);
     DTWJ_HIP(hipGetLastError());
     return 0;
 }
