@@ -50,14 +50,16 @@ namespace {
 
 constexpr int KD = 4;                  // diagonals per lane
 constexpr int DG = 64 * KD;            // diagonals per item
-constexpr int RC = 64;                 // rows per LDS chunk
-constexpr int JW = RC + DG + 8;        // staged column entries per chunk
+constexpr int RC = 64;                 // rows per cascade part (the drain stages RC rows at a time)
+constexpr int JW = RC + DG + 8;        // columns an RC-row part spans
 constexpr int NJARR = 6;               // column arrays: dfa dga dfv dgv 1/sigma thetaB
-constexpr int kSweepFloats = NJARR * JW + NJARR * RC;  // column + row tables
+constexpr int RC2 = 128;               // rows per chunk of the two-sided sweep (6 + 6 tables)
+constexpr int JW2 = RC2 + DG + 8;      // its staged column entries
+constexpr int kSweepFloats = NJARR * JW2 + NJARR * RC2;  // column + row tables
 constexpr int RC1 = 256;               // rows per chunk of the one-sided sweep (3 + 3 tables)
 constexpr int JW1 = RC1 + DG + 8;
 static_assert(3 * JW1 + 3 * RC1 <= kSweepFloats, "one-sided tables fit the two-sided area");
-static_assert(RC1 % RC == 0, "the cascade drains a one-sided chunk in RC-row parts");
+static_assert(RC1 % RC == 0 && RC2 % RC == 0, "the cascade drains a chunk in RC-row parts");
 constexpr int PLCAP = 64 * KD;         // expanded pairs of one 64-entry queue batch
 constexpr int Q2CAP = 256;             // LB_Keogh queue (packed pairs)
 constexpr int Q2RUN = Q2CAP - 64;      // run LB_Keogh once this many pairs are queued
@@ -307,8 +309,8 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     int64_t cb = 0;  // first row of the RC rows staged for the cascade
     int m = 0;
     Thr th{};
-    float *jarr = smem;               // NJARR x JW column tables of the sweep
-    float *rarr = jarr + NJARR * JW;  // NJARR x RC row tables of the sweep
+    float *jarr = smem;                // NJARR x JW2 column tables of the sweep
+    float *rarr = jarr + NJARR * JW2;  // NJARR x RC2 row tables of the sweep
     // cascade staging (valid only while draining), laid out per series from its window length
     uint32_t *plist = reinterpret_cast<uint32_t *>(smem + uni);  // expanded pairs of a batch
     uint32_t *q2 = plist + PLCAP;                                // LB_Keogh queue
@@ -964,10 +966,10 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
         const float EA = 0x1p-22f * steps * (float)m * smaxJ * svmaxI * ismaxJ;
         const float EB = 0x1p-22f * steps * (float)m * smaxI * svmaxJ * ismaxI;
 
-        // LDS tables of the sweep.  Two-sided: NJARR column tables of JW entries and NJARR row
-        // tables of RC entries.  One-sided: 3 + 3 tables of chunks RC1 rows high in the same area
-        // (columns dfa dga 1/sigma, rows dfv dgv thetaA), so that its chunks are 4x longer.
-        const int rc = two_sided ? RC : RC1;
+        // LDS tables of the sweep.  Two-sided: NJARR column tables of JW2 entries and NJARR row
+        // tables of RC2 entries.  One-sided: 3 + 3 tables of chunks RC1 rows high (columns dfa dga
+        // 1/sigma, rows dfv dgv thetaA).  Either chunk is drained in RC-row parts.
+        const int rc = two_sided ? RC2 : RC1;
         float *const rarr1 = jarr + 3 * JW1;
         for (c0 = i_beg; c0 < i_end; c0 += rc) {
             const int nrows = (int)(((c0 + rc < i_end) ? c0 + rc : i_end) - c0);
@@ -998,7 +1000,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                     return (0.5f * (hi.y - t * t) - 0x1p-20f * (hi.y + t * t)) * hi.w - E;
                 };
                 if (two_sided) {
-                    constexpr int NRL = RC / 64, NCL = (JW + 63) / 64;
+                    constexpr int NRL = RC2 / 64, NCL = (JW2 + 63) / 64;
                     vf4 rlo[NRL], rhi[NRL], clo[NCL], chi[NCL];
 #pragma unroll
                     for (int k = 0; k < NRL; ++k) {
@@ -1017,24 +1019,24 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                     for (int k = 0; k < NRL; ++k) {
                         const int x = lane + 64 * k;
                         const bool ok = x < nrows;
-                        rarr[0 * RC + x] = ok ? rlo[k].x : 0.f;        // dfa
-                        rarr[1 * RC + x] = ok ? rlo[k].y : 0.f;        // dga
-                        rarr[2 * RC + x] = ok ? rlo[k].w : 0.f;        // dfv
-                        rarr[3 * RC + x] = ok ? rhi[k].x : 0.f;        // dgv
-                        rarr[4 * RC + x] = ok ? rlo[k].z : 0.f;        // 1/sigma
-                        rarr[5 * RC + x] = ok ? theta(rlo[k], rhi[k], th.T, EA) : 0.f;
+                        rarr[0 * RC2 + x] = ok ? rlo[k].x : 0.f;        // dfa
+                        rarr[1 * RC2 + x] = ok ? rlo[k].y : 0.f;        // dga
+                        rarr[2 * RC2 + x] = ok ? rlo[k].w : 0.f;        // dfv
+                        rarr[3 * RC2 + x] = ok ? rhi[k].x : 0.f;        // dgv
+                        rarr[4 * RC2 + x] = ok ? rlo[k].z : 0.f;        // 1/sigma
+                        rarr[5 * RC2 + x] = ok ? theta(rlo[k], rhi[k], th.T, EA) : 0.f;
                     }
 #pragma unroll
                     for (int k = 0; k < NCL; ++k) {
                         const int x = lane + 64 * k;
-                        if (x < JW) {
+                        if (x < JW2) {
                             const bool ok = c0 + d0 + x <= nw - 1;
-                            jarr[0 * JW + x] = ok ? clo[k].x : 0.f;
-                            jarr[1 * JW + x] = ok ? clo[k].y : 0.f;
-                            jarr[2 * JW + x] = ok ? clo[k].w : 0.f;
-                            jarr[3 * JW + x] = ok ? chi[k].x : 0.f;
-                            jarr[4 * JW + x] = ok ? clo[k].z : 0.f;
-                            jarr[5 * JW + x] = ok ? theta(clo[k], chi[k], th.T, EB) : 0.f;
+                            jarr[0 * JW2 + x] = ok ? clo[k].x : 0.f;
+                            jarr[1 * JW2 + x] = ok ? clo[k].y : 0.f;
+                            jarr[2 * JW2 + x] = ok ? clo[k].w : 0.f;
+                            jarr[3 * JW2 + x] = ok ? chi[k].x : 0.f;
+                            jarr[4 * JW2 + x] = ok ? clo[k].z : 0.f;
+                            jarr[5 * JW2 + x] = ok ? theta(clo[k], chi[k], th.T, EB) : 0.f;
                         }
                     }
                 } else {
@@ -1080,10 +1082,10 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 // x = r0 + 4 (lane + 1), read from LDS (LDS pipe, no VALU shifts).
                 // Table r (dfa dga dfv dgv 1/sigma theta) of the mode's layout.
                 auto colp = [&](auto two_tag, int r) __attribute__((always_inline)) {
-                    return decltype(two_tag)::value ? jarr + r * JW : jarr + (r == 4 ? 2 : r) * JW1;
+                    return decltype(two_tag)::value ? jarr + r * JW2 : jarr + (r == 4 ? 2 : r) * JW1;
                 };
                 auto rowp = [&](auto two_tag, int r) __attribute__((always_inline)) {
-                    return decltype(two_tag)::value ? rarr + r * RC : rarr1 + (r == 5 ? 2 : r - 2) * RC1;
+                    return decltype(two_tag)::value ? rarr + r * RC2 : rarr1 + (r == 5 ? 2 : r - 2) * RC1;
                 };
                 float cs[NJARR][KD];
                 // One 8-row group: rows rl..rl+7 of the chunk, in two halves of 4 rows whose
