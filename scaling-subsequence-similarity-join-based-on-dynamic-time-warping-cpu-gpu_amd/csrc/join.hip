@@ -33,7 +33,8 @@
 //     pair, so lanes stay busy whatever the abandon points.
 //  4. banded DTW on the wavefront (dtw_device.hpp), early-abandoned against the best-so-far.
 // Ball survivors are appended to a per-wave queue in global memory, one 4-byte entry per lane
-// and row (row | lane | 4-bit mask of the lane's surviving diagonals).  The queue is drained
+// and row (row | lane | 4-bit complement of the mask of the lane's surviving diagonals: the
+// sign bits of the four test values).  The queue is drained
 // completely at the end of every 64-row chunk (earlier only if it could overflow): the chunk's
 // series / envelope windows and window statistics are then staged in LDS over the sweep's
 // column tables, the entries are expanded into a pair list, and stages 2-4 read LDS, not
@@ -116,6 +117,8 @@ __device__ inline Thr threshold_of(unsigned long long key, const JoinArgs &A) {
 __device__ inline Thr load_threshold(const JoinArgs &A) { return threshold_of(load_bsf(A), A); }
 
 __device__ inline float sq(float x) { return x * x; }
+// x, with +0 replaced by -0 (equal as numbers; only the sign of a zero changes)
+__device__ inline float no_pos_zero(float x) { return x == 0.0f ? -0.0f : x; }
 
 __device__ inline void decode_item(const int64_t *item_start, int groups, int64_t item, int &g, int64_t &s) {
     int lo = 0, hi = groups - 1;  // g with item_start[g] <= item < item_start[g+1]
@@ -726,18 +729,18 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
         const int row0 = (int)(cb - i_beg);  // part start relative to the item start
         // the next batch's entries and best-so-far word are loaded one batch ahead, so that
         // their latency overlaps the current batch (an older word is only larger: still valid)
-        uint32_t e_next = h0 + lane < h1 ? q1[h0 + lane] : 0u;
+        uint32_t e_next = h0 + lane < h1 ? q1[h0 + lane] : 15u;
         unsigned long long key_next = load_bsf(A);
         // Batches of 64 entries; one extra pass after the last batch flushes the LB_Keogh queue,
         // so that process_q2 (and the DTW inside it) is inlined at one place.
         for (int h = h0;; h += 64) {
             const bool last = h >= h1;
-            // expand up to 64 entries (row | lane | 4-bit mask) into pairs
-            const uint32_t e = last ? 0u : e_next;
+            // expand up to 64 entries (row | lane | complemented 4-bit mask; 15 = no entry) into pairs
+            const uint32_t e = last ? 15u : e_next;
             const unsigned long long key = key_next;
-            e_next = h + 64 + lane < h1 ? q1[h + 64 + lane] : 0u;
+            e_next = h + 64 + lane < h1 ? q1[h + 64 + lane] : 15u;
             if (!last) key_next = load_bsf(A);
-            const uint32_t mask = e & 15u;
+            const uint32_t mask = ~e & 15u;
             const int li = (int)(e >> 10) - row0;
             const int lb = li + KD * (int)((e >> 4) & 63u);
             // pairs listed diagonal-bit-major: the pairs of bit q follow those of bits < q, in
@@ -994,10 +997,13 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 const global_ptr<vf4> win = (global_ptr<vf4>)A.win;
                 static_assert(sizeof(WinRow) == 32 && offsetof(WinRow, isig) == 8 && offsetof(WinRow, dfv) == 12,
                               "WinRow is read as two 16-byte vectors");
+                // A staged threshold is never +0 (+0 is stored as -0, the same number): the test
+                // value fma(X, 1/sigma, -theta) is then never -0, so its sign bit alone says
+                // whether the pair is below the threshold (the queue mask is built from sign bits).
                 auto theta = [&](const vf4 &lo, const vf4 &hi, float T, float E) __attribute__((always_inline)) {
                     // (mN, R, sig) = hi.y, hi.z, hi.w
                     const float t = T + hi.z;
-                    return (0.5f * (hi.y - t * t) - 0x1p-20f * (hi.y + t * t)) * hi.w - E;
+                    return no_pos_zero((0.5f * (hi.y - t * t) - 0x1p-20f * (hi.y + t * t)) * hi.w - E);
                 };
                 if (two_sided) {
                     constexpr int NRL = RC2 / 64, NCL = (JW2 + 63) / 64;
@@ -1024,7 +1030,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                         rarr[2 * RC2 + x] = ok ? rlo[k].w : 0.f;        // dfv
                         rarr[3 * RC2 + x] = ok ? rhi[k].x : 0.f;        // dgv
                         rarr[4 * RC2 + x] = ok ? rlo[k].z : 0.f;        // 1/sigma
-                        rarr[5 * RC2 + x] = ok ? theta(rlo[k], rhi[k], th.T, EA) : 0.f;
+                        rarr[5 * RC2 + x] = ok ? theta(rlo[k], rhi[k], th.T, EA) : -0.f;
                     }
 #pragma unroll
                     for (int k = 0; k < NCL; ++k) {
@@ -1036,7 +1042,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                             jarr[2 * JW2 + x] = ok ? clo[k].w : 0.f;
                             jarr[3 * JW2 + x] = ok ? chi[k].x : 0.f;
                             jarr[4 * JW2 + x] = ok ? clo[k].z : 0.f;
-                            jarr[5 * JW2 + x] = ok ? theta(clo[k], chi[k], th.T, EB) : 0.f;
+                            jarr[5 * JW2 + x] = ok ? theta(clo[k], chi[k], th.T, EB) : -0.f;
                         }
                     }
                 } else {
@@ -1060,7 +1066,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                         const bool ok = x < nrows;
                         rarr1[0 * RC1 + x] = ok ? rlo[k].w : 0.f;  // dfv
                         rarr1[1 * RC1 + x] = ok ? rhi[k].x : 0.f;  // dgv
-                        rarr1[2 * RC1 + x] = ok ? theta(rlo[k], rhi[k], th.T, EA) : 0.f;
+                        rarr1[2 * RC1 + x] = ok ? theta(rlo[k], rhi[k], th.T, EA) : -0.f;
                     }
 #pragma unroll
                     for (int k = 0; k < NCL; ++k) {
@@ -1094,7 +1100,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 // pair is inside the triangle.  Test per pair: X^A / sigma_j >= thetaA_i and
                 // (TWO) X^B / sigma_i >= thetaB_j as signs of single-rounding FMAs; a lane-wide
                 // min/max reduces the four pairs to one compare per row, and a queue entry
-                // (row | lane | mask of the lane's surviving diagonals) is only formed on rows
+                // (row | lane | sign bits of the lane's four test values) is only formed on rows
                 // with a survivor.  Without TWO only X^A and its three tables are carried.
                 auto group = [&](auto full_tag, auto two_tag) __attribute__((always_inline)) {
                     constexpr bool FULL = decltype(full_tag)::value;
@@ -1104,6 +1110,8 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                     auto colused = [](int r) { return TWO || r == 0 || r == 1 || r == 4; };
                     auto rowused = [](int r) { return TWO || r == 2 || r == 3 || r == 5; };
                     (void)full_tag;
+                    // entry of this lane for the group's first row (rows add their offset << 10)
+                    const uint32_t ebase = code0 + ((uint32_t)rl << 10) + ((uint32_t)lane << 4);
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
                         const int r0 = rl + 4 * h;
@@ -1136,13 +1144,16 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                             const unsigned long long mk = __builtin_amdgcn_ballot_w64(any);
                             if (__builtin_expect(mk != 0ull, 0)) {
                                 if (any) {
-                                    const uint32_t bits = (t[u][0] >= 0.0f ? 1u : 0u) | (t[u][1] >= 0.0f ? 2u : 0u) |
-                                                          (t[u][2] >= 0.0f ? 4u : 0u) | (t[u][3] >= 0.0f ? 8u : 0u);
-                                    q1[q1n + (int)mbcnt64(mk)] =
-                                        code0 + ((uint32_t)(r0 + u) << 10) + ((uint32_t)lane << 4) + bits;
+                                    // the entry stores the complement of the survivor mask: bit q
+                                    // is the sign bit of t (t is never -0, so a clear sign bit
+                                    // means t >= 0; a NaN test value can only add a pair)
+                                    const uint32_t neg = (__float_as_uint(t[u][0]) >> 31) |
+                                                         ((__float_as_uint(t[u][1]) >> 30) & 2u) |
+                                                         ((__float_as_uint(t[u][2]) >> 29) & 4u) |
+                                                         ((__float_as_uint(t[u][3]) >> 28) & 8u);
+                                    q1[(uint32_t)q1n + mbcnt64(mk)] = ebase + ((uint32_t)(4 * h + u) << 10) + neg;
                                 }
                                 q1n += __popcll(mk);
-                                item_entries += __popcll(mk);
                             }
                         };
                         // one-sided full halves test all 16 pairs with one ballot (survivors are
@@ -1235,6 +1246,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 else
                     sweep(F_());
                 // single drain point (end of chunk, or before the queue could overflow)
+                item_entries += q1n;
                 if (q1n > 0) drain();
                 __syncthreads();
             }
