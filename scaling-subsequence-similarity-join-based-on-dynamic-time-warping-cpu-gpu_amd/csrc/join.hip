@@ -1154,6 +1154,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                                     q1[(uint32_t)q1n + mbcnt64(mk)] = ebase + ((uint32_t)(4 * h + u) << 10) + neg;
                                 }
                                 q1n += __popcll(mk);
+                                item_entries += __popcll(mk);
                             }
                         };
                         // one-sided full halves test all 16 pairs with one ballot (survivors are
@@ -1246,7 +1247,6 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 else
                     sweep(F_());
                 // single drain point (end of chunk, or before the queue could overflow)
-                item_entries += q1n;
                 if (q1n > 0) drain();
                 __syncthreads();
             }
