@@ -9,6 +9,9 @@ deterministic generator; its printed result lines are parsed and stored with the
     tests/golden/golden.json  per case: generator parameters, m, w and the reference outputs
                               (1-based first/second index, final distance, seed distance,
                               diag_fast / global prune fractions, its own timer).
+    tests/golden/grid.json    the reference CPU main's grid (CPU_version1/AAAA.cpp:30-38: band
+                              ratios 0.01, 0.02, 0.05, 0.1 outer, m = 32 .. 1024 inner,
+                              w = MAX(round(m r), 1)) on one series, one result per configuration.
 
 Run in the build container only (needs /root/reference): python tests/golden/make_golden.py
 """
@@ -46,6 +49,23 @@ CASES = [
     ("n8192_noplant_m128_w6", 8192, 21, dict(), 128, 6),
     ("n4096_m64_w2", 4096, 22, dict(n_plants=1, plant_len=64), 64, 2),
 ]
+
+# the reference CPU main's configuration grid (CPU_version1/AAAA.cpp:30-38)
+GRID_M = [32, 64, 128, 256, 512, 1024]
+GRID_R = [0.01, 0.02, 0.05, 0.1]
+GRID_CASE = ("grid_n4096", 4096, 23, dict(n_plants=1, plant_len=128))
+
+
+def grid_configs():
+    """(m, r, w) in the reference's order, w = MAX(round(m r), 1) (round half away from zero)."""
+    import math
+    out = []
+    for r in GRID_R:
+        for m in GRID_M:
+            x = m * r
+            w = max(int(math.floor(x + 0.5)) if x >= 0 else -int(math.floor(-x + 0.5)), 1)
+            out.append((m, r, w))
+    return out
 
 
 def load_pkg():
@@ -96,5 +116,32 @@ def main(names=None):
         json.dump(db, open(path, "w"), indent=1, sort_keys=True)
 
 
+def main_grid():
+    """Runs the reference once per grid configuration on one series; writes grid.json."""
+    if not os.path.exists(REF):
+        sys.exit("reference binary missing: run bash oracle/build_ref.sh first")
+    pkg = load_pkg()
+    name, n, seed, kw = GRID_CASE
+    series, src, dst = pkg.generate(n, seed, **kw)
+    results = []
+    with tempfile.TemporaryDirectory() as td:
+        txt = os.path.join(td, "ts.txt")
+        np.savetxt(txt, series, fmt="%.17g")
+        assert np.array_equal(np.loadtxt(txt), series)
+        for m, r, w in grid_configs():
+            out = subprocess.run([REF, txt, os.path.join(td, "time.txt"), str(m), str(w)], capture_output=True,
+                                 text=True, check=True).stdout
+            res = parse(out)
+            results.append(dict(m=m, r=r, w=w, ref=res))
+            print(m, r, w, res, flush=True)
+    np.save(os.path.join(HERE, name + ".npy"), series)
+    gen = {k: (list(map(int, v)) if isinstance(v, (list, tuple)) else v) for k, v in kw.items()}
+    json.dump(dict(name=name, n=n, seed=seed, gen=gen, configs=results), open(os.path.join(HERE, "grid.json"), "w"),
+              indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
-    main(sys.argv[1:] or None)
+    if sys.argv[1:] == ["--grid"]:
+        main_grid()
+    else:
+        main(sys.argv[1:] or None)

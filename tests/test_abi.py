@@ -100,3 +100,22 @@ def test_cli_reports_errors(dtwj, tmp_path):
     assert r.returncode != 0 and "Error opening file" in r.stderr
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 2
+
+
+def test_cli_grid_order_without_a_device(dtwj, tmp_path):
+    """--grid runs the reference CPU main's 24 configurations (CPU_version1/AAAA.cpp:30-38) in its
+    order; without a HIP device every join reports an error and the exit status is non-zero."""
+    if dtwj.device_count() > 0:
+        pytest.skip("a HIP device is visible: the GPU test covers the grid")
+    data = tmp_path / "ts.txt"
+    np.savetxt(data, np.cumsum(np.random.default_rng(5).standard_normal(3000)), fmt="%.17g")
+    r = subprocess.run([dtwj.CLI_PATH, str(data), str(tmp_path / "t.txt"), "--grid"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 5
+    got = re.findall(r"subsequence length m=(\d+), band radius w=(\d+), n=3000", r.stdout)
+    want = [(m, max(int(np.floor(m * q + 0.5)), 1)) for q in (0.01, 0.02, 0.05, 0.1)
+            for m in (32, 64, 128, 256, 512, 1024)]
+    assert [(int(m), int(w)) for m, w in got] == want
+    assert r.stderr.count("error ") == 24
+    r = subprocess.run([dtwj.CLI_PATH, str(data), "--grid"], capture_output=True, text=True)
+    assert r.returncode == 2

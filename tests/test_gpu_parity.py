@@ -5,13 +5,15 @@ Tolerances: integer outputs (indices) bit-exact; FP64 stages (statistics, verifi
 relative 1e-9; FP32 DTW cells relative 1e-4 of the cost (the FP32-vs-FP64 error is reported);
 required bar for the final distance: relative 1e-5.
 """
+import json
 import os
+import re
 import subprocess
 
 import numpy as np
 import pytest
 
-from conftest import golden_cases, golden_series
+from conftest import GOLDEN_DIR, golden_cases, golden_series
 
 pytestmark = pytest.mark.gpu
 CASES = golden_cases()
@@ -168,6 +170,35 @@ def test_cli_drop_in(gpu, tmp_path):
     assert "final distance :%f" % ref["final_distance"] in out.stdout
     lines = tfile.read_text().split()
     assert len(lines) == 2 and all(float(x) > 0 for x in lines)
+
+
+def test_cli_grid_matches_reference_grid(gpu, oracle, tmp_path):
+    """gpu_dtw --grid against the compiled reference's 24-configuration grid
+    (CPU_version1/AAAA.cpp:30-38) on one series (tests/golden/grid.json).  Where the reference's
+    sweep found the pair, the printed pair is its pair; where it kept its seed, the reference
+    prints the last examined seed's windows (CPU_version1/new_dtw_motifGUI.cpp:75-76), and
+    gpu_dtw prints the true pair, which the oracle's pair pins."""
+    with open(os.path.join(GOLDEN_DIR, "grid.json")) as f:
+        g = json.load(f)
+    a = golden_series(g["name"])
+    data = tmp_path / "ts.txt"
+    np.savetxt(data, a, fmt="%.17g")
+    tfile = tmp_path / "time.txt"
+    out = subprocess.run([gpu.CLI_PATH, str(data), str(tfile), "--grid"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    blocks = re.split(r"subsequence length m=", out.stdout)[1:]
+    assert len(blocks) == len(g["configs"]) == 24
+    for blk, c in zip(blocks, g["configs"]):
+        ref = c["ref"]
+        assert blk.startswith("%d, band radius w=%d, n=%d" % (c["m"], c["w"], g["n"])), blk[:60]
+        o = oracle.motif_discovery(a, c["m"], c["w"])
+        assert (o.first_min, o.sec_min) == (int(ref["first_min"]), int(ref["sec_min"])), c
+        assert "first min is %d sec min is %d" % (o.i + 1, o.j + 1) in blk, (c, blk)
+        if o.found_in_sweep:
+            assert (o.i + 1, o.j + 1) == (int(ref["first_min"]), int(ref["sec_min"])), c
+        assert "final distance :%f" % ref["final_distance"] in blk, (c, blk)
+    lines = tfile.read_text().split()
+    assert len(lines) == 24 and all(float(x) > 0 for x in lines)
 
 
 def test_python_mirror_prints_reference_lines(gpu, capfd, tmp_path):

@@ -37,3 +37,36 @@ def test_golden_pairs_are_admissible():
         ref = c["ref"]
         i, j = int(ref["first_min"]), int(ref["sec_min"])
         assert j > i and j - i >= c["m"]
+
+
+def grid_fixture():
+    import json
+    import os
+    from conftest import GOLDEN_DIR
+    with open(os.path.join(GOLDEN_DIR, "grid.json")) as f:
+        return json.load(f)
+
+
+def test_grid_fixture_follows_the_reference_grid():
+    # CPU_version1/AAAA.cpp:30-38: ratios outer, lengths inner, w = MAX(round(m r), 1)
+    g = grid_fixture()
+    cfg = [(c["m"], c["r"], c["w"]) for c in g["configs"]]
+    want = [(m, r, max(int(math.floor(m * r + 0.5)), 1)) for r in (0.01, 0.02, 0.05, 0.1)
+            for m in (32, 64, 128, 256, 512, 1024)]
+    assert cfg == want
+    for c in g["configs"]:
+        i, j = int(c["ref"]["first_min"]), int(c["ref"]["sec_min"])
+        assert j > i and j - i >= c["m"]
+
+
+def test_oracle_matches_reference_grid(oracle):
+    """The reference CPU main's 24-configuration grid on one series (tests/golden/grid.json)."""
+    g = grid_fixture()
+    a = golden_series(g["name"])
+    assert len(a) == g["n"]
+    for c in g["configs"]:
+        r = oracle.motif_discovery(a, c["m"], c["w"])
+        ref = c["ref"]
+        assert (r.first_min, r.sec_min) == (int(ref["first_min"]), int(ref["sec_min"])), c
+        assert abs(r.bsf - ref["final_distance"]) <= 5e-7, c
+        assert abs(r.seed_bsf - ref["input_bsf"]) <= 5e-4, c
