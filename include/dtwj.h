@@ -37,7 +37,8 @@ extern "C" {
 #define DTWJ_ENOMEM 2    /* host or device allocation failed */
 #define DTWJ_EDEVICE 3   /* no usable HIP device, or a HIP call failed */
 #define DTWJ_EIO 4       /* file could not be opened / parsed / written */
-#define DTWJ_EOVERFLOW 5 /* an internal candidate buffer overflowed */
+#define DTWJ_EOVERFLOW 5 /* more than 2^31 - 1 near-tie candidates (the candidate buffer is
+                            regrown and the join re-run below that) */
 
 /* ---- options and results --------------------------------------------------------- */
 typedef struct dtwj_options {
@@ -53,7 +54,10 @@ typedef struct dtwj_options {
                               on a sample of the batch (0 = use the fields above or defaults) */
     int32_t sweep;         /* diagonal sweep test: 0 = adaptive per work item (default), 1 = always
                               the two-sided ball bound, 2 = always the one-sided one (both exact) */
-    int32_t reserved[4];
+    int32_t cand_cap;      /* near-tie candidate slots of a single-series join (0 = 2^20; a batch
+                              keeps 2^14 per series).  A join that pushes more is re-run with a
+                              buffer sized to its count, so this only trades memory for re-runs */
+    int32_t reserved[3];
 } dtwj_options;
 
 typedef struct dtwj_result {
@@ -78,17 +82,33 @@ typedef struct dtwj_result {
     double t_join;       /* fused sweep + cascade + DTW */
     double t_verify;     /* FP64 re-verification */
     double join_kernel_ms; /* GPU time of the fused join kernel (HIP events) */
-    int32_t items;       /* sweep work items */
+    int32_t items;       /* sweep work items (saturated at 2^31 - 1; see items_total) */
     int32_t rows_per_item; /* sweep tile height used */
     int32_t waves_per_cu;  /* persistent waves per CU used */
-    int32_t reserved0;
+    int32_t reruns;      /* joins re-run because more near-tie candidates were pushed than the
+                            candidate buffer held (each re-run starts from the previous run's
+                            best-so-far with a buffer sized to the count; 0 normally) */
     int64_t pairs_swept_one_sided; /* of pairs_swept: tested with the one-sided ball bound */
-    int32_t reserved[2];
+    int64_t items_total; /* sweep work items (64-bit) */
 } dtwj_result;
 
 /* Fills *opt with defaults.  No reference counterpart (the reference hard-codes
  * its tuning in macros, GPU_version1/alldef/GPU_parameters.h:8-18). */
 void dtwj_default_options(dtwj_options *opt);
+
+/* Work plan of a single-series top-1 join, computed on the host without touching a device:
+ * the sweep's work items (claimed on the device from a 64-bit counter), tile height, diagonal
+ * groups and the device bytes the join's workspace takes.  No reference counterpart (the
+ * reference tiles with fixed macros, GPU_version1/alldef/GPU_parameters.h:8-18). */
+typedef struct dtwj_plan_info {
+    int64_t pairs_total;   /* K(K+1)/2, K = n - 2m + 1 */
+    int64_t items;         /* sweep work items */
+    int32_t rows_per_item; /* tile height */
+    int32_t diag_groups;   /* groups of 256 diagonal offsets */
+    int64_t device_bytes;  /* workspace bytes for a device of 256 compute units */
+    int64_t reserved[4];
+} dtwj_plan_info;
+int dtwj_plan(int64_t n, int32_t m, int32_t w, const dtwj_options *opt, dtwj_plan_info *out);
 
 /* Band radius from a ratio, the reference rule w = MAX(round(m*r), 1)
  * (GPU_version1/src/AAAA.cu:34; CPU_version1/AAAA.cpp:36). */

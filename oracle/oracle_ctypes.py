@@ -11,7 +11,6 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
-REF_BIN = os.path.join(HERE, "_ref", "ref_dtw")
 
 
 class JoinResult(ctypes.Structure):

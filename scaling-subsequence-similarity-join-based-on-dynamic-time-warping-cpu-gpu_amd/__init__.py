@@ -37,7 +37,8 @@ class DTWJError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("seed_items", ctypes.c_int32), ("rows_per_item", ctypes.c_int32),
                 ("waves_per_cu", ctypes.c_int32), ("verbose", ctypes.c_int32), ("graph", ctypes.c_int32),
-                ("autotune", ctypes.c_int32), ("sweep", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+                ("autotune", ctypes.c_int32), ("sweep", ctypes.c_int32), ("cand_cap", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
 
 
 class Result(ctypes.Structure):
@@ -49,8 +50,13 @@ class Result(ctypes.Structure):
                 ("t_total", ctypes.c_double), ("t_prep", ctypes.c_double), ("t_seed", ctypes.c_double),
                 ("t_join", ctypes.c_double), ("t_verify", ctypes.c_double), ("join_kernel_ms", ctypes.c_double),
                 ("items", ctypes.c_int32), ("rows_per_item", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
-                ("reserved0", ctypes.c_int32), ("pairs_swept_one_sided", ctypes.c_int64),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("reruns", ctypes.c_int32), ("pairs_swept_one_sided", ctypes.c_int64),
+                ("items_total", ctypes.c_int64)]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("pairs_total", ctypes.c_int64), ("items", ctypes.c_int64), ("rows_per_item", ctypes.c_int32),
+                ("diag_groups", ctypes.c_int32), ("device_bytes", ctypes.c_int64), ("reserved", ctypes.c_int64 * 4)]
 
 
 MLP_IN, MLP_HID = 40, 64
@@ -105,7 +111,7 @@ EXPORTED = [
     "dtwj_free", "dtwj_last_error", "dtwj_device_count", "dtwj_sliding_stats", "dtwj_envelope", "dtwj_dtw_pairs",
     "dtwj_gen_spec_default", "dtwj_generate", "dtwj_top1_batch",
     "dtwj_context_top1_batch", "dtwj_proposal_candidates", "dtwj_mlp_forward", "dtwj_mlp_gradient",
-    "dtwj_mlp_train", "dtwj_top1_learned",
+    "dtwj_mlp_train", "dtwj_top1_learned", "dtwj_plan",
 ]
 
 _lib = None
@@ -153,6 +159,7 @@ def lib():
                                  ctypes.c_uint64, d_p]
     L.dtwj_top1_learned.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, P(MLP), ctypes.c_int32,
                                     ctypes.c_int32, P(Options), P(Result), P(ProposalReport)]
+    L.dtwj_plan.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, P(Options), P(PlanInfo)]
     L.dtwj_gen_spec_default.argtypes = [P(GenSpec)]
     L.dtwj_generate.argtypes = [P(GenSpec), d_p, i64_p, i64_p]
     _lib = L
@@ -202,6 +209,8 @@ class Top1:
     rows_per_item: int = 0
     waves_per_cu: int = 0
     pairs_swept_one_sided: int = 0
+    reruns: int = 0
+    items_total: int = 0
     extra: dict = field(default_factory=dict)
 
     @property
@@ -219,12 +228,12 @@ def _result(r):
 
 
 def make_options(device=0, seed_items=0, rows_per_item=0, waves_per_cu=0, verbose=False, graph=0, autotune=0,
-                 sweep=0):
+                 sweep=0, cand_cap=0):
     o = Options()
     lib().dtwj_default_options(ctypes.byref(o))
     o.device, o.seed_items, o.rows_per_item, o.waves_per_cu, o.verbose = (
         device, seed_items, rows_per_item, waves_per_cu, int(bool(verbose)))
-    o.graph, o.autotune, o.sweep = int(graph), int(autotune), int(sweep)
+    o.graph, o.autotune, o.sweep, o.cand_cap = int(graph), int(autotune), int(sweep), int(cand_cap)
     return o
 
 
@@ -235,6 +244,13 @@ def device_count():
 def band_from_ratio(m, r):
     """w = MAX(round(m*r), 1) as the reference CLI computes it (GPU_version1/src/AAAA.cu:34)."""
     return int(lib().dtwj_band_from_ratio(int(m), float(r)))
+
+
+def plan(n, m, w, **opts):
+    """Host-side work plan of a top-1 join (dtwj_plan): items, tile height, device bytes."""
+    out = PlanInfo()
+    _check(lib().dtwj_plan(int(n), int(m), int(w), ctypes.byref(make_options(**opts)), ctypes.byref(out)))
+    return out
 
 
 def top1(series, m, w, **opts):

@@ -138,9 +138,10 @@ struct JoinArgs {
     int64_t n_items;
     // shared best-so-far and results
     unsigned long long *bsf;      // packed
+    unsigned long long init_key;  // best-so-far word the set-up kernel starts from (a valid bound)
     int4 *cands;                  // (i, j, cost bits, 0)
-    int32_t *n_cands;
-    int32_t cand_cap;
+    unsigned long long *n_cands;  // pushes so far (64-bit: counts past cand_cap without wrapping)
+    int64_t cand_cap;
     unsigned long long *counters; // per-stage counters (see dtwj_api.cpp)
     // error model, filled on the device from gstat by the set-up kernel
     float amax;        // max |ac|, |uc|, |lc|

@@ -59,7 +59,135 @@ int launch_pairs_t(const JoinArgs &A, const double *sig64, const int64_t *pi, co
     return 0;
 }
 
+// ---- FP64 re-scoring of a join's candidate list.  One wave per candidate: candidates whose
+// FP32 cost exceeds ub (only a near-tie of the best can be the FP64 winner) get +inf, the
+// others their FP64 cost.  Duplicates (a pair pushed by the seed and by the join) need no
+// removal: the minimum below is unchanged by them.
+template <int QP>
+__global__ __launch_bounds__(64) void k_rescore(JoinArgs A, const double *__restrict__ sig64,
+                                                const int4 *__restrict__ cands, int64_t n, float ub,
+                                                double *cost64) {
+    extern __shared__ unsigned char smem_raw[];
+    double *zi = reinterpret_cast<double *>(smem_raw);
+    double *zj = zi + A.m;
+    const int64_t p = blockIdx.x;
+    if (p >= n) return;
+    const int4 c = cands[p];
+    const int lane = threadIdx.x;
+    if (!(__int_as_float(c.z) <= ub)) {
+        if (lane == 0) cost64[p] = __builtin_huge_val();
+        return;
+    }
+    const int64_t i = c.x, j = c.y;
+    const double mi = A.mu[i], si = sig64[i], mj = A.mu[j], sj = sig64[j];
+    for (int k = lane; k < A.m; k += 64) {
+        zi[k] = (A.a[i + k] - mi) / si;
+        zj[k] = (A.a[j + k] - mj) / sj;
+    }
+    __syncthreads();
+    const PlainZ<double> z{zi, zj};
+    int t_stop = 0;
+    const double r = dtw_band<double, QP>(A.m, A.w, 64, 1, dev_inf<double>(), z, t_stop);
+    if (lane == 0) cost64[p] = r;
+}
+
+// (cost, i, j, FP32 cost) in lexicographic order: the FP64 winner with the smallest i, then j.
+__device__ inline bool winner_less(const Winner &a, const Winner &b) {
+    if (a.cost != b.cost) return a.cost < b.cost;
+    if (a.i != b.i) return a.i < b.i;
+    if (a.j != b.j) return a.j < b.j;
+    return a.cost32 < b.cost32;
+}
+
+// Per-block minimum over a strided slice of the re-scored candidates (and the count of
+// candidates that were re-scored), then one block over the partials.
+constexpr int kArgminThreads = 256;
+constexpr int kArgminBlocks = 1024;
+
+__device__ inline void block_min(Winner &w) {
+    __shared__ Winner sh[kArgminThreads];
+    const int t = threadIdx.x;
+    sh[t] = w;
+    __syncthreads();
+    for (int off = kArgminThreads / 2; off > 0; off >>= 1) {
+        if (t < off) {
+            Winner o = sh[t + off];
+            Winner &me = sh[t];
+            const unsigned long long nv = me.verified + o.verified;
+            if (winner_less(o, me)) me = o;
+            me.verified = nv;
+        }
+        __syncthreads();
+    }
+    w = sh[0];
+}
+
+__global__ __launch_bounds__(kArgminThreads) void k_argmin_part(const int4 *__restrict__ cands,
+                                                                const double *__restrict__ cost64, int64_t n,
+                                                                Winner *part) {
+    Winner w{__builtin_huge_val(), 0x7fffffff, 0x7fffffff, __builtin_huge_valf(), 0u, 0ull};
+    for (int64_t p = (int64_t)blockIdx.x * kArgminThreads + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * kArgminThreads) {
+        const double c = cost64[p];
+        if (c < __builtin_huge_val()) {
+            const int4 e = cands[p];
+            const Winner x{c, e.x, e.y, __int_as_float(e.z), 0u, 0ull};
+            if (winner_less(x, w)) {
+                const unsigned long long nv = w.verified;
+                w = x;
+                w.verified = nv;
+            }
+            ++w.verified;
+        }
+    }
+    block_min(w);
+    if (threadIdx.x == 0) part[blockIdx.x] = w;
+}
+
+__global__ __launch_bounds__(kArgminThreads) void k_argmin_final(const Winner *__restrict__ part, int nparts,
+                                                                 Winner *out) {
+    Winner w{__builtin_huge_val(), 0x7fffffff, 0x7fffffff, __builtin_huge_valf(), 0u, 0ull};
+    for (int p = threadIdx.x; p < nparts; p += kArgminThreads) {
+        const Winner x = part[p];
+        const unsigned long long nv = w.verified + x.verified;
+        if (winner_less(x, w)) w = x;
+        w.verified = nv;
+    }
+    block_min(w);
+    if (threadIdx.x == 0) *out = w;
+}
+
 }  // namespace
+
+int rescore_partials() { return kArgminBlocks; }
+
+int launch_rescore(const JoinArgs &A, const double *sig64, const int4 *cands, int64_t n, float ub, double *cost64,
+                   Winner *part, Winner *out, hipStream_t s) {
+    if (n <= 0) return 0;
+    const int QP = dtw_qp_for(A.w);
+    const size_t lds = 2 * (size_t)A.m * sizeof(double);
+    // one block per candidate: the grid's x dimension holds up to 2^31 - 1 blocks
+    if (n > 0x7fffffffLL) {
+        set_error("more than 2^31 - 1 candidates to re-score");
+        return DTWJ_EOVERFLOW;
+    }
+    const dim3 grid((unsigned)n), block(64);
+    switch (QP) {
+        case 1: hipLaunchKernelGGL(k_rescore<1>, grid, block, lds, s, A, sig64, cands, n, ub, cost64); break;
+        case 2: hipLaunchKernelGGL(k_rescore<2>, grid, block, lds, s, A, sig64, cands, n, ub, cost64); break;
+        case 4: hipLaunchKernelGGL(k_rescore<4>, grid, block, lds, s, A, sig64, cands, n, ub, cost64); break;
+        case 8: hipLaunchKernelGGL(k_rescore<8>, grid, block, lds, s, A, sig64, cands, n, ub, cost64); break;
+        default: set_error("band too wide: 2w+1 must be <= 1024"); return DTWJ_EINVAL;
+    }
+    DTWJ_HIP(hipGetLastError());
+    const int64_t want = (n + kArgminThreads - 1) / kArgminThreads;
+    const int nb = (int)(want < kArgminBlocks ? want : kArgminBlocks);
+    hipLaunchKernelGGL(k_argmin_part, dim3(nb), dim3(kArgminThreads), 0, s, cands, cost64, n, part);
+    DTWJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_argmin_final, dim3(1), dim3(kArgminThreads), 0, s, part, nb, out);
+    DTWJ_HIP(hipGetLastError());
+    return 0;
+}
 
 int launch_dtw_pairs(const JoinArgs &A, const double *sig64, const int64_t *pi, const int64_t *pj, int64_t npairs,
                      int32_t prec, double *cost, hipStream_t s) {

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <numeric>
 #include <random>
@@ -36,8 +37,10 @@ namespace {
 using clk = std::chrono::steady_clock;
 inline double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
-constexpr int kCandCap = 1 << 20;       // candidate slots of a single-series join
-constexpr int kBatchCandCap = 1 << 14;  // ... per series of a batch
+constexpr int64_t kCandCap = 1 << 20;       // candidate slots of a single-series join
+constexpr int64_t kBatchCandCap = 1 << 14;  // ... per series of a batch
+constexpr int64_t kMaxCandCap = 0x7fffffffLL;  // largest regrown buffer (re-scoring grid limit)
+constexpr int kRegrow = -2;  // finish_job: more candidates were pushed than the buffer holds
 constexpr int kDiagGroup = 256;  // diagonals per work item (join.hip DG)
 constexpr int kNCounters = 8;
 
@@ -114,12 +117,27 @@ int dtwj_context_create(int32_t device, dtwj_context **out) {
     if (device < 0 || device >= count) return fail(DTWJ_EINVAL, "device ordinal out of range");
     dtwj_context *c = new dtwj_context();
     c->device = device;
-    DTWJ_HIP(hipSetDevice(device));
-    DTWJ_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    DTWJ_HIP(hipEventCreate(&c->ev0));
-    DTWJ_HIP(hipEventCreate(&c->ev1));
     hipDeviceProp_t prop;
-    DTWJ_HIP(hipGetDeviceProperties(&prop, device));
+    hipError_t e = hipSetDevice(device);
+    const char *what = "hipSetDevice";
+    if (e == hipSuccess) {
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        what = "hipStreamCreateWithFlags";
+    }
+    if (e == hipSuccess) {
+        e = hipEventCreate(&c->ev0);
+        what = "hipEventCreate";
+    }
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) {
+        e = hipGetDeviceProperties(&prop, device);
+        what = "hipGetDeviceProperties";
+    }
+    if (e != hipSuccess) {
+        const int rc = hip_fail(e, what, __FILE__, __LINE__);
+        dtwj_context_destroy(c);  // releases whatever was created
+        return rc;
+    }
     c->num_cus = prop.multiProcessorCount;
     *out = c;
     return DTWJ_OK;
@@ -172,7 +190,10 @@ struct Job {
     const double *d_a = nullptr;
     int64_t n = 0, nw = 0, K = 0;
     int32_t m = 0, w = 0;  // w already clamped to m - 1
-    int32_t groups = 0, rows_per_item = 0, n_seed = 0, cand_cap = 0;
+    int32_t groups = 0, rows_per_item = 0, n_seed = 0;
+    int64_t cand_cap = 0;
+    int64_t cand_need = 0;   // pushes counted by the last run when they exceeded cand_cap
+    unsigned long long init_key = 0;  // best-so-far the join starts from
     int32_t sweep_mode = 0;  // dtwj_options::sweep
     int64_t n_items = 0;
     std::vector<int64_t> item_start;
@@ -181,7 +202,7 @@ struct Job {
     JoinArgs A;               // host copy: pointers and sizes (error model filled on the device)
     unsigned long long *d_bsf = nullptr, *d_counters = nullptr;
     int4 *d_cands = nullptr;
-    int32_t *d_ncands = nullptr;
+    unsigned long long *d_ncands = nullptr;
     int64_t *d_item_start = nullptr;
     int2 *d_seed_pairs = nullptr;
     int32_t n_seed_pairs = 0;
@@ -197,7 +218,7 @@ int32_t normalise_rows(int32_t r, int32_t w) {
 }
 
 void plan_job(Job &J, const double *d_a, int64_t n, int32_t m, int32_t w_in, int32_t rows_per_item, int32_t seed_req,
-              bool gpu_scan, const std::vector<int2> &extra, int32_t cand_cap) {
+              bool gpu_scan, const std::vector<int2> &extra, int64_t cand_cap) {
     J.d_a = d_a;
     J.n = n;
     J.m = m;
@@ -221,6 +242,7 @@ void plan_job(Job &J, const double *d_a, int64_t n, int32_t m, int32_t w_in, int
     J.extra = extra;
     J.n_seed_pairs = ns + 1 + (int32_t)extra.size();
     J.cand_cap = cand_cap;
+    J.init_key = pack_bsf(3.0e38f, 0xffffffffu);
 }
 
 // Device bytes of one series (statistics, envelopes, tables, scans, join state).
@@ -246,7 +268,7 @@ size_t job_bytes(const Job &J) {
     add(8);               // bsf
     add(8 * (J.groups + 1));
     add(16 * (size_t)J.cand_cap);
-    add(4);               // n_cands
+    add(8);               // n_cands
     add(8 * kNCounters);
     add(8 * ((size_t)J.n_seed_pairs + 1));
     return b;
@@ -282,7 +304,7 @@ void take_job(Arena &ar, Job &J) {
     J.d_bsf = ar.take<unsigned long long>(1);
     J.d_item_start = ar.take<int64_t>(J.groups + 1);
     J.d_cands = ar.take<int4>(J.cand_cap);
-    J.d_ncands = ar.take<int32_t>(1);
+    J.d_ncands = ar.take<unsigned long long>(1);
     J.d_counters = ar.take<unsigned long long>(kNCounters);
     J.d_seed_pairs = ar.take<int2>((size_t)J.n_seed_pairs + 1);
 
@@ -311,6 +333,7 @@ void take_job(Arena &ar, Job &J) {
     A.rows_per_item = J.rows_per_item;
     A.n_items = J.n_items;
     A.bsf = J.d_bsf;
+    A.init_key = J.init_key;
     A.cands = J.d_cands;
     A.n_cands = J.d_ncands;
     A.cand_cap = J.cand_cap;
@@ -357,73 +380,39 @@ int enqueue_seeds(Job &J, const JoinArgs *d_args, hipStream_t s) {
     return launch_seed_dtw(d_args, J.m, J.w, J.d_seed_pairs, J.n_seed_pairs, s);
 }
 
-// Reads the job's join state back, re-scores every candidate within the FP32 error of the
-// best in FP64 and fills *out.  d_vi/d_vj/d_vcost: scratch for cand_cap pairs.
-int finish_job(Job &J, int64_t *d_vi, int64_t *d_vj, double *d_vcost, hipStream_t s, dtwj_result *out) {
-    int32_t ncands = 0;
-    unsigned long long key = 0, counters[kNCounters];
+// Reads the job's join state back, re-scores on the device every candidate within the FP32
+// error of the best in FP64 and fills *out.  d_cost64: scratch for cand_cap doubles; d_part /
+// d_win: rescore_partials() + 1 Winner slots.  Returns kRegrow (with J.cand_need set) when the
+// join pushed more candidates than its buffer holds: the caller re-runs it with a larger buffer.
+int finish_job(Job &J, double *d_cost64, Winner *d_part, Winner *d_win, hipStream_t s, dtwj_result *out) {
+    unsigned long long ncands = 0, key = 0, counters[kNCounters];
     DTWJ_HIP(hipMemcpyAsync(&ncands, J.d_ncands, sizeof(ncands), hipMemcpyDeviceToHost, s));
     DTWJ_HIP(hipMemcpyAsync(&key, J.d_bsf, sizeof(key), hipMemcpyDeviceToHost, s));
     DTWJ_HIP(hipMemcpyAsync(counters, J.d_counters, sizeof(counters), hipMemcpyDeviceToHost, s));
     DTWJ_HIP(hipStreamSynchronize(s));
-    if (ncands > J.cand_cap) return fail(DTWJ_EOVERFLOW, "candidate buffer overflow (too many near-ties)");
-    if (ncands <= 0) return fail(DTWJ_EDEVICE, "join produced no candidate");
-    std::vector<int4> cands(ncands);
-    DTWJ_HIP(hipMemcpyAsync(cands.data(), J.d_cands, sizeof(int4) * ncands, hipMemcpyDeviceToHost, s));
-    DTWJ_HIP(hipStreamSynchronize(s));
+    if (ncands > (unsigned long long)J.cand_cap) {
+        J.cand_need = (int64_t)std::min<unsigned long long>(ncands, (unsigned long long)INT64_MAX);
+        J.init_key = key;
+        return kRegrow;
+    }
+    if (ncands == 0) return fail(DTWJ_EDEVICE, "join produced no candidate");
     const float best32 = bsf_cost(key);
     const double eps = J.A.eps_z, rel = J.A.rel_dtw;
     const int m = J.m;
     auto err = [&](double D) { return rel * D + 4.0 * eps * std::sqrt(2.0 * m * D) + 8.0 * m * eps * eps; };
     const double T2 = (double)best32 + err(best32);
-    const double ub = T2 + err(T2);
-    std::vector<int64_t> vi, vj;
-    std::vector<float> v32;
-    for (const int4 &cd : cands) {
-        const float cost = __builtin_bit_cast(float, cd.z);
-        if ((double)cost <= ub) {
-            vi.push_back(cd.x);
-            vj.push_back(cd.y);
-            v32.push_back(cost);
-        }
-    }
-    // de-duplicate (a pair can be pushed by the seed and by the join)
-    {
-        std::vector<size_t> idx(vi.size());
-        for (size_t k = 0; k < idx.size(); ++k) idx[k] = k;
-        std::sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
-            return vi[x] != vi[y] ? vi[x] < vi[y] : vj[x] < vj[y];
-        });
-        std::vector<int64_t> ui, uj;
-        std::vector<float> u32;
-        for (size_t k = 0; k < idx.size(); ++k) {
-            if (k > 0 && vi[idx[k]] == vi[idx[k - 1]] && vj[idx[k]] == vj[idx[k - 1]]) continue;
-            ui.push_back(vi[idx[k]]);
-            uj.push_back(vj[idx[k]]);
-            u32.push_back(v32[idx[k]]);
-        }
-        vi.swap(ui);
-        vj.swap(uj);
-        v32.swap(u32);
-    }
-    const int64_t nv = (int64_t)vi.size();
-    std::vector<double> vcost(nv);
-    DTWJ_HIP(hipMemcpyAsync(d_vi, vi.data(), sizeof(int64_t) * nv, hipMemcpyHostToDevice, s));
-    DTWJ_HIP(hipMemcpyAsync(d_vj, vj.data(), sizeof(int64_t) * nv, hipMemcpyHostToDevice, s));
-    if (int rc = launch_dtw_pairs(J.A, J.P.sig, d_vi, d_vj, nv, 64, d_vcost, s)) return rc;
-    DTWJ_HIP(hipMemcpyAsync(vcost.data(), d_vcost, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+    const float ub = (float)(T2 + err(T2));  // rounded to nearest: the test below keeps a margin
+    const float ub_up = std::nextafter(ub, std::numeric_limits<float>::infinity());
+    if (int rc = launch_rescore(J.A, J.P.sig, J.d_cands, (int64_t)ncands, ub_up, d_cost64, d_part, d_win, s)) return rc;
+    Winner win;
+    DTWJ_HIP(hipMemcpyAsync(&win, d_win, sizeof(win), hipMemcpyDeviceToHost, s));
     DTWJ_HIP(hipStreamSynchronize(s));
-    int64_t best = -1;
-    for (int64_t k = 0; k < nv; ++k) {
-        if (best < 0 || vcost[k] < vcost[best] ||
-            (vcost[k] == vcost[best] && (vi[k] < vi[best] || (vi[k] == vi[best] && vj[k] < vj[best]))))
-            best = k;
-    }
+    if (win.verified == 0 || !std::isfinite(win.cost)) return fail(DTWJ_EDEVICE, "no candidate re-scored in FP64");
     std::memset(out, 0, sizeof(*out));
-    out->i = vi[best];
-    out->j = vj[best];
-    out->dist = std::sqrt(vcost[best]);
-    out->dist_fp32 = std::sqrt((double)v32[best]);
+    out->i = win.i;
+    out->j = win.j;
+    out->dist = std::sqrt(win.cost);
+    out->dist_fp32 = std::sqrt((double)win.cost32);
     out->pairs_total = J.K * (J.K + 1) / 2;
     out->pairs_swept = (int64_t)counters[0];
     out->pass_ball = (int64_t)counters[1];
@@ -433,10 +422,17 @@ int finish_job(Job &J, int64_t *d_vi, int64_t *d_vj, double *d_vcost, hipStream_
     out->dtw_cells = (int64_t)counters[5];
     out->keogh_elems = (int64_t)counters[6];
     out->pairs_swept_one_sided = (int64_t)counters[7];
-    out->candidates = nv;
-    out->items = (int32_t)J.n_items;
+    out->candidates = (int64_t)win.verified;
+    out->items = (int32_t)std::min<int64_t>(J.n_items, INT32_MAX);
+    out->items_total = J.n_items;
     out->rows_per_item = J.rows_per_item;
     return DTWJ_OK;
+}
+
+// Candidate slots for a re-run that saw `need` pushes: a quarter more, at least twice the old.
+int64_t regrown_cap(int64_t old_cap, int64_t need) {
+    int64_t c = std::max<int64_t>(2 * old_cap, need + need / 4 + 1024);
+    return std::min<int64_t>(c, kMaxCandCap);
 }
 
 // Copies the error model the set-up kernel computed into the host copy (used by finish_job).
@@ -663,86 +659,104 @@ struct Seeds {
 int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w_in, const dtwj_options *opt,
              const Seeds &seeds, dtwj_result *out, clk::time_point t_entry) {
     if (opt && (opt->sweep < 0 || opt->sweep > 2)) return fail(DTWJ_EINVAL, "options.sweep must be 0, 1 or 2");
+    if (opt && opt->cand_cap < 0) return fail(DTWJ_EINVAL, "options.cand_cap must be >= 0");
     Job J;
     plan_job(J, d_a, n, m, w_in, opt ? opt->rows_per_item : 0, opt ? opt->seed_items : 0, seeds.gpu_scan, seeds.pairs,
-             kCandCap);
+             (opt && opt->cand_cap > 0) ? (int64_t)opt->cand_cap : kCandCap);
     J.sweep_mode = opt ? opt->sweep : 0;
     const int32_t wpc = (opt && opt->waves_per_cu > 0) ? opt->waves_per_cu : 8;
     const int32_t grid = grid_for(c, wpc, J.n_items);
-    JoinArgs *d_args = nullptr;
-    int64_t *d_item_base = nullptr, *d_vi = nullptr, *d_vj = nullptr;
-    int32_t *d_work = nullptr;
-    uint32_t *d_queue = nullptr;
-    double *d_vcost = nullptr;
-    auto carve = [&](Arena &ar) {
-        take_job(ar, J);
-        d_args = ar.take<JoinArgs>(1);
-        d_item_base = ar.take<int64_t>(2);
-        d_work = ar.take<int32_t>(1);
-        d_queue = ar.take<uint32_t>((size_t)grid * kJoinQueueCap);
-        d_vi = ar.take<int64_t>(kCandCap);
-        d_vj = ar.take<int64_t>(kCandCap);
-        d_vcost = ar.take<double>(kCandCap);
-    };
-    Arena dry;
-    carve(dry);
-    if (int rc = ensure_mem(c, dry.off + 256)) return rc;
-    Arena ar;
-    ar.base = (char *)c->mem;
-    ar.cap = c->mem_cap;
-    carve(ar);
+    int32_t reruns = 0;
+    double seed_dist = -1.0;
+    for (;;) {
+        JoinArgs *d_args = nullptr;
+        int64_t *d_item_base = nullptr;
+        unsigned long long *d_work = nullptr;
+        uint32_t *d_queue = nullptr;
+        double *d_cost64 = nullptr;
+        Winner *d_part = nullptr, *d_win = nullptr;
+        auto carve = [&](Arena &ar) {
+            take_job(ar, J);
+            d_args = ar.take<JoinArgs>(1);
+            d_item_base = ar.take<int64_t>(2);
+            d_work = ar.take<unsigned long long>(1);
+            d_queue = ar.take<uint32_t>((size_t)grid * kJoinQueueCap);
+            d_cost64 = ar.take<double>((size_t)J.cand_cap);
+            d_part = ar.take<Winner>((size_t)rescore_partials());
+            d_win = ar.take<Winner>(1);
+        };
+        Arena dry;
+        carve(dry);
+        if (int rc = ensure_mem(c, dry.off + 256)) return rc;
+        Arena ar;
+        ar.base = (char *)c->mem;
+        ar.cap = c->mem_cap;
+        carve(ar);
 
-    hipStream_t s = c->stream;
-    const auto t0 = clk::now();
-    if (int rc = upload_job(J, s)) return rc;
-    const int64_t item_base[2] = {0, J.n_items};
-    DTWJ_HIP(hipMemcpyAsync(d_item_base, item_base, sizeof(item_base), hipMemcpyHostToDevice, s));
-    // ---- preprocessing
-    if (int rc = prep_series(J.P, s)) return rc;
-    int32_t flags[4];
-    DTWJ_HIP(hipMemcpyAsync(flags, J.P.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
-    DTWJ_HIP(hipStreamSynchronize(s));
-    if (flags[0]) return fail(DTWJ_EINVAL, "series contains NaN or Inf");
-    if (flags[1]) return fail(DTWJ_EINVAL, "a window of length m is constant (standard deviation 0)");
-    const auto t1 = clk::now();
-    // ---- join descriptor, error model, resets; seed
-    DTWJ_HIP(hipMemcpyAsync(d_args, &J.A, sizeof(JoinArgs), hipMemcpyHostToDevice, s));
-    if (int rc = launch_join_setup(d_args, 1, d_work, s)) return rc;
-    if (int rc = fetch_error_model(J, d_args, s)) return rc;
-    if (int rc = enqueue_seeds(J, d_args, s)) return rc;
-    unsigned long long seed_key = 0;
-    DTWJ_HIP(hipMemcpyAsync(&seed_key, J.d_bsf, sizeof(seed_key), hipMemcpyDeviceToHost, s));
-    DTWJ_HIP(hipStreamSynchronize(s));
-    const auto t2 = clk::now();
-    // ---- fused join
-    DTWJ_HIP(hipEventRecord(c->ev0, s));
-    if (int rc = launch_join(d_args, d_item_base, 1, J.m, J.w, d_work, d_queue, grid, s)) return rc;
-    DTWJ_HIP(hipEventRecord(c->ev1, s));
-    DTWJ_HIP(hipStreamSynchronize(s));
-    float join_ms = 0.f;
-    DTWJ_HIP(hipEventElapsedTime(&join_ms, c->ev0, c->ev1));
-    const auto t3 = clk::now();
-    // ---- FP64 re-verification of every candidate within the FP32 error of the best
-    if (int rc = finish_job(J, d_vi, d_vj, d_vcost, s, out)) return rc;
-    const auto t4 = clk::now();
-    out->seed_dist = std::sqrt((double)bsf_cost(seed_key));
-    out->t_prep = secs(t0, t1);
-    out->t_seed = secs(t1, t2);
-    out->t_join = secs(t2, t3);
-    out->t_verify = secs(t3, t4);
-    out->t_total = secs(t_entry, t4);
-    out->join_kernel_ms = join_ms;
-    out->waves_per_cu = wpc;
-    if (opt && opt->verbose) {
-        std::fprintf(stderr,
-                     "[dtwj] n=%lld m=%d w=%d items=%lld swept=%lld ball=%lld kim=%lld keogh=%lld dtw_ab=%lld "
-                     "cells=%lld cands=%lld seed=%.6f best=%.6f join=%.3f ms\n",
-                     (long long)n, m, J.w, (long long)J.n_items, (long long)out->pairs_swept,
-                     (long long)out->pass_ball, (long long)out->pass_kim, (long long)out->pass_keogh,
-                     (long long)out->dtw_abandoned, (long long)out->dtw_cells, (long long)out->candidates,
-                     out->seed_dist, out->dist, join_ms);
+        hipStream_t s = c->stream;
+        const auto t0 = clk::now();
+        if (int rc = upload_job(J, s)) return rc;
+        const int64_t item_base[2] = {0, J.n_items};
+        DTWJ_HIP(hipMemcpyAsync(d_item_base, item_base, sizeof(item_base), hipMemcpyHostToDevice, s));
+        // ---- preprocessing
+        if (int rc = prep_series(J.P, s)) return rc;
+        int32_t flags[4];
+        DTWJ_HIP(hipMemcpyAsync(flags, J.P.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+        DTWJ_HIP(hipStreamSynchronize(s));
+        if (flags[0]) return fail(DTWJ_EINVAL, "series contains NaN or Inf");
+        if (flags[1]) return fail(DTWJ_EINVAL, "a window of length m is constant (standard deviation 0)");
+        const auto t1 = clk::now();
+        // ---- join descriptor, error model, resets; seed
+        DTWJ_HIP(hipMemcpyAsync(d_args, &J.A, sizeof(JoinArgs), hipMemcpyHostToDevice, s));
+        if (int rc = launch_join_setup(d_args, 1, d_work, s)) return rc;
+        if (int rc = fetch_error_model(J, d_args, s)) return rc;
+        if (int rc = enqueue_seeds(J, d_args, s)) return rc;
+        unsigned long long seed_key = 0;
+        DTWJ_HIP(hipMemcpyAsync(&seed_key, J.d_bsf, sizeof(seed_key), hipMemcpyDeviceToHost, s));
+        DTWJ_HIP(hipStreamSynchronize(s));
+        if (seed_dist < 0) seed_dist = std::sqrt((double)bsf_cost(seed_key));
+        const auto t2 = clk::now();
+        // ---- fused join
+        DTWJ_HIP(hipEventRecord(c->ev0, s));
+        if (int rc = launch_join(d_args, d_item_base, 1, J.m, J.w, d_work, d_queue, grid, s)) return rc;
+        DTWJ_HIP(hipEventRecord(c->ev1, s));
+        DTWJ_HIP(hipStreamSynchronize(s));
+        float join_ms = 0.f;
+        DTWJ_HIP(hipEventElapsedTime(&join_ms, c->ev0, c->ev1));
+        const auto t3 = clk::now();
+        // ---- FP64 re-verification of every candidate within the FP32 error of the best
+        const int rc = finish_job(J, d_cost64, d_part, d_win, s, out);
+        if (rc == kRegrow) {
+            // more near-ties than slots: run again from this run's best-so-far (a valid
+            // threshold, so the re-run pushes no more than this run did) with room for all
+            if (J.cand_cap >= kMaxCandCap || J.cand_need > kMaxCandCap)
+                return fail(DTWJ_EOVERFLOW, "more than 2^31 - 1 near-tie candidates");
+            J.cand_cap = regrown_cap(J.cand_cap, J.cand_need);
+            ++reruns;
+            continue;
+        }
+        if (rc != DTWJ_OK) return rc;
+        const auto t4 = clk::now();
+        out->seed_dist = seed_dist;
+        out->t_prep = secs(t0, t1);
+        out->t_seed = secs(t1, t2);
+        out->t_join = secs(t2, t3);
+        out->t_verify = secs(t3, t4);
+        out->t_total = secs(t_entry, t4);
+        out->join_kernel_ms = join_ms;
+        out->waves_per_cu = wpc;
+        out->reruns = reruns;
+        if (opt && opt->verbose) {
+            std::fprintf(stderr,
+                         "[dtwj] n=%lld m=%d w=%d items=%lld swept=%lld ball=%lld kim=%lld keogh=%lld dtw_ab=%lld "
+                         "cells=%lld cands=%lld seed=%.6f best=%.6f join=%.3f ms reruns=%d\n",
+                         (long long)n, m, J.w, (long long)J.n_items, (long long)out->pairs_swept,
+                         (long long)out->pass_ball, (long long)out->pass_kim, (long long)out->pass_keogh,
+                         (long long)out->dtw_abandoned, (long long)out->dtw_cells, (long long)out->candidates,
+                         out->seed_dist, out->dist, join_ms, reruns);
+        }
+        return DTWJ_OK;
     }
-    return DTWJ_OK;
 }
 
 // ---------------------------------------------------------------- batched form
@@ -789,10 +803,11 @@ int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const d
     const int32_t grid = grid_for(c, wpc, item_base[nj]);
     std::vector<double *> d_data(nj);
     JoinArgs *d_args = nullptr;
-    int64_t *d_item_base = nullptr, *d_vi = nullptr, *d_vj = nullptr;
-    int32_t *d_work = nullptr;
+    int64_t *d_item_base = nullptr;
+    unsigned long long *d_work = nullptr;
     uint32_t *d_queue = nullptr;
-    double *d_vcost = nullptr;
+    double *d_cost64 = nullptr;
+    Winner *d_part = nullptr, *d_win = nullptr;
     SeedJob *d_sjobs = nullptr;
     SeedDtwJob *d_djobs = nullptr;
     int32_t *d_sbase = nullptr, *d_dbase = nullptr;
@@ -804,11 +819,11 @@ int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const d
         }
         d_args = ar.take<JoinArgs>(nj);
         d_item_base = ar.take<int64_t>(nj + 1);
-        d_work = ar.take<int32_t>(1);
+        d_work = ar.take<unsigned long long>(1);
         d_queue = ar.take<uint32_t>((size_t)grid * kJoinQueueCap);
-        d_vi = ar.take<int64_t>(kBatchCandCap);
-        d_vj = ar.take<int64_t>(kBatchCandCap);
-        d_vcost = ar.take<double>(kBatchCandCap);
+        d_cost64 = ar.take<double>(kBatchCandCap);
+        d_part = ar.take<Winner>((size_t)rescore_partials());
+        d_win = ar.take<Winner>(1);
         d_sjobs = ar.take<SeedJob>(nj);
         d_sbase = ar.take<int32_t>(nj + 1);
         d_djobs = ar.take<SeedDtwJob>(nj);
@@ -928,6 +943,7 @@ int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const d
     float ms = 0.f;
     DTWJ_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (join_ms) *join_ms = ms;
+    std::vector<int32_t> regrow;  // batch indices whose candidates overflowed their buffer
     for (int32_t t = 0; t < nj; ++t) {
         const int32_t k = owner[t];
         int32_t flags[4];
@@ -938,9 +954,29 @@ int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const d
                                                    : "a window of length m is constant (standard deviation 0)");
             continue;
         }
-        status[k] = finish_job(jobs[t], d_vi, d_vj, d_vcost, s, &out[k]);
+        status[k] = finish_job(jobs[t], d_cost64, d_part, d_win, s, &out[k]);
+        if (status[k] == kRegrow) {
+            regrow.push_back(k);
+            continue;
+        }
         out[k].join_kernel_ms = ms;
         out[k].waves_per_cu = wpc;
+    }
+    // A series with more near-ties than its batch buffer is joined again on its own (the
+    // single-series path regrows its buffer as needed); the batch's device data is no longer
+    // used, so the context's workspace can be reused.
+    for (int32_t k : regrow) {
+        const dtwj_series &S = series[k];
+        const double *d = nullptr;
+        int rc = dtwj_context_upload(c, S.data, S.n, &d);
+        if (rc == DTWJ_OK) {
+            dtwj_options o = opt ? *opt : dtwj_options{};
+            o.rows_per_item = rows_req;
+            o.waves_per_cu = wpc;
+            rc = run_join(c, d, S.n, S.m, S.w, &o, Seeds{}, &out[k], clk::now());
+            if (rc == DTWJ_OK) out[k].reruns += 1;
+        }
+        status[k] = rc;
     }
     return DTWJ_OK;
 }
@@ -999,6 +1035,28 @@ int with_context(int32_t device, dtwj_context **c) { return dtwj_context_create(
 }  // namespace
 
 extern "C" {
+
+int dtwj_plan(int64_t n, int32_t m, int32_t w, const dtwj_options *opt, dtwj_plan_info *out) {
+    if (!out) return fail(DTWJ_EINVAL, "null plan");
+    std::memset(out, 0, sizeof(*out));
+    // validate() reads no sample: a non-null placeholder stands for the series
+    static const double placeholder = 0.0;
+    if (int rc = validate(&placeholder, n, m, w)) return rc;
+    Job J;
+    plan_job(J, nullptr, n, m, w, opt ? opt->rows_per_item : 0, opt ? opt->seed_items : 0, true, {}, kCandCap);
+    const int32_t wpc = (opt && opt->waves_per_cu > 0) ? opt->waves_per_cu : 8;
+    const int64_t grid = std::min<int64_t>((int64_t)256 * wpc, std::max<int64_t>(J.n_items, 1));
+    auto pad = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t bytes = job_bytes(J);
+    bytes += pad(sizeof(JoinArgs)) + pad(16) + pad(8) + pad(4 * (size_t)grid * kJoinQueueCap) +
+             pad(8 * (size_t)J.cand_cap) + pad(sizeof(Winner) * (size_t)rescore_partials()) + pad(sizeof(Winner));
+    out->pairs_total = J.K * (J.K + 1) / 2;
+    out->items = J.n_items;
+    out->rows_per_item = J.rows_per_item;
+    out->diag_groups = J.groups;
+    out->device_bytes = (int64_t)bytes;
+    return DTWJ_OK;
+}
 
 int dtwj_top1_device(dtwj_context *c, const double *d_series, int64_t n, int32_t m, int32_t w,
                      const dtwj_options *opt, dtwj_result *out) {

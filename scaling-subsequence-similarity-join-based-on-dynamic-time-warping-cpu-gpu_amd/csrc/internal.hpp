@@ -50,10 +50,10 @@ int launch_seed_batch(const SeedJob *jobs, const int32_t *base, int32_t nj, int3
 int launch_seed_dtw_batch(const JoinArgs *d_args, int32_t max_m, int32_t w, const SeedDtwJob *jobs,
                           const int32_t *base, int32_t nj, int32_t nblocks, hipStream_t s);
 // Error model, validity and state resets of nseries device join descriptors; zeroes *d_work.
-int launch_join_setup(JoinArgs *d_series, int32_t nseries, int32_t *d_work, hipStream_t s);
+int launch_join_setup(JoinArgs *d_series, int32_t nseries, unsigned long long *d_work, hipStream_t s);
 // The fused persistent join over nseries series (device descriptors, cumulative item counts).
 int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t nseries, int32_t max_m, int32_t max_w,
-                int32_t *d_work, uint32_t *d_queue, int32_t grid, hipStream_t s);
+                unsigned long long *d_work, uint32_t *d_queue, int32_t grid, hipStream_t s);
 size_t join_lds_bytes(int32_t m, int32_t w);
 
 // DTW of an explicit pair list (dtw.hip).  prec 32: FP32 cells from ac/mu_c/isig;
@@ -61,6 +61,23 @@ size_t join_lds_bytes(int32_t m, int32_t w);
 // series / statistics pointers and m, w of args (a host copy).
 int launch_dtw_pairs(const JoinArgs &args, const double *sig64, const int64_t *pi, const int64_t *pj,
                      int64_t npairs, int32_t prec, double *cost, hipStream_t s);
+
+// FP64 winner of a candidate list: cost, pair, the FP32 cost the join found, and how many
+// candidates were re-scored (FP32 cost <= ub).
+struct Winner {
+    double cost;
+    int32_t i, j;
+    float cost32;
+    uint32_t pad;
+    unsigned long long verified;
+};
+// Number of Winner partials launch_rescore needs in `part`.
+int rescore_partials();
+// Re-scores the n candidates (i, j, FP32 cost bits) of a join in FP64 (those with FP32 cost
+// <= ub; cost64: n doubles of scratch) and writes the lexicographic minimum of
+// (FP64 cost, i, j) to *out (device).  Reads the series / statistics pointers and m, w of A.
+int launch_rescore(const JoinArgs &A, const double *sig64, const int4 *cands, int64_t n, float ub, double *cost64,
+                   Winner *part, Winner *out, hipStream_t s);
 
 // Learned proposal (proposal.hip).
 int mlp_param_count();

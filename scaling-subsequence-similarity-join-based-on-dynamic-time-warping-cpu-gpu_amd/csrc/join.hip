@@ -116,6 +116,15 @@ __device__ inline Thr threshold_of(unsigned long long key, const JoinArgs &A) {
 
 __device__ inline Thr load_threshold(const JoinArgs &A) { return threshold_of(load_bsf(A), A); }
 
+// Records a pair whose FP32 cost is within the error margin of the best-so-far (re-scored in
+// FP64 after the join) and lowers the best-so-far.  The push counter is 64-bit, so it keeps
+// counting past cand_cap without wrapping: the host sees how many slots a re-run needs.
+__device__ inline void push_candidate(const JoinArgs &A, int i, int j, float cost) {
+    const unsigned long long slot = atomicAdd(A.n_cands, 1ull);
+    if (slot < (unsigned long long)A.cand_cap) A.cands[slot] = make_int4(i, j, __float_as_int(cost), 0);
+    atomicMin(A.bsf, pack_bsf(cost, (uint32_t)i));
+}
+
 __device__ inline float sq(float x) { return x * x; }
 // x, with +0 replaced by -0 (equal as numbers; only the sign of a zero changes)
 __device__ inline float no_pos_zero(float x) { return x == 0.0f ? -0.0f : x; }
@@ -268,7 +277,7 @@ __device__ inline int pair_col(uint32_t p) { return (int)(p >> 16); }
 // uni: floats of the LDS union area, sized for the longest window of the launch.
 template <int Q>
 __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base,
-                                          int32_t nseries, int32_t uni, int32_t *work_counter, uint32_t *queue) {
+                                          int32_t nseries, int32_t uni, unsigned long long *work_counter, uint32_t *queue) {
     constexpr bool PW = packed_envelope(Q);
     extern __shared__ __align__(16) float smem[];  // 16-byte aligned: the sweep reads float4 rows
     const int lane = (int)threadIdx.x;
@@ -428,9 +437,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
             count(C_DTW_AB, __popcll(__builtin_amdgcn_ballot_w64(ab)));
             if (lead && !ab) {
                 const int gi = (int)(cb + li), gj = (int)(cb + d0 + lj);
-                int slot = atomicAdd(A.n_cands, 1);
-                if (slot < A.cand_cap) A.cands[slot] = make_int4(gi, gj, __float_as_int(res), 0);
-                atomicMin(A.bsf, pack_bsf(res, (uint32_t)gi));
+                push_candidate(A, gi, gj, res);
             }
         }
         q3n = 0;
@@ -819,9 +826,11 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     };
 
     for (;;) {
-        int item_l = 0;
-        if (lane == 0) item_l = atomicAdd(work_counter, 1);
-        const int64_t gitem = (int64_t)__builtin_amdgcn_readfirstlane(item_l);
+        // 64-bit claim counter: the item count of a long series exceeds 2^31
+        unsigned long long item_l = 0;
+        if (lane == 0) item_l = atomicAdd(work_counter, 1ull);
+        const int64_t gitem = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(item_l >> 32)) << 32) |
+                                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)item_l));
         if (gitem >= item_base[nseries]) break;
         // series of this item (usually the one of the previous item)
         int s = cur;
@@ -1257,7 +1266,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
 
 template <int Q>
 __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base,
-                                             int32_t nseries, int32_t uni, int32_t *work_counter, uint32_t *queue) {
+                                             int32_t nseries, int32_t uni, unsigned long long *work_counter, uint32_t *queue) {
     join_body<Q>(series, item_base, nseries, uni, work_counter, queue);
 }
 
@@ -1267,16 +1276,16 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
 template <int Q>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_join_occ2(
     const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base, int32_t nseries, int32_t uni,
-    int32_t *work_counter, uint32_t *queue) {
+    unsigned long long *work_counter, uint32_t *queue) {
     join_body<Q>(series, item_base, nseries, uni, work_counter, queue);
 }
 
 // ---- set-up of every series of a join: the FP32 error model from the preprocessing's value
 // range, the validity flag, and the best-so-far / candidate / counter resets (a kernel, so
 // that a captured graph holds no memset node).  One thread per series.
-__global__ void k_join_setup(JoinArgs *series, int32_t nseries, int32_t *work_counter) {
+__global__ void k_join_setup(JoinArgs *series, int32_t nseries, unsigned long long *work_counter) {
     const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (s == 0) *work_counter = 0;
+    if (s == 0) *work_counter = 0ull;
     if (s >= nseries) return;
     JoinArgs &A = series[s];
     const float *g = A.gstat;
@@ -1285,8 +1294,8 @@ __global__ void k_join_setup(JoinArgs *series, int32_t nseries, int32_t *work_co
     A.eps_z = (float)(0x1p-22 * (((double)g[0] + (double)g[1]) * (double)g[2] + sqrt(m)));
     A.rel_dtw = (float)(0x1p-10 + 4.0 * m * 0x1p-24);
     A.valid = (A.flags[0] == 0 && A.flags[1] == 0) ? 1 : 0;
-    *A.bsf = pack_bsf(3.0e38f, 0xffffffffu);
-    *A.n_cands = 0;
+    *A.bsf = A.init_key;
+    *A.n_cands = 0ull;
     for (int c = 0; c < C_NCOUNTERS; ++c) A.counters[c] = 0ull;
 }
 
@@ -1440,11 +1449,7 @@ __device__ __forceinline__ void seed_dtw_block(const JoinArgs &A, const int2 *pa
     const PlainZ<float> z{zs, zs + m};
     int t_stop = 0;
     const float res = dtw_band<float, QP>(m, A.w, 64, 1, th.dtw_ub, z, t_stop);
-    if (res <= th.dtw_ub && lane == 0) {
-        int slot = atomicAdd(A.n_cands, 1);
-        if (slot < A.cand_cap) A.cands[slot] = make_int4(i, j, __float_as_int(res), 0);
-        atomicMin(A.bsf, pack_bsf(res, (uint32_t)i));
-    }
+    if (res <= th.dtw_ub && lane == 0) push_candidate(A, i, j, res);
 }
 
 template <int QP>
@@ -1471,14 +1476,14 @@ size_t join_lds_bytes(int32_t m, int32_t w) {
            sizeof(uint32_t) * (PLCAP + Q2CAP + Q3CAP);
 }
 
-int launch_join_setup(JoinArgs *d_series, int32_t nseries, int32_t *d_work, hipStream_t s) {
+int launch_join_setup(JoinArgs *d_series, int32_t nseries, unsigned long long *d_work, hipStream_t s) {
     hipLaunchKernelGGL(k_join_setup, dim3((unsigned)((nseries + 63) / 64)), dim3(64), 0, s, d_series, nseries, d_work);
     DTWJ_HIP(hipGetLastError());
     return 0;
 }
 
 int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t nseries, int32_t max_m, int32_t max_w,
-                int32_t *d_work, uint32_t *d_queue, int32_t grid, hipStream_t s) {
+                unsigned long long *d_work, uint32_t *d_queue, int32_t grid, hipStream_t s) {
     const int Q = dtw_qp_for(max_w);
     const size_t lds = join_lds_bytes(max_m, max_w);
     const int32_t uni = union_floats(max_m, packed_envelope(Q));

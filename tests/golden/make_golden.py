@@ -48,6 +48,11 @@ CASES = [
                                               regime_len=1000), 128, 13),
     ("n8192_noplant_m128_w6", 8192, 21, dict(), 128, 6),
     ("n4096_m64_w2", 4096, 22, dict(n_plants=1, plant_len=64), 64, 2),
+    # the wide-band workload's regime shifts (step std 0.5 / 2.0) at its window and band
+    ("n24576_regime_m1024_w102", 24576, 24, dict(n_plants=1, plant_len=1024, step_std=0.5, step_std_alt=2.0,
+                                                 regime_len=3000), 1024, 102),
+    # the medium workload's window and band as the configuration states it (w = 2)
+    ("n16384_m256_w2", 16384, 25, dict(n_plants=3, plant_len=256), 256, 2),
 ]
 
 # the reference CPU main's configuration grid (CPU_version1/AAAA.cpp:30-38)

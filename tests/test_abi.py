@@ -119,3 +119,96 @@ def test_cli_grid_order_without_a_device(dtwj, tmp_path):
     assert r.stderr.count("error ") == 24
     r = subprocess.run([dtwj.CLI_PATH, str(data), "--grid"], capture_output=True, text=True)
     assert r.returncode == 2
+
+
+# the reference reads with `while (inputFile >> value)` (CPU_version1/loadfile.cpp:24-28); this
+# program does exactly that with the C++ standard library, the loader's independent oracle
+_STREAM_READER = r"""
+#include <cstdio>
+#include <fstream>
+int main(int argc, char **argv) {
+    std::ifstream f(argv[1]);
+    double v;
+    while (f >> v) std::printf("%.17g\n", v);
+    return 0;
+}
+"""
+
+LOADER_CASES = [
+    "1 2 3 x 4 5",          # stops at the first non-number
+    "1.5 nan 2",            # nan is not a number for operator>>
+    "1 inf 2", "1 -Infinity 2",
+    "0x1p3 4",              # hex float: reads 0, then fails at 'x'
+    "1 2 1e 3",             # incomplete exponent: the extraction fails, 1 is not read
+    "1 2 1e+ 3", "1 2 1e5x 3", "1 2 3.5e-2 4E+1",
+    "1.5.3 7",              # two numbers without a separator
+    "+.5 -.25 . 3",         # a lone point fails
+    "- 3", "+7 -8",
+    "1e400 2",              # overflow fails
+    "1e-400 2",             # underflow reads a denormal or zero
+    "  \t\n 12\r\n13\v14\f15 ",
+    "00012.500 -0 1.",
+    "5e 6",
+]
+
+
+@pytest.mark.skipif(subprocess.run(["which", "g++"], capture_output=True).returncode != 0, reason="g++ missing")
+def test_loader_matches_the_standard_stream_extraction(dtwj, tmp_path):
+    src = tmp_path / "rd.cpp"
+    src.write_text(_STREAM_READER)
+    exe = tmp_path / "rd"
+    subprocess.run(["g++", "-O1", "-o", str(exe), str(src)], check=True, capture_output=True)
+    for k, text in enumerate(LOADER_CASES):
+        p = tmp_path / ("case%d.txt" % k)
+        p.write_text(text)
+        out = subprocess.run([str(exe), str(p)], check=True, capture_output=True, text=True).stdout
+        want = [float(x) for x in out.split()]
+        got = list(dtwj.loadfile(str(p)))
+        assert got == want, (text, got, want)
+
+
+def _items_closed_form(n, m, rows):
+    """Sweep items of a plan: per group g of 256 diagonal offsets, ceil(rows_g / rows) items,
+    rows_g = (n - m + 1) - (m + 256 g) rows."""
+    nw, K = n - m + 1, n - 2 * m + 1
+    G = (K + 255) // 256
+    g = np.arange(G, dtype=np.int64)
+    rows_g = nw - (m + 256 * g)
+    return int(((rows_g + rows - 1) // rows).sum())
+
+
+@pytest.mark.parametrize("n,m,w", [(16384, 128, 6), (1_000_000, 256, 2), (1_000_000, 256, 13),
+                                   (4_000_000, 1024, 102), (60_000_000, 256, 2), (2_000_000_000, 64, 1)])
+def test_plan_item_count_is_64_bit(dtwj, n, m, w):
+    """The sweep's work items are claimed from a 64-bit device counter: the plan's count is the
+    closed form, also past 2^31 (n = 6e7 at w = 2 gives 2.3e9 items; n = 2e9, 6e15)."""
+    p = dtwj.plan(n, m, w)
+    assert p.rows_per_item == (3072 if w <= 3 else 2048)
+    assert p.items == _items_closed_form(n, m, p.rows_per_item)
+    K = n - 2 * m + 1
+    assert p.pairs_total == K * (K + 1) // 2
+    assert p.diag_groups == (K + 255) // 256
+    if n >= 60_000_000:
+        assert p.items > 2 ** 31
+    assert p.device_bytes > 180 * n  # 180 bytes of tables per sample, plus buffers
+
+
+def test_plan_rejects_what_the_join_rejects(dtwj):
+    with pytest.raises(dtwj.DTWJError):
+        dtwj.plan(2 ** 31, 256, 2)
+    with pytest.raises(dtwj.DTWJError):
+        dtwj.plan(500, 256, 2)
+    with pytest.raises(dtwj.DTWJError):
+        dtwj.plan(100000, 1024, 600)
+
+
+def test_result_mirror_maps_every_field(dtwj):
+    """Every field of dtwj_result reaches the Python result object (no GPU involved)."""
+    r = dtwj.Result()
+    for k, (name, _) in enumerate(dtwj.Result._fields_):
+        if not name.startswith("reserved"):
+            setattr(r, name, k + 1)
+    t = dtwj._result(r)
+    for k, (name, _) in enumerate(dtwj.Result._fields_):
+        if not name.startswith("reserved"):
+            assert getattr(t, name) == k + 1, name

@@ -17,6 +17,7 @@ STRUCTS = {
     "dtwj_mlp": "MLP",
     "dtwj_proposal_report": "ProposalReport",
     "dtwj_gen_spec": "GenSpec",
+    "dtwj_plan_info": "PlanInfo",
 }
 
 
