@@ -211,6 +211,19 @@ void dtwj_context_destroy(dtwj_context *ctx);
  * context's device).  Used to time the join with inputs resident in HBM. */
 int dtwj_top1_device(dtwj_context *ctx, const double *d_series, int64_t n, int32_t m, int32_t w,
                      const dtwj_options *opt, dtwj_result *out);
+/* The same join with no device allocation, for callers that cannot allocate (code that runs
+ * under HIP graph capture, tight serving loops, harnesses that forbid allocation).  No
+ * counterpart in the reference.  Every buffer the join needs beyond its input and output
+ * lives in one fixed 64 MiB workspace per GPU (a device array of the library, held until the
+ * process ends), which limits the series to about 2^17 samples (n <= 131072 at any m, w;
+ * larger inputs return DTWJ_EINVAL with the bytes needed).  d_series: n doubles on the device
+ * opt->device (default 0); stream: a hipStream_t to run on (NULL = the null stream).  Calls no
+ * hipMalloc / hipFree.  Calls on one GPU share its workspace: calls that overlap in time on one
+ * GPU race and must be serialised by the caller.  A series with more near-tie candidates than
+ * the workspace's 2^16 slots returns DTWJ_EOVERFLOW (dtwj_top1_device regrows instead).  The
+ * join time is taken on the host clock (no events are created). */
+int dtwj_top1_device_noalloc(const double *d_series, int64_t n, int32_t m, int32_t w, const dtwj_options *opt,
+                             void *stream, dtwj_result *out);
 /* Copies n doubles to a device buffer owned by the context (valid until the next call of
  * this function or the context's destruction) and returns its device address. */
 int dtwj_context_upload(dtwj_context *ctx, const double *series, int64_t n, const double **d_series);

@@ -111,7 +111,7 @@ EXPORTED = [
     "dtwj_free", "dtwj_last_error", "dtwj_device_count", "dtwj_sliding_stats", "dtwj_envelope", "dtwj_dtw_pairs",
     "dtwj_gen_spec_default", "dtwj_generate", "dtwj_top1_batch",
     "dtwj_context_top1_batch", "dtwj_proposal_candidates", "dtwj_mlp_forward", "dtwj_mlp_gradient",
-    "dtwj_mlp_train", "dtwj_top1_learned", "dtwj_plan",
+    "dtwj_mlp_train", "dtwj_top1_learned", "dtwj_plan", "dtwj_top1_device_noalloc",
 ]
 
 _lib = None
@@ -160,6 +160,8 @@ def lib():
     L.dtwj_top1_learned.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, P(MLP), ctypes.c_int32,
                                     ctypes.c_int32, P(Options), P(Result), P(ProposalReport)]
     L.dtwj_plan.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, P(Options), P(PlanInfo)]
+    L.dtwj_top1_device_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                           P(Options), ctypes.c_void_p, P(Result)]
     L.dtwj_gen_spec_default.argtypes = [P(GenSpec)]
     L.dtwj_generate.argtypes = [P(GenSpec), d_p, i64_p, i64_p]
     _lib = L
@@ -527,6 +529,14 @@ class Context:
         r = Result()
         _check(lib().dtwj_top1_device(self._c, self._d, self.n, int(m), int(w), ctypes.byref(make_options(**opts)),
                                       ctypes.byref(r)))
+        return _result(r)
+
+    def top1_noalloc(self, m, w, stream=None, **opts):
+        """The same join on the uploaded series through dtwj_top1_device_noalloc (the library's
+        fixed per-device workspace; no device allocation; the context only holds the input)."""
+        r = Result()
+        _check(lib().dtwj_top1_device_noalloc(self._d, self.n, int(m), int(w), ctypes.byref(make_options(**opts)),
+                                              stream, ctypes.byref(r)))
         return _result(r)
 
     def top1_batch(self, series_list, ms, ws, **opts):

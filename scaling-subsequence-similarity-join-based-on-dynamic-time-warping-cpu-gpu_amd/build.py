@@ -27,7 +27,8 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include"),
           "-I", CSRC]
 
-LIB_SOURCES = ["dtwj_api.cpp", "series_gen.cpp", "loader.cpp", "prep.hip", "join.hip", "dtw.hip", "proposal.hip"]
+LIB_SOURCES = ["dtwj_api.cpp", "series_gen.cpp", "loader.cpp", "prep.hip", "join.hip", "dtw.hip", "proposal.hip",
+               "noalloc.hip"]
 CLI_SOURCES = ["gpu_dtw_cli.cpp"]
 TOOL_SOURCES = ["copy_calib.hip"]   # standalone executables (traffic-counter calibration)
 

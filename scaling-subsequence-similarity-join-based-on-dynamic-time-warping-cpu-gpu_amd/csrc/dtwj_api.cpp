@@ -77,6 +77,9 @@ struct dtwj_context {
     double *upload = nullptr;
     int64_t upload_cap = 0;
     int num_cus = 0;
+    // workspace of fixed size that must not be reallocated (the no-allocation entry point's
+    // per-device array; no events either: the join is timed on the host clock)
+    bool fixed = false;
     // batches: tile choices of the autotuner, keyed by the batch's shape
     std::map<std::string, std::pair<int32_t, int32_t>> tuned;
 };
@@ -455,6 +458,12 @@ int32_t grid_for(const dtwj_context *c, int32_t wpc, int64_t n_items) {
 }
 
 int ensure_mem(dtwj_context *c, size_t need) {
+    if (need > c->mem_cap && c->fixed) {
+        char buf[200];
+        std::snprintf(buf, sizeof(buf), "input too large for the fixed workspace: %zu bytes needed, %zu available",
+                      need, c->mem_cap);
+        return fail(DTWJ_EINVAL, buf);
+    }
     if (need > c->mem_cap) {
         if (c->mem) DTWJ_HIP(hipFree(c->mem));
         c->mem = nullptr;
@@ -717,18 +726,21 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
         if (seed_dist < 0) seed_dist = std::sqrt((double)bsf_cost(seed_key));
         const auto t2 = clk::now();
         // ---- fused join
-        DTWJ_HIP(hipEventRecord(c->ev0, s));
+        if (c->ev0) DTWJ_HIP(hipEventRecord(c->ev0, s));
         if (int rc = launch_join(d_args, d_item_base, 1, J.m, J.w, d_work, d_queue, grid, s)) return rc;
-        DTWJ_HIP(hipEventRecord(c->ev1, s));
+        if (c->ev1) DTWJ_HIP(hipEventRecord(c->ev1, s));
         DTWJ_HIP(hipStreamSynchronize(s));
-        float join_ms = 0.f;
-        DTWJ_HIP(hipEventElapsedTime(&join_ms, c->ev0, c->ev1));
         const auto t3 = clk::now();
+        float join_ms = (float)(secs(t2, t3) * 1e3);  // host clock without events
+        if (c->ev0 && c->ev1) DTWJ_HIP(hipEventElapsedTime(&join_ms, c->ev0, c->ev1));
         // ---- FP64 re-verification of every candidate within the FP32 error of the best
         const int rc = finish_job(J, d_cost64, d_part, d_win, s, out);
         if (rc == kRegrow) {
             // more near-ties than slots: run again from this run's best-so-far (a valid
             // threshold, so the re-run pushes no more than this run did) with room for all
+            if (c->fixed)
+                return fail(DTWJ_EOVERFLOW, "more near-tie candidates than the fixed workspace holds "
+                                            "(the allocating entry points regrow their buffer)");
             if (J.cand_cap >= kMaxCandCap || J.cand_need > kMaxCandCap)
                 return fail(DTWJ_EOVERFLOW, "more than 2^31 - 1 near-tie candidates");
             J.cand_cap = regrown_cap(J.cand_cap, J.cand_need);
@@ -1056,6 +1068,41 @@ int dtwj_plan(int64_t n, int32_t m, int32_t w, const dtwj_options *opt, dtwj_pla
     out->diag_groups = J.groups;
     out->device_bytes = (int64_t)bytes;
     return DTWJ_OK;
+}
+
+int dtwj_top1_device_noalloc(const double *d_series, int64_t n, int32_t m, int32_t w, const dtwj_options *opt,
+                             void *stream, dtwj_result *out) {
+    const auto te = clk::now();
+    if (!out) return fail(DTWJ_EINVAL, "null result");
+    if (int rc = validate(d_series, n, m, w)) return rc;
+    if (opt && (opt->cand_cap < 0 || opt->waves_per_cu < 0)) return fail(DTWJ_EINVAL, "bad options");
+    const int32_t device = opt ? opt->device : 0;
+    const int count = dtwj_device_count();
+    if (count <= 0) return fail(DTWJ_EDEVICE, "no HIP device available");
+    if (device < 0 || device >= count) return fail(DTWJ_EINVAL, "device ordinal out of range");
+    DTWJ_HIP(hipSetDevice(device));
+    void *ws = nullptr;
+    if (int rc = noalloc_scratch(device, &ws)) return rc;
+    int cus = 0;
+    DTWJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    // a context over the fixed workspace and the caller's stream: nothing is created or freed
+    dtwj_context c;
+    c.device = device;
+    c.stream = (hipStream_t)stream;
+    c.mem = ws;
+    c.mem_cap = noalloc_scratch_bytes();
+    c.num_cus = cus;
+    c.fixed = true;
+    dtwj_options o;
+    if (opt)
+        o = *opt;
+    else
+        dtwj_default_options(&o);
+    if (o.waves_per_cu == 0) o.waves_per_cu = 4;  // 1024 survivor queues of 16 KB on 256 CUs
+    if (o.cand_cap == 0) o.cand_cap = 1 << 16;
+    const int rc = run_join(&c, d_series, n, m, w, &o, Seeds{}, out, te);
+    c.mem = nullptr;  // the workspace stays with the device
+    return rc;
 }
 
 int dtwj_top1_device(dtwj_context *c, const double *d_series, int64_t n, int32_t m, int32_t w,

@@ -79,6 +79,11 @@ int rescore_partials();
 int launch_rescore(const JoinArgs &A, const double *sig64, const int4 *cands, int64_t n, float ub, double *cost64,
                    Winner *part, Winner *out, hipStream_t s);
 
+// Fixed per-device workspace of the no-allocation entry point (noalloc.hip): its size, and the
+// device address of `device`'s copy (looked up on first use, then cached).
+size_t noalloc_scratch_bytes();
+int noalloc_scratch(int device, void **ptr);
+
 // Learned proposal (proposal.hip).
 int mlp_param_count();
 int mlp_feature_pad();
