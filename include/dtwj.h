@@ -232,10 +232,25 @@ int dtwj_context_upload(dtwj_context *ctx, const double *series, int64_t n, cons
 int dtwj_context_top1_batch(dtwj_context *ctx, const dtwj_series *series, int32_t count, const dtwj_options *opt,
                             dtwj_result *out, int32_t *status);
 
+/* ED matrix profile with trivial-match exclusion m, on the GPU in FP64.  Replaces mpx_v2(TS,
+ * minlag = m, m) (GPU_version1/src/mpx_v2.cu:14-150, CPU_version1/mpx_v2.cpp:13-150) as
+ * DTWMotifDiscoveryGUI calls it (GPU_version1/src/DTWMotifDiscoveryGUI.cu:20-24): mp_ed[t] =
+ * z-normalised Euclidean distance of window t to its nearest window p with |t - p| >= m,
+ * mp_idx_1based[t] = p + 1 (0 and sqrt(4m) where no window is that far).  n - m + 1 entries each.
+ * Partners whose squared distances agree to FP32 precision count as tied: the smallest lag |t - p|
+ * wins, the left one at equal lags (the reference's visiting order); the distance is the chosen
+ * pair's own FP64 value.  Non-finite samples and constant windows return DTWJ_EINVAL. */
+int dtwj_matrix_profile(const double *series, int64_t n, int32_t m, double *mp_ed, int32_t *mp_idx_1based);
+
 /* Whole pipeline of the reference CLI for one (m, w): DTWMotifDiscoveryGUI(TS, m, w, file)
- * (GPU_version1/src/DTWMotifDiscoveryGUI.cu:12-28): runs the join, prints the reference's
- * result lines to stdout and appends one "%lf\n" seconds line to time_file
- * (GPU_version1/src/new_dtw_motifGUI_malloc.cu:744-753).  time_file may be NULL. */
+ * (GPU_version1/src/DTWMotifDiscoveryGUI.cu:12-28): the ED matrix profile (dtwj_matrix_profile),
+ * the reference's seed (its 100 smallest entries in scan order, GPU_version1/allmatrixop/min_v.cu:15-46,
+ * each pair verified by FP64 DTW, GPU_version1/src/new_dtw_motifGUI_malloc.cu:82-107), then the
+ * join seeded with those pairs; prints the reference's result lines to stdout ("input of bsf"
+ * is that seed) and appends one "%lf\n" seconds line to time_file
+ * (GPU_version1/src/new_dtw_motifGUI_malloc.cu:744-753).  time_file may be NULL.  out->seed_dist
+ * is the FP64 seed; out->t_total covers the join only, after the profile (the reference's timer
+ * also starts after its profile, GPU_version1/src/new_dtw_motifGUI_malloc.cu:602). */
 int dtwj_motif_discovery(const double *series, int64_t n, int32_t m, int32_t w, const char *time_file,
                          const dtwj_options *opt, dtwj_result *out);
 

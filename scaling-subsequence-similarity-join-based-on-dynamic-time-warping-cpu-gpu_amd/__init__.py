@@ -111,7 +111,7 @@ EXPORTED = [
     "dtwj_free", "dtwj_last_error", "dtwj_device_count", "dtwj_sliding_stats", "dtwj_envelope", "dtwj_dtw_pairs",
     "dtwj_gen_spec_default", "dtwj_generate", "dtwj_top1_batch",
     "dtwj_context_top1_batch", "dtwj_proposal_candidates", "dtwj_mlp_forward", "dtwj_mlp_gradient",
-    "dtwj_mlp_train", "dtwj_top1_learned", "dtwj_plan", "dtwj_top1_device_noalloc",
+    "dtwj_mlp_train", "dtwj_top1_learned", "dtwj_plan", "dtwj_top1_device_noalloc", "dtwj_matrix_profile",
 ]
 
 _lib = None
@@ -160,6 +160,7 @@ def lib():
     L.dtwj_top1_learned.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, P(MLP), ctypes.c_int32,
                                     ctypes.c_int32, P(Options), P(Result), P(ProposalReport)]
     L.dtwj_plan.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, P(Options), P(PlanInfo)]
+    L.dtwj_matrix_profile.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, d_p, i32_p]
     L.dtwj_top1_device_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                            P(Options), ctypes.c_void_p, P(Result)]
     L.dtwj_gen_spec_default.argtypes = [P(GenSpec)]
@@ -425,10 +426,23 @@ def top1_seeded(series, m, w, mp_ed, mp_idx_1based, **opts):
     return _result(r)
 
 
+def matrix_profile(series, m):
+    """ED matrix profile with exclusion m on the GPU (mpx_v2, GPU_version1/src/mpx_v2.cu:14-150):
+    returns (mp_ed, mp_idx_1based), n - m + 1 entries each."""
+    a = _as_series(series)
+    nw = a.size - int(m) + 1
+    mp = np.empty(max(nw, 1))
+    mpi = np.empty(max(nw, 1), dtype=np.int32)
+    _check(lib().dtwj_matrix_profile(_dptr(a), a.size, int(m), _dptr(mp),
+                                     mpi.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    return mp, mpi
+
+
 def DTWMotifDiscoveryGUI(TS, subseqlen, maxwarp, output_time_file_path, **opts):
-    """Mirror of DTWMotifDiscoveryGUI (GPU_version1/src/DTWMotifDiscoveryGUI.cu:12-28):
-    runs the join for one (m, w), prints the reference's result lines and appends the
-    seconds to output_time_file_path.  Returns the Top1 result."""
+    """Mirror of DTWMotifDiscoveryGUI (GPU_version1/src/DTWMotifDiscoveryGUI.cu:12-28): the ED
+    matrix profile, the reference's seed from its 100 smallest entries, then the join for one
+    (m, w); prints the reference's result lines and appends the seconds to
+    output_time_file_path.  Returns the Top1 result (seed_dist: the printed "input of bsf")."""
     a = _as_series(TS)
     r = Result()
     path = None if output_time_file_path is None else os.fsencode(output_time_file_path)

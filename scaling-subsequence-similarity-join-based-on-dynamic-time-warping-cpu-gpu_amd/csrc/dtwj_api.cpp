@@ -1164,12 +1164,108 @@ int dtwj_top1_seeded(const double *series, int64_t n, int32_t m, int32_t w, cons
     return top1_host(series, n, m, w, seeds, opt, out);
 }
 
+// ED matrix profile of a series already on the device, into host arrays mp / mpi (nw each).
+static int matrix_profile_on(dtwj_context *c, const double *d_a, int64_t n, int32_t m, double *mp, int32_t *mpi) {
+    const int64_t nw = n - m + 1;
+    const int64_t nsc = mp_scratch_doubles(n, m);
+    double *d_scr = nullptr, *d_mp = nullptr;
+    int32_t *d_mpi = nullptr, *d_flags = nullptr;
+    auto carve = [&](Arena &ar) {
+        d_scr = ar.take<double>((size_t)nsc);
+        d_mp = ar.take<double>((size_t)nw);
+        d_mpi = ar.take<int32_t>((size_t)nw);
+        d_flags = ar.take<int32_t>(2);
+    };
+    Arena dry;
+    carve(dry);
+    if (int rc = ensure_mem(c, dry.off + 256)) return rc;
+    Arena ar;
+    ar.base = (char *)c->mem;
+    ar.cap = c->mem_cap;
+    carve(ar);
+    hipStream_t s = c->stream;
+    if (int rc = launch_matrix_profile(d_a, n, m, d_scr, d_flags, d_mp, d_mpi, s)) return rc;
+    int32_t flags[2] = {0, 0};
+    DTWJ_HIP(hipMemcpyAsync(flags, d_flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    if (flags[0]) return fail(DTWJ_EINVAL, "series contains NaN or Inf");
+    if (flags[1]) return fail(DTWJ_EINVAL, "a window of length m is constant (standard deviation 0)");
+    DTWJ_HIP(hipMemcpyAsync(mp, d_mp, sizeof(double) * nw, hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipMemcpyAsync(mpi, d_mpi, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipStreamSynchronize(s));
+    return DTWJ_OK;
+}
+
+// The reference's seed windows: the indices whose profile value is strictly below the k-th
+// smallest value, in scan order, padded with index 0 to k + 10 entries, of which the seed loop
+// visits the first k (min_v_k, GPU_version1/allmatrixop/min_v.cu:15-46, and the loop at
+// GPU_version1/src/new_dtw_motifGUI_malloc.cu:82-107).
+static std::vector<int64_t> reference_seed_windows(const double *mp, int64_t nw, int k) {
+    std::vector<double> heap(k, std::numeric_limits<double>::infinity());  // max-heap of the k smallest
+    for (int64_t t = 0; t < nw; ++t) {
+        if (mp[t] < heap.front()) {
+            std::pop_heap(heap.begin(), heap.end());
+            heap.back() = mp[t];
+            std::push_heap(heap.begin(), heap.end());
+        }
+    }
+    const double kth = heap.front();
+    std::vector<int64_t> out((size_t)k + 10, 0);
+    size_t idx = 0;
+    for (int64_t t = 0; t < nw && idx < out.size(); ++t)
+        if (mp[t] < kth) out[idx++] = t;
+    out.resize((size_t)k);
+    return out;
+}
+
+int dtwj_matrix_profile(const double *series, int64_t n, int32_t m, double *mp_ed, int32_t *mp_idx_1based) {
+    if (!mp_ed || !mp_idx_1based) return fail(DTWJ_EINVAL, "null output");
+    if (int rc = validate(series, n, m, 0)) return rc;
+    for (int64_t t = 0; t < n; ++t)
+        if (!std::isfinite(series[t])) return fail(DTWJ_EINVAL, "series contains NaN or Inf");
+    dtwj_context *c = nullptr;
+    if (int rc = with_context(0, &c)) return rc;
+    const double *d = nullptr;
+    int rc = dtwj_context_upload(c, series, n, &d);
+    if (rc == DTWJ_OK) rc = matrix_profile_on(c, d, n, m, mp_ed, mp_idx_1based);
+    dtwj_context_destroy(c);
+    return rc;
+}
+
 int dtwj_motif_discovery(const double *series, int64_t n, int32_t m, int32_t w, const char *time_file,
                          const dtwj_options *opt, dtwj_result *out) {
     dtwj_result local;
     dtwj_result *r = out ? out : &local;
-    int rc = dtwj_top1(series, n, m, w, opt, r);
-    if (rc != DTWJ_OK) return rc;
+    if (int rc = validate(series, n, m, w)) return rc;
+    // ED matrix profile (mpx_v2 with minlag = m, GPU_version1/src/DTWMotifDiscoveryGUI.cu:20-24)
+    const int64_t nw = n - m + 1;
+    std::vector<double> mp((size_t)nw);
+    std::vector<int32_t> mpi((size_t)nw);
+    if (int rc = dtwj_matrix_profile(series, n, m, mp.data(), mpi.data())) return rc;
+    // the reference's seed pairs: window t of the 100 smallest entries and its nearest neighbour
+    const int k = 100;
+    const std::vector<int64_t> win = reference_seed_windows(mp.data(), nw, k);
+    std::vector<int64_t> pi, pj;
+    for (int64_t t : win) {
+        const int64_t p = (int64_t)mpi[(size_t)t] - 1;
+        if (p < 0 || p >= nw) continue;
+        pi.push_back(std::min(t, p));
+        pj.push_back(std::max(t, p));
+    }
+    // FP64 DTW of every seed pair: the seed is min(mp_ed of the first seed window, their DTW)
+    // (GPU_version1/src/new_dtw_motifGUI_malloc.cu:85-106)
+    double seed = mp[(size_t)win[0]];
+    if (!pi.empty()) {
+        std::vector<double> cost(pi.size());
+        if (int rc = dtwj_dtw_pairs(series, n, m, w, pi.data(), pj.data(), (int64_t)pi.size(), 64, cost.data()))
+            return rc;
+        for (double cst : cost) seed = std::min(seed, std::sqrt(cst));
+    }
+    Seeds seeds;
+    seeds.gpu_scan = false;
+    for (size_t q = 0; q < pi.size(); ++q) seeds.pairs.push_back(make_int2((int)pi[q], (int)pj[q]));
+    if (int rc = top1_host(series, n, m, w, seeds, opt, r)) return rc;
+    r->seed_dist = seed;
     // reference result lines (CPU_version1/new_dtw_motifGUI.cpp:575-579,
     // GPU_version1/src/new_dtw_motifGUI_malloc.cu:744-748); indices 1-based there
     std::printf("dtw cnt = %lld\n", (long long)(r->pass_keogh));

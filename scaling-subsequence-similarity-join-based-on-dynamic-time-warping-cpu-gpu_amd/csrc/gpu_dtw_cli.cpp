@@ -1,7 +1,8 @@
 // gpu_dtw <data> <time_out> <m> <r>
 // Drop-in for the reference's GPU_dtw / CPU_dtw command line (README.md:13-34,
 // GPU_version1/src/AAAA.cu:17-36): loads the series, derives w = MAX(round(m*r), 1), runs the
-// top-1 join once, prints the reference's result lines and appends the seconds to <time_out>.
+// reference pipeline once (ED matrix profile, seed from its 100 smallest entries, top-1 join),
+// prints the reference's result lines and appends the seconds to <time_out>.
 // Extension: "w=<int>" as the fourth argument passes the band radius directly.
 //
 // gpu_dtw <data> <time_out> [<m> <r>] --grid

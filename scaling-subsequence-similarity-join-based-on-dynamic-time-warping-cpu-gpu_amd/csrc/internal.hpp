@@ -79,6 +79,14 @@ int rescore_partials();
 int launch_rescore(const JoinArgs &A, const double *sig64, const int4 *cands, int64_t n, float ub, double *cost64,
                    Winner *part, Winner *out, hipStream_t s);
 
+// ED matrix profile (mpx.hip) with exclusion m: mp (FP64 distances) and mpi (1-based partner
+// indices, 0 where no window is m or more away), nw = n - m + 1 entries each; scratch holds
+// mp_scratch_doubles(n, m) doubles; flags[0] / flags[1]: non-finite sample / constant window.
+int64_t mp_scratch_doubles(int64_t n, int32_t m);
+size_t mp_lds_bytes(int32_t m);
+int launch_matrix_profile(const double *d_a, int64_t n, int32_t m, double *scratch, int32_t *flags, double *d_mp,
+                          int32_t *d_mpi, hipStream_t s);
+
 // Fixed per-device workspace of the no-allocation entry point (noalloc.hip): its size, and the
 // device address of `device`'s copy (looked up on first use, then cached).
 size_t noalloc_scratch_bytes();

@@ -197,6 +197,8 @@ def test_cli_grid_matches_reference_grid(gpu, oracle, tmp_path):
         if o.found_in_sweep:
             assert (o.i + 1, o.j + 1) == (int(ref["first_min"]), int(ref["sec_min"])), c
         assert "final distance :%f" % ref["final_distance"] in blk, (c, blk)
+        # the seed from the ED matrix profile, as the reference prints it
+        assert "input of bsf:%5.3f" % ref["input_bsf"] in blk, (c, blk)
     lines = tfile.read_text().split()
     assert len(lines) == 24 and all(float(x) > 0 for x in lines)
 

@@ -152,3 +152,18 @@ def test_ball_bound_used_by_the_join_is_valid(oracle):
         keogh = np.sum(np.maximum(zj - U, 0) ** 2 + np.maximum(L - zj, 0) ** 2)
         assert ball <= keogh + 1e-9
         assert keogh <= oracle.dtw_sq(zi, zj, w) + 1e-9
+
+
+def test_oracle_profile_is_the_brute_force_profile(oracle):
+    """orc_mpx (the reference's mpx_v2 restated) against a brute-force NumPy ED matrix profile
+    with exclusion m: every distance and nearest-neighbour index."""
+    a = np.cumsum(np.random.default_rng(5).standard_normal(1500))
+    m = 64
+    W = np.lib.stride_tricks.sliding_window_view(a, m)
+    Z = (W - W.mean(1, keepdims=True)) / W.std(1, keepdims=True)
+    D2 = np.maximum(2 * m - 2 * (Z @ Z.T), 0.0)
+    idx = np.arange(Z.shape[0])
+    D2[np.abs(idx[:, None] - idx[None, :]) < m] = np.inf
+    mp0, mpi0 = oracle.mpx(a, m)
+    assert np.max(np.abs(mp0 - np.sqrt(D2.min(1))) / np.sqrt(D2.min(1))) < 1e-9
+    assert np.array_equal(mpi0, D2.argmin(1) + 1)
