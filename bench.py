@@ -115,17 +115,7 @@ def batched_leg(d, count):
     """The batched form: `count` float32 random walks (seeds 100.., lengths uniform in
     [8192, 262144], m in {64,128,256,512}, band ratio in {0.01,0.05,0.1}, one planted motif each)
     in one persistent graph-captured launch; autotuned on a first call, then timed."""
-    series, ms, ws = [], [], []
-    for k in range(count):
-        s = 100 + k
-        r = np.random.default_rng(s)
-        n = int(r.integers(8192, 262145))
-        m = int(r.choice([64, 128, 256, 512]))
-        ratio = float(r.choice([0.01, 0.05, 0.1]))
-        a, _, _ = d.generate(n, s, n_plants=1, plant_len=m)
-        series.append(a.astype(np.float32).astype(np.float64))
-        ms.append(m)
-        ws.append(d.band_from_ratio(m, ratio))
+    series, ms, ws = d.batch_series(range(100, 100 + count))
     ctx = d.Context()
     ctx.top1_batch(series, ms, ws, autotune=1)  # tuning + warm-up
     t0 = time.perf_counter()

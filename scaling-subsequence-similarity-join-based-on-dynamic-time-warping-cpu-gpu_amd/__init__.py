@@ -525,6 +525,29 @@ def generate(n, seed, step_std=1.0, step_std_alt=0.0, regime_len=0, n_plants=0, 
     return out, src[:int(n_plants)], dst[:int(n_plants)]
 
 
+def batch_spec(seed):
+    """(n, m, band ratio) of the batched workload's series `seed`: a random walk whose length is
+    uniform in [8192, 262144], m drawn from {64, 128, 256, 512} and the band ratio from
+    {0.01, 0.05, 0.1}, all from numpy's default_rng(seed); one planted motif of length m."""
+    r = np.random.default_rng(seed)
+    n = int(r.integers(8192, 262145))
+    m = int(r.choice([64, 128, 256, 512]))
+    ratio = float(r.choice([0.01, 0.05, 0.1]))
+    return n, m, ratio
+
+
+def batch_series(seeds):
+    """The batched workload's series for the given seeds: (list of FP32-valued series, ms, ws)."""
+    series, ms, ws = [], [], []
+    for s in seeds:
+        n, m, ratio = batch_spec(s)
+        a, _, _ = generate(n, s, n_plants=1, plant_len=m)
+        series.append(a.astype(np.float32).astype(np.float64))
+        ms.append(m)
+        ws.append(band_from_ratio(m, ratio))
+    return series, ms, ws
+
+
 class Context:
     """A device context that keeps buffers between joins (for repeated, device-resident runs)."""
 

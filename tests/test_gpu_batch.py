@@ -90,3 +90,21 @@ def test_bad_series_is_rejected_alone(gpu):
     for t in (0, 3):
         r1 = gpu.top1(series[t], ms[t], ws[t])
         assert (res[t].i, res[t].j) == (r1.i, r1.j)
+
+
+def test_subset_of_the_512_series_workload_matches_single_series(gpu):
+    """64 series of the 512-series batched workload (seeds 100..155 and the first eight more of
+    its m = 512, band ratio 0.1 members, w = 51, the batch's heaviest class), in one batch:
+    every series equal to its own single-series join (pair, FP64 distance, pairs swept)."""
+    heavy = [s for s in range(156, 612) if gpu.batch_spec(s)[1:] == (512, 0.1)][:8]
+    seeds = list(range(100, 156)) + heavy
+    series, ms, ws = gpu.batch_series(seeds)
+    assert sum(1 for m, w in zip(ms, ws) if (m, w) == (512, 51)) >= 10
+    res = gpu.top1_batch(series, ms, ws)
+    for t, (a, m, w) in enumerate(zip(series, ms, ws)):
+        rb = res[t]
+        assert not isinstance(rb, Exception), (seeds[t], rb)
+        r1 = gpu.top1(a, m, w)
+        assert (rb.i, rb.j) == (r1.i, r1.j), (seeds[t], m, w)
+        assert abs(rb.dist - r1.dist) <= 1e-9 * r1.dist
+        assert rb.pairs_swept == r1.pairs_swept == r1.pairs_total

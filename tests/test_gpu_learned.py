@@ -174,3 +174,23 @@ def test_learned_join_is_exact_and_ranks_better_than_ed(gpu):
         np.mean(rho_l), np.mean(rho_e), np.mean(top_l), np.mean(top_e)))
     assert np.mean(rho_l) > np.mean(rho_e) + 0.02
     assert np.mean(top_l) < np.mean(top_e)
+
+
+@pytest.mark.parametrize("name", ["n16384_m256_w13", "n16384_m256_w26", "n16384_m256_w3", "n16384_m256_w2"])
+def test_committed_model_join_matches_the_reference(gpu, oracle, name):
+    """The learned join with the shipped network (models/proposal_m256_w13.json, trained on
+    generator seeds 1000..1099) on the m = 256 golden series: the compiled reference's pair and
+    distance (tests/golden/golden.json) and the oracle's FP64 distance."""
+    import os
+    from conftest import PKG_DIR, golden_cases, golden_series
+    c = golden_cases()[name]
+    a = golden_series(name)
+    model = gpu.load_mlp(os.path.join(PKG_DIR, "models", "proposal_m256_w13.json"))
+    r, rep = gpu.top1_learned(a, c["m"], c["w"], model, n_items=4096, n_verify=256)
+    o = oracle.motif_discovery(a, c["m"], c["w"])
+    assert (r.i, r.j) == (o.i, o.j)
+    assert abs(r.dist - o.bsf) <= 1e-9 * o.bsf
+    assert abs(r.dist - c["ref"]["final_distance"]) <= 5e-7 + 1e-5 * r.dist
+    if o.found_in_sweep:
+        assert (r.i + 1, r.j + 1) == (int(c["ref"]["first_min"]), int(c["ref"]["sec_min"]))
+    assert rep["verified"] > 0 and rep["seed_dist"] >= r.dist * (1 - 1e-9)
