@@ -7,16 +7,27 @@ motif pairs of length 256, m = 256, w = 2 (band ratio 0.01 as the configuration 
 the reference's rounding rule gives w = 3, available with --w 3).
 
 Prints one JSON line.  value = mean time-to-exact-top-1 in seconds (lower is better).
-roofline: the fused join kernel is FP32-VALU-bound; achieved = algorithmic FP32 operations
-of the launch (pairs swept x 12 two-sided or x 6 one-sided, ball survivors x 84, LB_Keogh
-positions x 20, DTW cells x 5)
-/ its HIP-event time, peak 157.3 TFLOP/s.  traffic = HBM bytes per launch from the rocprofv3
-FETCH_SIZE / WRITE_SIZE summary under profiles/ for the same workload (null otherwise).
-A second workload (band ratio 0.05: seed 3, 5 plants, w = 13) is timed beside the headline one
-and reported under "band_ratio_0_05"; the same workload with the learned proposal against the plain
-join under "learned"; the 512-series batched form under "batched".
+roofline: the fused join kernel is FP32-VALU-bound; achieved = FP32 operations the launch
+executes (pairs swept x 12 two-sided or x 6 one-sided, ball survivors x 84, LB_Keogh positions
+x 20, DTW cells x 5; FMA = 2) / its HIP-event time, peak 157.3 TFLOP/s.  The scope table's unit
+of 22 operations for every swept pair (both ball directions) would credit work the kernel does
+not do: at this band 99 % of the pairs take the one-sided test (6 operations) and the other
+direction follows in LB_Keogh for the few survivors; that figure is reported beside it
+(frac_at_22_ops_per_swept_pair, above 1).  valu_issue_frac: VALU instructions of the launch
+(SQ_INSTS_VALU, profiles/k_join_counters.json) over the issue slots of its kernel time (256 CUs
+x 4 SIMDs, one wave64 instruction per 4 cycles at 2.4 GHz; packed FP32 is not used).
+traffic = HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary under
+profiles/ for the same workload (null otherwise); prep_kernels: HBM rates of the preprocessing
+kernels from the committed kernel-trace summary.
+Other legs: the band ratio 0.05 workload (seed 3, 5 plants, w = 13) under "band_ratio_0_05";
+the same workload with the learned proposal against the plain join under "learned"; the
+512-series batched form under "batched"; the reference pipeline (GPU ED matrix profile,
+reference seed, seeded join) under "reference_pipeline"; the wide-band workload (n = 4M,
+m = 1024, w = 102, regime shifts) under "wide_band_m1024".
 cpu_baseline: the repository's FP64 CPU restatement of the reference join (oracle/, the
-"port"), single-threaded, on a bounded prefix of the same series, extrapolated by pair count.
+"port"), single-threaded, on a bounded prefix of the same series, extrapolated by pair count,
+with the host CPU model, and the compiled reference's own measured run of this workload where
+the fixtures hold one.
 """
 import argparse
 import importlib.util
@@ -37,6 +48,11 @@ OPS_KIM_PAIR = 84        # 12 z values (FMA) + 18 squared differences + 10 min/a
 OPS_KEOGH_ELEM = 20      # per position, both directions: 2 FMA envelope, 2 sub, 2 max, 1 FMA
 OPS_DTW_CELL = 5         # sub, mul, add, 2-way min of three
 PEAK_FP32 = 157.3e12     # MI355X FP32, dense
+
+
+def log(msg):
+    """Progress line on stderr (stdout carries only the JSON line)."""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def load_traffic(cfg):
@@ -144,9 +160,45 @@ def pairs_of(n, m):
     return K * (K + 1) // 2
 
 
-def cpu_baseline(series, m, w, budget_s):
-    """Times the oracle (restated reference CPU join with its ED matrix-profile seed) on the
-    largest prefix that fits the budget, then extrapolates to the full series by pair count."""
+def host_cpu():
+    """Model name and logical CPU count of the host (Linux /proc/cpuinfo)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def reference_cpu_measured(cfg):
+    """The compiled reference CPU path's own run on this exact workload, if the fixture holds one
+    (tests/golden/fullsize.json, measured in the build container by tests/golden/make_fullsize.py)."""
+    path = os.path.join(ROOT, "tests", "golden", "fullsize.json")
+    try:
+        with open(path) as f:
+            db = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for name, e in db.items():
+        if (e.get("n"), e.get("seed"), e.get("m"), e.get("w")) == (cfg["n"], cfg["seed"], cfg["m"], cfg["w"]) \
+                and "ref" in e:
+            r = e["ref"]
+            return {"case": name, "join_s": r.get("ref_seconds"), "process_wall_s": r.get("wall_seconds"),
+                    "threads": r.get("threads", 1), "where": "build container (Intel Xeon, 1 thread), not the GPU box",
+                    "note": "the reference's own timer covers the join after its ED matrix profile; the wall time "
+                            "covers the whole process"}
+    return None
+
+
+def cpu_baseline(series, m, w, budget_s, cfg=None):
+    """Times the oracle (the repository's FP64 restatement of the reference CPU join with its ED
+    matrix-profile seed; it leaves out the reference's alpha-tuned local masks and LB_Petitjean,
+    which only prune) on the largest prefix that fits the budget, single-threaded on this host,
+    and extrapolates to the full series by pair count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_ctypes as oc
     oc.lib()
@@ -159,10 +211,79 @@ def cpu_baseline(series, m, w, budget_s):
             break
         n_s *= 2
     full = dt * pairs_of(series.size, m) / pairs_of(n_s, m)
-    return {"value": full, "unit": "s (extrapolated to the full workload by pair count)", "cores": 1,
-            "kind": "port", "sample": "first %d samples of the workload series, m=%d, w=%d: %.2f s measured "
-            "(ED matrix-profile seed + join, FP64, single thread)" % (n_s, m, w, dt),
-            "sample_seconds": dt, "sample_pairs_per_s": pairs_of(n_s, m) / dt}
+    model, ncpu = host_cpu()
+    out = {"value": full, "unit": "s (extrapolated to the full workload by pair count)", "cores": 1,
+           "kind": "port", "sample": "first %d samples of the workload series, m=%d, w=%d: %.2f s measured "
+           "(ED matrix-profile seed + join, FP64, single thread; the restatement omits the reference's "
+           "alpha-tuned local masks and LB_Petitjean)" % (n_s, m, w, dt),
+           "sample_seconds": dt, "sample_pairs_per_s": pairs_of(n_s, m) / dt,
+           "sample_ns_per_pair": 1e9 * dt / pairs_of(n_s, m), "host_cpu": model, "host_logical_cpus": ncpu,
+           "reference_ns_per_pair_container": 16.0}
+    if cfg is not None:
+        out["reference_measured"] = reference_cpu_measured(cfg)
+    return out
+
+
+# MI355X issue model of the VALU: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4 cycles
+# at 2.4 GHz (packed FP32 disabled in this build, so one FP32 operation per lane and instruction)
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
+
+
+def load_counters(cfg):
+    """Instruction counters of k_join for this exact workload (profiles/k_join_counters.json,
+    rocprofv3 --pmc), or None."""
+    path = os.path.join(ROOT, "profiles", "k_join_counters.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = t.get("workload", {})
+    if all(w.get(k) == cfg.get(k) for k in ("n", "m", "w", "seed", "planted_pairs")):
+        return t
+    return None
+
+
+def load_prep_kernels(cfg):
+    """Per-kernel average durations of the preprocessing for this workload
+    (profiles/prep_kernels.json, rocprofv3 --kernel-trace --stats) with their bytes, or None."""
+    path = os.path.join(ROOT, "profiles", "prep_kernels.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = t.get("workload", {})
+    if not all(w.get(k) == cfg.get(k) for k in ("n", "m", "w", "seed", "planted_pairs")):
+        return None
+    out = {}
+    for name, e in t.get("kernels", {}).items():
+        sec = e["avg_ns"] * 1e-9
+        out[name] = {"avg_us": e["avg_ns"] * 1e-3, "bytes_moved": e["bytes_moved"],
+                     "hbm_gb_s": e["bytes_moved"] / sec / 1e9, "hbm_frac": e["bytes_moved"] / sec / 8e12}
+        if e.get("bytes_algorithmic"):
+            out[name]["algorithmic_bytes"] = e["bytes_algorithmic"]
+            out[name]["algorithmic_gb_s"] = e["bytes_algorithmic"] / sec / 1e9
+    return out
+
+
+def pipeline_leg(d, n, m, w, seed, plants):
+    """The reference pipeline as gpu_dtw runs it (DTWMotifDiscoveryGUI): the ED matrix profile on
+    the GPU, the reference's seed from its 100 smallest entries, then the seeded join; inputs in
+    host memory (as the process receives them)."""
+    a, _, _ = d.generate(n, seed, n_plants=plants, plant_len=m)
+    d.matrix_profile(a[: 4 * m], m)  # warm-up (module load)
+    t0 = time.perf_counter()
+    mp, mpi = d.matrix_profile(a, m)
+    t1 = time.perf_counter()
+    r = d.top1_seeded(a, m, w, mp, mpi)
+    t2 = time.perf_counter()
+    K = n - 2 * m + 1
+    return {"config": {"workload": "reference_pipeline_mp_seed_join", "n": n, "m": m, "w": w, "seed": seed,
+                       "planted_pairs": plants},
+            "matrix_profile_s": t1 - t0, "matrix_profile_pairs_per_s": K * (K + 1) / 2 / (t1 - t0),
+            "seeded_join_s": t2 - t1, "total_s": t2 - t0, "seed_dist": r.seed_dist,
+            "top1": {"i": r.i, "j": r.j, "dist": r.dist}}
 
 
 def main():
@@ -177,12 +298,13 @@ def main():
     ap.add_argument("--plants", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-wide-band", action="store_true")
+    ap.add_argument("--no-w13", action="store_true", help="skip the band ratio 0.05 workload (w = 13)")
+    ap.add_argument("--no-wide-band", action="store_true",
+                    help="skip the wide-band workload (n=4M, m=1024, w=102, regime shifts; about a minute)")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-pipeline", action="store_true", help="skip the matrix-profile-seeded reference pipeline")
     ap.add_argument("--batch-count", type=int, default=512)
-    ap.add_argument("--wide-1024", action="store_true",
-                    help="also run the wide-band workload once (n=4M, m=1024, w=102; about 2 minutes)")
     ap.add_argument("--rows-per-item", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     args = ap.parse_args()
@@ -200,9 +322,25 @@ def main():
     ops = ((r.pairs_swept - r.pairs_swept_one_sided) * OPS_SWEPT_PAIR + r.pairs_swept_one_sided * OPS_SWEPT_ONE +
            r.pass_ball * OPS_KIM_PAIR + r.keogh_elems * OPS_KEOGH_ELEM + r.dtw_cells * OPS_DTW_CELL)
     achieved = ops / (kern_ms * 1e-3)
+    # the scope table's unit counts 22 operations for every swept pair (both ball directions);
+    # the kernel tests 99 % of the pairs one-sided (the other direction follows in LB_Keogh), so
+    # that unit over-credits the launch: reported beside the executed-operation figure
+    ops_table_unit = (r.pairs_swept * 22 + r.pass_ball * OPS_KIM_PAIR + r.keogh_elems * OPS_KEOGH_ELEM +
+                      r.dtw_cells * OPS_DTW_CELL)
     config = {"workload": "top1_dtw_join_plain", "n": args.n, "m": args.m, "w": args.w, "seed": args.seed,
               "planted_pairs": args.plants, "parallelism": "single GPU, persistent grid"}
     traffic = load_traffic(config)
+    counters = load_counters(config)
+    roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32, "traffic": traffic, "kernel": "k_join", "kernel_ms": kern_ms,
+                "ops_unit": "executed: two-sided pair 12, one-sided pair 6, ball survivor 84, LB_Keogh position 20, "
+                            "DTW cell 5 (FMA = 2)",
+                "frac_at_22_ops_per_swept_pair": ops_table_unit / (kern_ms * 1e-3) / PEAK_FP32}
+    if counters:
+        # fraction of the VALU issue slots the launch used (SQ_INSTS_VALU from the committed profile
+        # of this workload, over this run's kernel time)
+        roofline["valu_issue_frac"] = counters["sq_insts_valu"] / (kern_ms * 1e-3) / VALU_ISSUE_PER_S
+        roofline["valu_instructions_per_pair"] = counters["sq_insts_valu"] * 64 / r.pairs_total
     line = {
         "metric": "time_to_exact_top1_s",
         "value": sec,
@@ -230,11 +368,11 @@ def main():
         "sweep_one_sided_frac": r.pairs_swept_one_sided / r.pairs_swept,
         "seconds": {"prep": r.t_prep, "seed": r.t_seed, "join": r.t_join, "verify": r.t_verify,
                     "join_kernel": kern_ms * 1e-3},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": traffic, "kernel": "k_join",
-                     "kernel_ms": kern_ms},
+        "roofline": roofline,
+        "prep_kernels": load_prep_kernels(config),
     }
-    if not args.no_wide_band:
+    log("headline done; next: band ratio 0.05 leg")
+    if not args.no_w13:
         _, res5, walls5, planted5 = run_workload(d, args.n, args.m, 13, 3, 5, args.steps, args.warmup, opts)
         r5 = res5[-1]
         k5 = float(np.mean([x.join_kernel_ms for x in res5]))
@@ -249,28 +387,47 @@ def main():
             "pass_rates": {"ball": r5.pass_ball / r5.pairs_swept, "kim": r5.pass_kim / r5.pairs_swept,
                            "keogh_to_dtw": r5.pass_keogh / r5.pairs_swept},
         }
+    log("next: learned leg")
     if not args.no_learned:
         line["learned"] = learned_leg(d, args.n, args.m, args.steps, args.warmup,
                                       0.0 if args.no_cpu_baseline else args.cpu_budget)
+    log("next: batched leg")
     if not args.no_batched:
         line["batched"] = batched_leg(d, args.batch_count)
-    if args.wide_1024:
+    log("next: reference pipeline leg")
+    if not args.no_pipeline:
+        line["reference_pipeline"] = pipeline_leg(d, args.n, args.m, d.band_from_ratio(args.m, 0.01), args.seed,
+                                                  args.plants)
+    log("next: wide-band leg")
+    if not args.no_wide_band:
         # random walk n = 4M, seed 2, step std alternating 0.5 / 2.0 every 100k samples
         wa, _, _ = d.generate(4_000_000, 2, step_std=0.5, step_std_alt=2.0, regime_len=100_000)
+        ctx = d.Context()
+        ctx.upload(wa)
         t0 = time.perf_counter()
-        rw = d.top1(wa, 1024, 102)
+        rw = ctx.top1(1024, 102)
         tw = time.perf_counter() - t0
+        ctx.close()
+        kw = rw.join_kernel_ms * 1e-3
+        opsw = (rw.pairs_swept - rw.pairs_swept_one_sided) * OPS_SWEPT_PAIR + rw.pairs_swept_one_sided * \
+            OPS_SWEPT_ONE + rw.pass_ball * OPS_KIM_PAIR + rw.keogh_elems * OPS_KEOGH_ELEM + rw.dtw_cells * OPS_DTW_CELL
         line["wide_band_m1024"] = {
             "config": {"workload": "top1_dtw_join_plain", "n": 4_000_000, "m": 1024, "w": 102, "seed": 2,
                        "regimes": "step std 0.5 / 2.0, 100k samples each"},
-            "time_to_exact_top1_s": tw, "join_kernel_s": rw.join_kernel_ms * 1e-3,
-            "top1": {"i": rw.i, "j": rw.j, "dist": rw.dist, "dist_fp32": rw.dist_fp32},
-            "dtw_cells": rw.dtw_cells, "dtw_cells_per_s": rw.dtw_cells / (rw.join_kernel_ms * 1e-3),
+            "time_to_exact_top1_s": tw, "join_kernel_s": kw,
+            "top1": {"i": rw.i, "j": rw.j, "dist": rw.dist, "dist_fp32": rw.dist_fp32,
+                     "fp32_rel_err": abs(rw.dist_fp32 - rw.dist) / max(rw.dist, 1e-300)},
+            "pairs_total": rw.pairs_total, "pairs_per_s": rw.pairs_total / tw,
+            "dtw_cells": rw.dtw_cells, "dtw_cells_per_s": rw.dtw_cells / kw,
+            "keogh_elems": rw.keogh_elems,
             "pass_rates": {"ball": rw.pass_ball / rw.pairs_swept, "kim": rw.pass_kim / rw.pairs_swept,
                            "keogh_to_dtw": rw.pass_keogh / rw.pairs_swept},
+            "roofline": {"bound": "valu", "achieved": opsw / kw / 1e12, "peak": PEAK_FP32 / 1e12,
+                         "unit": "TFLOP/s", "frac": opsw / kw / PEAK_FP32},
         }
+    log("next: cpu baseline")
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(series, args.m, args.w, args.cpu_budget)
+        line["cpu_baseline"] = cpu_baseline(series, args.m, args.w, args.cpu_budget, config)
     print(json.dumps(line), flush=True)
 
 

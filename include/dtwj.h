@@ -237,7 +237,7 @@ int dtwj_context_top1_batch(dtwj_context *ctx, const dtwj_series *series, int32_
  * DTWMotifDiscoveryGUI calls it (GPU_version1/src/DTWMotifDiscoveryGUI.cu:20-24): mp_ed[t] =
  * z-normalised Euclidean distance of window t to its nearest window p with |t - p| >= m,
  * mp_idx_1based[t] = p + 1 (0 and sqrt(4m) where no window is that far).  n - m + 1 entries each.
- * Partners whose squared distances agree to FP32 precision count as tied: the smallest lag |t - p|
+ * Partners whose squared distances agree to 2^-21 relative count as tied: the smallest lag |t - p|
  * wins, the left one at equal lags (the reference's visiting order); the distance is the chosen
  * pair's own FP64 value.  Non-finite samples and constant windows return DTWJ_EINVAL. */
 int dtwj_matrix_profile(const double *series, int64_t n, int32_t m, double *mp_ed, int32_t *mp_idx_1based);

@@ -16,15 +16,16 @@
 // four diagonals are anchored exactly at the first row (windows staged in LDS), then carried
 // down the rows with the update above; the column terms rotate through registers (slot t of a
 // lane holds column r0 + d0 + 4 lane + t of a 4-row block starting at row r0; the four columns
-// entering a block are one contiguous 32-byte load per table and lane).  Each row's minimum over
-// the task's diagonals goes into an LDS array of row keys (FP32 bits of ED^2 << 32 | partner:
-// smaller distance first, then the smaller partner index), flushed to the profile with one
-// 64-bit atomicMin per row and task.  One sweep gives every window its nearest partner to the
+// entering a block are read by each lane from memory, where they are L1 hits).  Each row's minimum over
+// the task's diagonals goes into an LDS array of row keys (FP32 bits of ED^2, the lowest two
+// cleared, << 32 | partner: smaller distance first, then the smaller partner index), flushed to
+// the profile with one 64-bit atomicMin per row and task.  One sweep gives every window its nearest partner to the
 // right (p >= t + m); the same sweep over the reversed series gives the partners to the left.
 // A final pass merges both and recomputes the distance of the chosen pair exactly by a direct
 // FP64 dot product.
 //
-// Tie rule: partners whose ED^2 round to the same FP32 value are tied, and the one with the
+// Tie rule: partners whose ED^2 agree in the FP32 bits kept (22 of the 23 fraction bits: a
+// relative difference below 2^-21) are tied, and the one with the
 // smallest lag |t - p| is taken, the left one (p < t) at equal lags: the partner the reference
 // meets first in its diagonal order among exactly equal correlations (CPU_version1/mpx_v2.cpp:
 // diagonals in increasing lag, and at one lag window t as the column before t as the row).  The
@@ -178,37 +179,48 @@ __global__ __launch_bounds__(64) void k_mpx(MpSweep S) {
                 rdg[u] = S.dg[i];
                 ris[u] = S.isig[i];
             }
+            // The block's four rows.  Each distance becomes a key: its FP32 bits with the two lowest
+            // bits replaced by the diagonal q of the lane, so that one FP32 minimum per pair keeps the
+            // smallest distance and, among distances equal in the other 30 bits, the smallest q
+            // (the partner index is rebuilt from the winning lane and q).  Pairs past the last
+            // window (only in blocks at the triangle's edge) get +inf.
+            const bool full = r0 + 3 < i_end && r0 + 3 + d0 + MP_DG - 1 <= nw - 1;  // uniform
+            const int64_t jl0 = d0 + MP_KD * lane;  // j - i of the lane's first diagonal
+            float best[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int64_t i = r0 + u;
-                if (i >= i_end) break;  // uniform
-                float best = __builtin_huge_valf();
-                int bj = -1;
+                const double s2 = -2.0 * ris[u];
+                best[u] = __builtin_huge_valf();
 #pragma unroll
                 for (int q = 0; q < MP_KD; ++q) {
-                    const int64_t j = i + d0 + MP_KD * lane + q;
-                    const double e2 = fma(-2.0 * ris[u], cov[q] * cis[u + q], m2);
-                    const float e = (float)(e2 > 0.0 ? e2 : 0.0);
-                    if (j <= nw - 1 && e < best) {
-                        best = e;
-                        bj = (int)j;
-                    }
+                    const double e2 = fma(s2, cov[q] * cis[u + q], m2);
+                    const float e = fmaxf((float)e2, 0.f);
+                    float key = __uint_as_float((__float_as_uint(e) & ~3u) | (uint32_t)q);
+                    if (!full && !(i < i_end && i + jl0 + q <= nw - 1)) key = __builtin_huge_valf();
+                    best[u] = fminf(best[u], key);
                     cov[q] = fma(rdf[u], cdg[u + q], cov[q]);
                     cov[q] = fma(cdf[u + q], rdg[u], cov[q]);
                 }
-                // row minimum over the wave (smallest distance, then smallest partner: lanes hold
-                // increasing diagonals, and each lane kept its first minimum)
-                float v = best;
+            }
+            float v[4];
 #pragma unroll
-                for (int off = 1; off < 64; off <<= 1) v = fminf(v, __shfl_xor(v, off, 64));
-                const unsigned long long hit = __builtin_amdgcn_ballot_w64(bj >= 0 && best == v);
-                if (hit) {
-                    const int src = (int)__builtin_ctzll(hit);
-                    const int jj = __shfl(bj, src, 64);
-                    if (lane == 0)
-                        atomicMin(&rk[i - i0],
-                                  ((unsigned long long)__float_as_uint(v) << 32) | (unsigned long long)(uint32_t)jj);
-                }
+            for (int u = 0; u < 4; ++u) v[u] = best[u];
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = fminf(v[u], __shfl_xor(v[u], off, 64));
+            }
+            // the lowest lane holding the minimum key has the smallest partner among equal keys
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (!(v[u] < __builtin_huge_valf())) continue;  // uniform: no valid pair in the row
+                const unsigned long long hit = __builtin_amdgcn_ballot_w64(best[u] == v[u]);
+                const int src = (int)__builtin_ctzll(hit);
+                const uint32_t kb = __float_as_uint(v[u]);
+                const int64_t jj = r0 + u + d0 + MP_KD * src + (kb & 3u);
+                if (lane == 0)
+                    atomicMin(&rk[r0 + u - i0], ((unsigned long long)(kb & ~3u) << 32) | (unsigned long long)(uint32_t)jj);
             }
 #pragma unroll
             for (int t = 0; t < 4; ++t) {

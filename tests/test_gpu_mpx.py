@@ -5,8 +5,8 @@ profile's 100 smallest entries seed the join and are printed as "input of bsf").
 Oracles: orc_mpx (the reference's mpx_v2 restated, FP64), itself pinned here by a brute-force
 NumPy profile on a small series; the compiled reference's printed seed and result lines in the
 golden fixtures.  Tolerances: profile values relative 1e-9 where the partner is the oracle's;
-partner indices equal except where two partners' squared distances agree to FP32 precision
-(both values then within 1e-7 relative); printed lines identical.
+partner indices equal except where two partners' squared distances agree to 2^-21 relative
+(both distances then within 5e-7 relative); printed lines identical.
 """
 import re
 import subprocess
@@ -35,8 +35,8 @@ def check_profile(a, m, mp, mpi, mp0, mpi0):
     same = mpi == mpi0
     rel = np.abs(mp - mp0) / np.maximum(mp0, 1e-6)
     assert np.all(rel[same] <= 1e-9), rel[same].max()
-    # a different partner only where the two are tied at FP32 precision
-    assert np.all(rel[~same] <= 1e-7), rel[~same].max()
+    # a different partner only where the two are tied at the sweep's key precision (2^-21 in ED^2)
+    assert np.all(rel[~same] <= 5e-7), rel[~same].max()
     assert (~same).mean() <= 1e-3
     nw = a.size - m + 1
     none = mpi0 == 0  # no window m or more away (n < 3m - 1): sqrt(4m), index 0
