@@ -450,6 +450,17 @@ int fetch_error_model(Job &J, const JoinArgs *d_args, hipStream_t s) {
     return 0;
 }
 
+// Claim-order multiplier of a work list of `total` items: near the golden-ratio fraction of the
+// list and coprime with its length (claim k -> item k * mul mod total is then a bijection that
+// spreads the first claims over the whole list).  0 (index order) for lists of 2^32 items and
+// more, where the 64-bit product could overflow.
+int64_t claim_multiplier(int64_t total) {
+    if (total < 3 || total >= ((int64_t)1 << 32)) return 0;
+    int64_t mul = (int64_t)((double)total * 0.6180339887498949) | 1;
+    while (std::gcd(mul, total) != 1) mul += 2;
+    return mul % total;
+}
+
 int32_t grid_for(const dtwj_context *c, int32_t wpc, int64_t n_items) {
     wpc = wpc > 0 ? wpc : 8;
     int64_t grid = (int64_t)c->num_cus * wpc;
@@ -687,7 +698,7 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
         auto carve = [&](Arena &ar) {
             take_job(ar, J);
             d_args = ar.take<JoinArgs>(1);
-            d_item_base = ar.take<int64_t>(2);
+            d_item_base = ar.take<int64_t>(3);
             d_work = ar.take<unsigned long long>(1);
             d_queue = ar.take<uint32_t>((size_t)grid * kJoinQueueCap);
             d_cost64 = ar.take<double>((size_t)J.cand_cap);
@@ -705,7 +716,7 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
         hipStream_t s = c->stream;
         const auto t0 = clk::now();
         if (int rc = upload_job(J, s)) return rc;
-        const int64_t item_base[2] = {0, J.n_items};
+        const int64_t item_base[3] = {0, J.n_items, claim_multiplier(J.n_items)};
         DTWJ_HIP(hipMemcpyAsync(d_item_base, item_base, sizeof(item_base), hipMemcpyHostToDevice, s));
         // ---- preprocessing
         if (int rc = prep_series(J.P, s)) return rc;
@@ -830,7 +841,7 @@ int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const d
             take_job(ar, jobs[t]);
         }
         d_args = ar.take<JoinArgs>(nj);
-        d_item_base = ar.take<int64_t>(nj + 1);
+        d_item_base = ar.take<int64_t>(nj + 2);
         d_work = ar.take<unsigned long long>(1);
         d_queue = ar.take<uint32_t>((size_t)grid * kJoinQueueCap);
         d_cost64 = ar.take<double>(kBatchCandCap);
@@ -907,7 +918,8 @@ int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const d
         hargs[t] = jobs[t].A;
     }
     DTWJ_HIP(hipMemcpyAsync(d_args, hargs.data(), sizeof(JoinArgs) * nj, hipMemcpyHostToDevice, s));
-    DTWJ_HIP(hipMemcpyAsync(d_item_base, item_base.data(), sizeof(int64_t) * (nj + 1), hipMemcpyHostToDevice, s));
+    item_base.push_back(claim_multiplier(item_base[nj]));
+    DTWJ_HIP(hipMemcpyAsync(d_item_base, item_base.data(), sizeof(int64_t) * (nj + 2), hipMemcpyHostToDevice, s));
     DTWJ_HIP(hipStreamSynchronize(s));
 
     auto enqueue_all = [&]() -> int {
@@ -1060,7 +1072,7 @@ int dtwj_plan(int64_t n, int32_t m, int32_t w, const dtwj_options *opt, dtwj_pla
     const int64_t grid = std::min<int64_t>((int64_t)256 * wpc, std::max<int64_t>(J.n_items, 1));
     auto pad = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t bytes = job_bytes(J);
-    bytes += pad(sizeof(JoinArgs)) + pad(16) + pad(8) + pad(4 * (size_t)grid * kJoinQueueCap) +
+    bytes += pad(sizeof(JoinArgs)) + pad(24) + pad(8) + pad(4 * (size_t)grid * kJoinQueueCap) +
              pad(8 * (size_t)J.cand_cap) + pad(sizeof(Winner) * (size_t)rescore_partials()) + pad(sizeof(Winner));
     out->pairs_total = J.K * (J.K + 1) / 2;
     out->items = J.n_items;

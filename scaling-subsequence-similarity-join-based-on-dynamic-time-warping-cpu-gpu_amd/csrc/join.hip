@@ -273,7 +273,8 @@ __device__ inline int pair_row(uint32_t p) { return (int)(p & 0xffffu); }
 __device__ inline int pair_col(uint32_t p) { return (int)(p >> 16); }
 
 // series: nseries join descriptors; item_base: nseries + 1 cumulative item counts (the
-// device work queue is the range [0, item_base[nseries]) claimed through work_counter);
+// device work queue is the range [0, item_base[nseries]) claimed through work_counter), then
+// the claim-order multiplier;
 // uni: floats of the LDS union area, sized for the longest window of the launch.
 template <int Q>
 __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base,
@@ -829,9 +830,15 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
         // 64-bit claim counter: the item count of a long series exceeds 2^31
         unsigned long long item_l = 0;
         if (lane == 0) item_l = atomicAdd(work_counter, 1ull);
-        const int64_t gitem = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(item_l >> 32)) << 32) |
+        const int64_t claim = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(item_l >> 32)) << 32) |
                                         (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)item_l));
-        if (gitem >= item_base[nseries]) break;
+        const int64_t total = item_base[nseries];
+        if (claim >= total) break;
+        // claim order: claim k takes item k * mul mod total (mul coprime with total, a bijection),
+        // so that the first claims are spread over the whole pair space and the best-so-far falls
+        // early wherever the best pairs lie; mul = 0 keeps the index order
+        const uint64_t mul = (uint64_t)item_base[nseries + 1];
+        const int64_t gitem = mul ? (int64_t)(((uint64_t)claim * mul) % (uint64_t)total) : claim;
         // series of this item (usually the one of the previous item)
         int s = cur;
         if (s < 0 || gitem < item_base[s] || gitem >= item_base[s + 1]) {
