@@ -88,6 +88,32 @@ def run_workload(d, n, m, w, seed, plants, steps, warmup, opts):
     return series, results, walls, planted
 
 
+def prep_sweep(d, n, seed):
+    """--prep-sweep: the preprocessing building blocks alone on the headline series (envelope at
+    w = 2, 13, 102, 511; sliding statistics at m = 256, 1024), three calls each, for a kernel-trace
+    profile of k_envelope and the statistics kernels across band widths.  Prints one JSON line of
+    host wall times (upload and download included); the kernel durations come from the profiler."""
+    series, _, _ = d.generate(n, seed, n_plants=3, plant_len=256)
+    out = {"mode": "prep_sweep", "n": n, "seed": seed, "envelope_s": {}, "sliding_stats_s": {}}
+    for w in (2, 13, 102, 511):
+        t = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            d.envelope(series, w)
+            t.append(time.perf_counter() - t0)
+        out["envelope_s"][str(w)] = min(t)
+        log(f"envelope w={w} done")
+    for m in (256, 1024):
+        t = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            d.sliding_stats(series, m)
+            t.append(time.perf_counter() - t0)
+        out["sliding_stats_s"][str(m)] = min(t)
+        log(f"sliding statistics m={m} done")
+    print(json.dumps(out), flush=True)
+
+
 def learned_leg(d, n, m, steps, warmup, cpu_budget=0.0):
     """Learned proposal vs the plain GPU join on the band-ratio-0.05 workload (seed 3, 5 planted
     motifs, w = 13), both from host memory, with the committed trained network; with a CPU
@@ -307,10 +333,15 @@ def main():
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--rows-per-item", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--prep-sweep", action="store_true",
+                    help="time only the envelope / statistics building blocks across w and m, then exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not supported", file=sys.stderr)
         sys.exit(2)
+    if args.prep_sweep:
+        prep_sweep(load_pkg(), args.n, args.seed)
+        return
 
     d = load_pkg()
     opts = dict(rows_per_item=args.rows_per_item, waves_per_cu=args.waves_per_cu)

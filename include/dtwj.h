@@ -274,7 +274,9 @@ int32_t dtwj_device_count(void);
 int dtwj_sliding_stats(const double *series, int64_t n, int32_t m, double *mu, double *sigma);
 
 /* Lower/upper envelope of radius w clipped at the series ends, computed on the GPU.
- * Reference: lower_upper_lemire (CPU_version1/allmatrixop/lower_upper_lemire.cpp:7-43). */
+ * Reference: lower_upper_lemire (CPU_version1/allmatrixop/lower_upper_lemire.cpp:7-43).
+ * min(w, n - 1) must be <= 1536 (a radius of n - 1 or more spans the whole series), else
+ * DTWJ_EINVAL. */
 int dtwj_envelope(const double *series, int64_t n, int32_t w, double *lower, double *upper);
 
 /* Banded DTW (squared cost D[m-1][m-1], not its square root) of z-normalised windows for

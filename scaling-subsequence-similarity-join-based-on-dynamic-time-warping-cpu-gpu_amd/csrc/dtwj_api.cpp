@@ -1561,6 +1561,9 @@ int dtwj_sliding_stats(const double *series, int64_t n, int32_t m, double *mu, d
 
 int dtwj_envelope(const double *series, int64_t n, int32_t w, double *lower, double *upper) {
     if (!series || !lower || !upper || w < 0 || n < 2) return fail(DTWJ_EINVAL, "bad arguments");
+    // a radius of n - 1 already spans the whole series from every position
+    if ((int64_t)w > n - 1) w = (int32_t)(n - 1);
+    if (w > kMaxEnvelopeRadius) return fail(DTWJ_EINVAL, "envelope radius min(w, n - 1) must be <= 1536");
     dtwj_context *c = nullptr;
     Prep P;
     const double *d_a = nullptr;

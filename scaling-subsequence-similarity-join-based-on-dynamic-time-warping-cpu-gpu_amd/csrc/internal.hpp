@@ -32,6 +32,9 @@ struct Prep {
 };
 
 int64_t prep_scan_blocks(int64_t n);
+// Largest envelope radius the preprocessing kernels take (their LDS tile holds 1024 + 2w doubles
+// twice); a join's radius is at most 511.
+constexpr int32_t kMaxEnvelopeRadius = 1536;
 // Runs every preprocessing kernel on stream s (no host synchronisation).
 int prep_series(Prep &P, hipStream_t s);
 

@@ -77,23 +77,65 @@ __global__ __launch_bounds__(256) void k_minmax_final(const double *red, int npa
     if (threadIdx.x == 0) *c0 = 0.5 * smin[0] + 0.5 * smax[0];
 }
 
-// Envelope: U[t] = max a[max(0,t-w) .. min(n-1,t+w)], L likewise; v = (U+L)/2.
-// The values are exact copies of series values (selection only).
-__global__ __launch_bounds__(256) void k_envelope(const double *__restrict__ a, int64_t n, int w, double *U, double *L,
-                                                  double *v) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int64_t lo = t - w < 0 ? 0 : t - w;
-    int64_t hi = t + w > n - 1 ? n - 1 : t + w;
-    double mx = a[lo], mn = a[lo];
-    for (int64_t k = lo + 1; k <= hi; ++k) {
-        double x = a[k];
-        mx = fmax(mx, x);
-        mn = fmin(mn, x);
+// Envelope: U[t] = max a[max(0,t-w) .. min(n-1,t+w)], L likewise; v = (U+L)/2.  The values are
+// exact copies of series values (selection only).
+//
+// Sliding max / min by doubling in LDS, O(log w) per output with every lane busy: a tile of
+// `tile` outputs stages its tile + 2w inputs once, with coalesced reads (positions outside
+// the series read as -inf for the max and +inf for the min, which clips the windows at the series
+// ends).  Pass k turns the max / min over runs of s = 2^k values into runs of 2s, in place
+// (mx[i] <- max(mx[i], mx[i + s]); a chunk of positions reads before any of it is written and reads
+// nothing a previous chunk wrote).  With s the largest power of two <= B = 2w + 1, the window of B
+// values at i is the union of the runs at i and at i + B - s.
+constexpr int kEnvThreads = 256;
+// 256 outputs per tile while the halo 2w is small against it (more workgroups for short bands),
+// 1024 beyond, so the halo stays under a fifth of the staged inputs up to w = 128.
+static int envelope_tile(int w) { return w <= 16 ? 256 : 1024; }
+
+__global__ __launch_bounds__(kEnvThreads) void k_envelope(const double *__restrict__ a, int64_t n, int w, int tile,
+                                                          double *U, double *L, double *v) {
+    extern __shared__ double env_lds[];
+    const int B = 2 * w + 1;
+    const int len = tile + 2 * w;
+    double *mx = env_lds, *mn = env_lds + len;
+    const int64_t t0 = (int64_t)blockIdx.x * tile;  // first output of the tile
+    const int64_t base = t0 - w;                    // series position of buffer index 0
+    for (int i = threadIdx.x; i < len; i += kEnvThreads) {
+        const int64_t p = base + i;
+        const bool in = p >= 0 && p < n;
+        const double x = in ? a[p] : 0.0;
+        mx[i] = in ? x : -INFINITY;
+        mn[i] = in ? x : INFINITY;
     }
-    U[t] = mx;
-    L[t] = mn;
-    v[t] = 0.5 * mx + 0.5 * mn;
+    __syncthreads();
+    int s = 1;  // mx[i] = max of buffer [i, i + s) wherever i + s <= len
+    for (; 2 * s <= B; s *= 2) {
+        const int cnt = len - 2 * s + 1;  // positions whose run of 2s stays inside the buffer
+        for (int c = 0; c < cnt; c += kEnvThreads) {
+            const int i = c + threadIdx.x;
+            double hi = 0.0, lo = 0.0;
+            if (i < cnt) {
+                hi = fmax(mx[i], mx[i + s]);
+                lo = fmin(mn[i], mn[i + s]);
+            }
+            __syncthreads();
+            if (i < cnt) {
+                mx[i] = hi;
+                mn[i] = lo;
+            }
+            __syncthreads();
+        }
+    }
+    const int off = B - s;  // 0 <= off < s, and tile - 1 + off + s <= len
+    for (int i = threadIdx.x; i < tile; i += kEnvThreads) {
+        const int64_t t = t0 + i;
+        if (t >= n) break;
+        const double hi = fmax(mx[i], mx[i + off]);
+        const double lo = fmin(mn[i], mn[i + off]);
+        U[t] = hi;
+        L[t] = lo;
+        v[t] = 0.5 * hi + 0.5 * lo;
+    }
 }
 
 // Channel values of the prefix-sum scan, squares as exact double-double products (TwoProd).
@@ -356,11 +398,14 @@ int prep_series(Prep &P, hipStream_t s) {
     const int m = P.m;
     const int64_t nw = n - m + 1;
     const int64_t nb = prep_scan_blocks(n);
+    if (P.w < 0 || P.w > kMaxEnvelopeRadius) return hip_fail(hipErrorInvalidValue, "envelope radius", __FILE__, __LINE__);
     hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, s, P.flags, P.gstat);
     int parts = (int)((n + 255) / 256 < kRedBlocks ? (n + 255) / 256 : kRedBlocks);
     hipLaunchKernelGGL(k_minmax_partial, dim3(parts), dim3(256), 0, s, P.a, n, P.red, P.flags);
     hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, s, P.red, parts, P.c0);
-    hipLaunchKernelGGL(k_envelope, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P.a, n, P.w, P.U, P.L, P.v);
+    const int etile = envelope_tile(P.w);
+    hipLaunchKernelGGL(k_envelope, dim3((unsigned)((n + etile - 1) / etile)), dim3(kEnvThreads),
+                       sizeof(double) * 2 * (size_t)(etile + 2 * P.w), s, P.a, n, P.w, etile, P.U, P.L, P.v);
     hipLaunchKernelGGL(k_scan_partial, dim3((unsigned)nb), dim3(kScanThreads), 0, s, P.a, P.U, P.L, P.c0, n, nb,
                        P.bsum);
     hipLaunchKernelGGL(k_scan_blocks, dim3(5), dim3(kScanThreads), 0, s, P.bsum, nb);

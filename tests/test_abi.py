@@ -44,6 +44,12 @@ def test_argument_validation(dtwj, n, m, w, msg):
     assert e.value.code == dtwj.DTWJ_EINVAL and msg in e.value.message
 
 
+def test_envelope_radius_limit(dtwj):
+    with pytest.raises(dtwj.DTWJError) as e:
+        dtwj.envelope(np.cumsum(np.ones(10000)), 2000)
+    assert e.value.code == dtwj.DTWJ_EINVAL and "1536" in e.value.message
+
+
 def test_nan_rejected(dtwj):
     a = np.cumsum(np.random.default_rng(0).standard_normal(500))
     a[77] = np.nan

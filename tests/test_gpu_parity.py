@@ -38,10 +38,26 @@ def test_sliding_stats_match_oracle(gpu, oracle):
 
 def test_envelope_is_exact(gpu, oracle):
     a = rw(50000, 2)
-    for w in (0, 1, 3, 13, 102):
+    for w in (0, 1, 2, 3, 13, 102, 511, 1024, 1536):
         lo, up = gpu.envelope(a, w)
         lo0, up0 = oracle.lemire(a, w)
-        assert np.array_equal(lo, lo0) and np.array_equal(up, up0)
+        assert np.array_equal(lo, lo0) and np.array_equal(up, up0), w
+
+
+@pytest.mark.parametrize("n", [2, 7, 255, 256, 257, 1023, 1024, 1025, 3000])
+def test_envelope_tile_edges(gpu, oracle, n):
+    """Series shorter than, equal to and just past one tile (256 outputs for w <= 16, 1024 above,
+    so both tile sizes meet each of these lengths); radii up to and past n - 1
+    (clipped windows at both ends, and the whole-series case); repeated values (ties)."""
+    a = np.round(rw(n, n), 1)
+    for w in sorted({0, 1, 5, 16, 17, 100, n // 2, n - 1, n + 7, 4000}):
+        if min(w, n - 1) > 1536:
+            with pytest.raises(gpu.DTWJError):
+                gpu.envelope(a, w)
+            continue
+        lo, up = gpu.envelope(a, w)
+        lo0, up0 = oracle.lemire(a, min(w, n - 1))
+        assert np.array_equal(lo, lo0) and np.array_equal(up, up0), (n, w)
 
 
 @pytest.mark.parametrize("m,w", [(64, 0), (64, 3), (128, 6), (256, 13), (512, 51), (1024, 102), (64, 63), (96, 200)])
