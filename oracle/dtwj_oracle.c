@@ -308,6 +308,45 @@ static int lb_kk(const double *q, const double *U, const double *L, int64_t seql
     return 1;
 }
 
+/* LB_Petitjean, CPU_version1/LB_PETITJEAN.cpp:9-123 (lbPetitjean_new).  q is the window
+ * LB_Keogh was taken of (uq, lq its per-window envelope), x the other window (ux, lx): the
+ * projection p of q (q clamped into [lx, ux] on positions [3, m-3), q elsewhere), its clipped
+ * radius-w envelope (up, lp: the reference's monotone deques compute the same clipped sliding
+ * max / min as orc_lemire), then for positions [3, m-3) the extra cost of x outside that
+ * envelope -- reduced by what the part beyond q's own envelope already paid -- is added to lbk
+ * (which holds LB_Keogh of q against x's envelope) and to cb, abandoning once the sum exceeds
+ * threshold_2.  The reference interleaves the three passes with a lag of w; each value is
+ * computed from the same operands in the same order here. */
+static int lb_petitjean(double *p, double *up, double *lp, const double *q, const double *uq, const double *lq,
+                        const double *x, const double *ux, const double *lx, int w, int m, double threshold_2,
+                        double *lbk, double *cb) {
+    memcpy(p, q, sizeof(double) * (size_t)m);
+    for (int i = 3; i < m - 3; i++) {
+        if (q[i] > ux[i])
+            p[i] = ux[i];
+        else if (q[i] < lx[i])
+            p[i] = lx[i];
+    }
+    orc_lemire(p, m, w, lp, up);
+    double d = *lbk;
+    for (int ii = 3; ii < m - 3; ii++) {
+        double local_dist = 0;
+        if (x[ii] > up[ii] && up[ii] > uq[ii])
+            local_dist = ORC_DIST(x[ii], uq[ii]) - ORC_DIST(up[ii], uq[ii]);
+        else if (x[ii] < lp[ii] && lp[ii] < lq[ii])
+            local_dist = ORC_DIST(x[ii], lq[ii]) - ORC_DIST(lp[ii], lq[ii]);
+        else if (x[ii] > up[ii])
+            local_dist = ORC_DIST(x[ii], up[ii]);
+        else if (x[ii] < lp[ii])
+            local_dist = ORC_DIST(x[ii], lp[ii]);
+        cb[ii] += local_dist;
+        d += local_dist;
+        if (d > threshold_2) return 0;
+    }
+    *lbk = d;
+    return 1;
+}
+
 /* CPU_version1/MON_DTW.cpp:11-18 */
 static inline size_t cap_start(size_t index, size_t window) { return index > window ? index - window : 0; }
 static inline size_t cap_stop(size_t index, size_t window, size_t end) {
@@ -909,6 +948,9 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
     double *cb = (double *)xcalloc(subseqLen + maxwarp + 1, sizeof(double));
     double *cb1 = (double *)xcalloc(subseqLen + maxwarp, sizeof(double));
     double *cb2 = (double *)xcalloc(subseqLen + maxwarp, sizeof(double));
+    double *proj = (double *)xcalloc(subseqLen, sizeof(double));
+    double *proj_u = (double *)xcalloc(subseqLen, sizeof(double)), *proj_l = (double *)xcalloc(subseqLen, sizeof(double));
+    double pe_cnt = 0, local_rounds = 0;
     int found = 0;
     for (int i_buffer = 0; i_buffer < buffer_length; i_buffer++) {
         for (int64_t diag = minlag + 1 + i_buffer; diag <= subcount; diag += buffer_length) {
@@ -927,6 +969,11 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
                 lb_kim(st, bsf, subcount - diag + 1, (int)diag, &cnt_of_purn);
             }
             global_prune_cnt += (cnt_of_purn - cnt1);
+            /* :405, :465, :484: the reference's alpha-tuned local masks (diag_mask_local_up /
+               _down) run on this diagonal only when fewer than 80 % of its pairs are pruned by
+               now; they are not restated here, so such a diagonal is counted (local_rounds):
+               with none, the stage sequence is the reference's */
+            if (cnt_of_purn / (double)(subcount - diag + 1) < 0.80) local_rounds++;
             for (int64_t i = 1; i <= subcount - diag + 1; i++) {
                 all_cnt++;
                 if (st->lb_vector[i - 1]) continue;
@@ -937,6 +984,12 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
                     continue;
                 if (!lb_kk(SUB(st, i + diag - 2), SUBU(st, i - 1), SUBL(st, i - 1), subseqLen, bsf * bsf, cb2,
                            st->special + 2 * (i - 1), &lbk2))
+                    continue;
+                /* :514-520 */
+                pe_cnt++;
+                if (!lb_petitjean(proj, proj_u, proj_l, SUB(st, i + diag - 2), SUBU(st, i + diag - 2),
+                                  SUBL(st, i + diag - 2), SUB(st, i - 1), SUBU(st, i - 1), SUBL(st, i - 1), maxwarp,
+                                  subseqLen, bsf * bsf, &lbk2, cb2))
                     continue;
                 dtw_cnt++;
                 int mm = subseqLen;
@@ -976,6 +1029,8 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
     out->all_cnt = all_cnt;
     out->kk_cnt = kk_cnt;
     out->dtw_cnt = dtw_cnt;
+    out->pe_cnt = pe_cnt;
+    out->local_rounds = local_rounds;
     /* free everything */
     free(st->mu); free(st->sig); free(st->invsig); free(st->subs); free(st->subsU); free(st->subsL);
     free(min_index); free(cb_temp); free(st->UTS); free(st->LTS); free(st->dr_bwd); free(st->dc_bwd);
@@ -986,7 +1041,7 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
     free(st->dr_fwdM); free(st->dc_bwdT2); free(st->dc_fwdT2); free(sUg); free(sLg); free(sU2g); free(sL2g);
     free(st->sumMASK_g); free(st->sumU_sumL_g); free(sU2sL2g); free(st->normUL_g); free(st->DUL2_g);
     free(st->DUL_g); free(st->special); free(st->lb_vector); free(st->lb_vector_new); free(cb); free(cb1);
-    free(cb2);
+    free(cb2); free(proj); free(proj_u); free(proj_l);
     return 0;
 }
 

@@ -2,8 +2,9 @@
 
 tests/golden/golden.json holds, per case, what CPU_version1 printed for the stored series:
 1-based first/second index (CPU_version1/new_dtw_motifGUI.cpp:575), final distance and seed
-distance (:578, printed with %f / %5.3f) and the diag_fast / global-stage prune fractions
-(:569-570, printed with 6 significant digits).
+distance (:578, printed with %f / %5.3f), the diag_fast / global-stage prune fractions
+(:569-570, printed with 6 significant digits) and the number of pairs that reached the DTW
+("DTW times", :565): the pairs that passed both LB_KK directions and LB_Petitjean.
 """
 import math
 
@@ -30,6 +31,12 @@ def test_oracle_matches_reference(name, oracle):
         got = val / all_cnt
         want = ref[key]
         assert math.isclose(got, want, rel_tol=6e-6, abs_tol=1e-12), (key, got, want)
+    # the DTW count, exactly: it pins LB_Petitjean (which removes pairs that passed both LB_KK
+    # directions) and every bound before it; the reference's alpha-tuned local masks, which the
+    # oracle does not restate, run on some diagonals of three of these cases (local_rounds) and
+    # only move pairs between stages there
+    assert r.dtw_cnt == ref["dtw_times"], (r.dtw_cnt, ref["dtw_times"], r.pe_cnt, r.local_rounds)
+    assert r.pe_cnt >= r.dtw_cnt
 
 
 def test_golden_pairs_are_admissible():
@@ -70,3 +77,4 @@ def test_oracle_matches_reference_grid(oracle):
         assert (r.first_min, r.sec_min) == (int(ref["first_min"]), int(ref["sec_min"])), c
         assert abs(r.bsf - ref["final_distance"]) <= 5e-7, c
         assert abs(r.seed_bsf - ref["input_bsf"]) <= 5e-4, c
+        assert r.dtw_cnt == ref["dtw_times"], (c, r.dtw_cnt, r.local_rounds)
