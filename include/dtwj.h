@@ -57,7 +57,10 @@ typedef struct dtwj_options {
     int32_t cand_cap;      /* near-tie candidate slots of a single-series join (0 = 2^20; a batch
                               keeps 2^14 per series).  A join that pushes more is re-run with a
                               buffer sized to its count, so this only trades memory for re-runs */
-    int32_t reserved[3];
+    int32_t paa;           /* segment-mean (PAA) bound between LB_Kim and LB_Keogh: 0 = automatic
+                              (bands w >= 8 with m >= 128; off in dtwj_top1_device_noalloc), 1 = on
+                              (m >= 16), 2 = off.  It only prunes: the answer is the same */
+    int32_t reserved[2];
 } dtwj_options;
 
 typedef struct dtwj_result {
@@ -90,6 +93,8 @@ typedef struct dtwj_result {
                             best-so-far with a buffer sized to the count; 0 normally) */
     int64_t pairs_swept_one_sided; /* of pairs_swept: tested with the one-sided ball bound */
     int64_t items_total; /* sweep work items (64-bit) */
+    int64_t pass_paa;    /* of pass_kim: pairs also surviving the segment-mean bound, which go on
+                            to LB_Keogh (equal to pass_kim when that stage is off) */
 } dtwj_result;
 
 /* Fills *opt with defaults.  No reference counterpart (the reference hard-codes
@@ -284,6 +289,17 @@ int dtwj_envelope(const double *series, int64_t n, int32_t w, double *lower, dou
  * Reference kernels: DTW_stairs / DTW_stairs_for_block (GPU_version1/src/DTW_stairs.cu:8-599). */
 int dtwj_dtw_pairs(const double *series, int64_t n, int32_t m, int32_t w, const int64_t *pi,
                    const int64_t *pj, int64_t npairs, int32_t prec, double *cost);
+
+/* Segment-mean (PAA) lower bound of LB_Keogh that the join applies between LB_Kim and
+ * LB_Keogh, for an explicit pair list: each window cut into S segments of len = floor(m / S)
+ * samples (S = 32 for m >= 512, else 16), len * sum over segments of max(Z - U, L - Z, 0)^2 of
+ * one window's segment means
+ * against the other's envelope segment means (raw-series envelope of radius w), the larger of
+ * the two directions, as the kernel computes it in FP32 (before its safety scaling).  m >= 16.
+ * No reference counterpart (the reference's cascade has no such stage; it stands for the
+ * LB_Keogh work of CPU_version1/lb_keogh.cpp:9-72 that it saves). */
+int dtwj_paa_pairs(const double *series, int64_t n, int32_t m, int32_t w, const int64_t *pi,
+                   const int64_t *pj, int64_t npairs, double *lb);
 
 /* ---- synthetic workloads ------------------------------------------------------------ */
 typedef struct dtwj_gen_spec {

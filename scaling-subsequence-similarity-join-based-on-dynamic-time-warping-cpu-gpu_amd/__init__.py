@@ -38,7 +38,7 @@ class Options(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("seed_items", ctypes.c_int32), ("rows_per_item", ctypes.c_int32),
                 ("waves_per_cu", ctypes.c_int32), ("verbose", ctypes.c_int32), ("graph", ctypes.c_int32),
                 ("autotune", ctypes.c_int32), ("sweep", ctypes.c_int32), ("cand_cap", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 3)]
+                ("paa", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
 
 
 class Result(ctypes.Structure):
@@ -51,7 +51,7 @@ class Result(ctypes.Structure):
                 ("t_join", ctypes.c_double), ("t_verify", ctypes.c_double), ("join_kernel_ms", ctypes.c_double),
                 ("items", ctypes.c_int32), ("rows_per_item", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
                 ("reruns", ctypes.c_int32), ("pairs_swept_one_sided", ctypes.c_int64),
-                ("items_total", ctypes.c_int64)]
+                ("items_total", ctypes.c_int64), ("pass_paa", ctypes.c_int64)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -112,6 +112,7 @@ EXPORTED = [
     "dtwj_gen_spec_default", "dtwj_generate", "dtwj_top1_batch",
     "dtwj_context_top1_batch", "dtwj_proposal_candidates", "dtwj_mlp_forward", "dtwj_mlp_gradient",
     "dtwj_mlp_train", "dtwj_top1_learned", "dtwj_plan", "dtwj_top1_device_noalloc", "dtwj_matrix_profile",
+    "dtwj_paa_pairs",
 ]
 
 _lib = None
@@ -148,6 +149,7 @@ def lib():
     L.dtwj_envelope.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, d_p, d_p]
     L.dtwj_dtw_pairs.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, i64_p, i64_p, ctypes.c_int64,
                                  ctypes.c_int32, d_p]
+    L.dtwj_paa_pairs.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, i64_p, i64_p, ctypes.c_int64, d_p]
     L.dtwj_top1_batch.argtypes = [P(Series), ctypes.c_int32, P(Options), P(Result), i32_p]
     L.dtwj_context_top1_batch.argtypes = [ctypes.c_void_p, P(Series), ctypes.c_int32, P(Options), P(Result), i32_p]
     f_p = P(ctypes.c_float)
@@ -214,6 +216,7 @@ class Top1:
     pairs_swept_one_sided: int = 0
     reruns: int = 0
     items_total: int = 0
+    pass_paa: int = 0
     extra: dict = field(default_factory=dict)
 
     @property
@@ -231,12 +234,13 @@ def _result(r):
 
 
 def make_options(device=0, seed_items=0, rows_per_item=0, waves_per_cu=0, verbose=False, graph=0, autotune=0,
-                 sweep=0, cand_cap=0):
+                 sweep=0, cand_cap=0, paa=0):
     o = Options()
     lib().dtwj_default_options(ctypes.byref(o))
     o.device, o.seed_items, o.rows_per_item, o.waves_per_cu, o.verbose = (
         device, seed_items, rows_per_item, waves_per_cu, int(bool(verbose)))
     o.graph, o.autotune, o.sweep, o.cand_cap = int(graph), int(autotune), int(sweep), int(cand_cap)
+    o.paa = int(paa)
     return o
 
 
@@ -499,6 +503,19 @@ def dtw_pairs(series, m, w, pi, pj, prec=32):
     i64 = ctypes.POINTER(ctypes.c_int64)
     _check(lib().dtwj_dtw_pairs(_dptr(a), a.size, int(m), int(w), pi.ctypes.data_as(i64), pj.ctypes.data_as(i64),
                                 pi.size, int(prec), _dptr(out)))
+    return out
+
+
+def paa_pairs(series, m, w, pi, pj):
+    """Segment-mean lower bound of LB_Keogh (the join's stage between LB_Kim and LB_Keogh, as
+    the kernel computes it in FP32) for each window pair (pi[k], pj[k])."""
+    a = _as_series(series)
+    pi = np.ascontiguousarray(np.asarray(pi, dtype=np.int64))
+    pj = np.ascontiguousarray(np.asarray(pj, dtype=np.int64))
+    out = np.empty(pi.size)
+    i64 = ctypes.POINTER(ctypes.c_int64)
+    _check(lib().dtwj_paa_pairs(_dptr(a), a.size, int(m), int(w), pi.ctypes.data_as(i64), pj.ctypes.data_as(i64),
+                                pi.size, _dptr(out)))
     return out
 
 

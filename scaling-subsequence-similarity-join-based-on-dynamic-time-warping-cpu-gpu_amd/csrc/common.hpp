@@ -110,6 +110,16 @@ struct WinRow {
     float sig;       // sigma
 };
 
+// Segment-mean (PAA) bound of LB_Keogh: each window is cut into S segments of floor(m / S)
+// samples (the rest of the window is left out), S = paa_segments(m).  Row p of the table (3 S
+// floats) holds, in the z-units of window p, the segment means of the upper envelope, of the
+// lower envelope and of the window itself: [U_0 .. U_S-1, L_0 .. L_S-1, Z_0 .. Z_S-1], FP32, U
+// rounded up and L rounded down after widening both by the FP32 rounding bound of any Z.
+// 32 segments for windows of 512 and more (measured: m = 512, w = 51 -2.6 %, m = 1024, w = 102
+// -4.1 % against 16), 16 below (m = 256, w = 13: +1.7 % with 32).
+constexpr int kPaaSegMax = 32;
+__host__ __device__ constexpr int paa_segments(int m) { return m >= 512 ? 32 : 16; }
+
 // One series of a join: everything the join kernels read.  Lives in device memory (an
 // array of them for a batch); device pointers unless noted.
 struct JoinArgs {
@@ -149,6 +159,9 @@ struct JoinArgs {
     float rel_dtw;     // relative inflation of the best-so-far cost used as pruning threshold
     int32_t valid;     // 0: the preprocessing flagged the series; the join skips it
     int32_t sweep_mode;  // 0 adaptive, 1 two-sided, 2 one-sided ball test in the sweep
+    const float *paa;    // nw x 3 paa_seg segment-mean table, or null: no segment-mean stage
+    int32_t paa_seg;     // segments per window (16 or 32; 0 without the stage)
+    int32_t paa_len;     // samples per segment, floor(m / paa_seg)
 };
 
 // seed scan: per-item best ED pair

@@ -42,7 +42,7 @@ constexpr int64_t kBatchCandCap = 1 << 14;  // ... per series of a batch
 constexpr int64_t kMaxCandCap = 0x7fffffffLL;  // largest regrown buffer (re-scoring grid limit)
 constexpr int kRegrow = -2;  // finish_job: more candidates were pushed than the buffer holds
 constexpr int kDiagGroup = 256;  // diagonals per work item (join.hip DG)
-constexpr int kNCounters = 8;
+constexpr int kNCounters = 9;
 
 int fail(int code, const std::string &msg) {
     set_error(msg);
@@ -198,6 +198,7 @@ struct Job {
     int64_t cand_need = 0;   // pushes counted by the last run when they exceeded cand_cap
     unsigned long long init_key = 0;  // best-so-far the join starts from
     int32_t sweep_mode = 0;  // dtwj_options::sweep
+    int32_t paa_seg = 0;     // segments per window of the segment-mean stage (0: stage off)
     int64_t n_items = 0;
     std::vector<int64_t> item_start;
     std::vector<int2> extra;  // explicit seed pairs (besides the scan's and (0, m))
@@ -248,6 +249,16 @@ void plan_job(Job &J, const double *d_a, int64_t n, int32_t m, int32_t w_in, int
     J.init_key = pack_bsf(3.0e38f, 0xffffffffu);
 }
 
+// Segments per window of the segment-mean stage for an effective band w (0: the stage is off).
+// Automatic (mode 0): bands of 8 and more with windows of 128 and more, where LB_Keogh takes a
+// visible share of the join; mode 1 forces it on (windows of 16 and more), mode 2 off.  The
+// stage only prunes, so the answer is the same either way.
+int32_t paa_seg_for(int32_t m, int32_t w, int32_t mode) {
+    if (mode == 2 || m < 16) return 0;
+    if (mode == 0 && (w < 8 || m < 128)) return 0;
+    return paa_segments(m);
+}
+
 // Device bytes of one series (statistics, envelopes, tables, scans, join state).
 size_t job_bytes(const Job &J) {
     const int64_t n = J.n, nw = J.nw;
@@ -264,8 +275,9 @@ size_t job_bytes(const Job &J) {
     add(sizeof(WinRow) * nw);
     add(4 * 3 * nblk);
     add(4 * 8);           // gstat
-    add(sizeof(dd_t) * 5 * (n + 1));
-    add(sizeof(dd_t) * 5 * nb);
+    add(sizeof(dd_t) * kPrepScanChannels * (n + 1));
+    add(sizeof(dd_t) * kPrepScanChannels * nb);
+    if (J.paa_seg > 0) add(sizeof(float) * 3 * (size_t)J.paa_seg * (size_t)nw);
     add(8 * 2 * 1024);
     add(4 * 4);           // flags
     add(8);               // bsf
@@ -300,8 +312,10 @@ void take_job(Arena &ar, Job &J) {
     P.nblk = (int32_t)((nw + 255) / 256);
     P.blkmax = ar.take<float>(3 * (size_t)P.nblk);
     P.gstat = ar.take<float>(8);
-    P.P = ar.take<dd_t>(5 * (size_t)(n + 1));
-    P.bsum = ar.take<dd_t>(5 * (size_t)prep_scan_blocks(n));
+    P.P = ar.take<dd_t>(kPrepScanChannels * (size_t)(n + 1));
+    P.bsum = ar.take<dd_t>(kPrepScanChannels * (size_t)prep_scan_blocks(n));
+    P.paa = J.paa_seg > 0 ? ar.take<float>(3 * (size_t)J.paa_seg * (size_t)nw) : nullptr;
+    P.paa_seg = J.paa_seg;
     P.red = ar.take<double>(2 * 1024);
     P.flags = ar.take<int32_t>(4);
     J.d_bsf = ar.take<unsigned long long>(1);
@@ -342,6 +356,9 @@ void take_job(Arena &ar, Job &J) {
     A.cand_cap = J.cand_cap;
     A.counters = J.d_counters;
     A.sweep_mode = J.sweep_mode;
+    A.paa = P.paa;
+    A.paa_seg = P.paa_seg;
+    A.paa_len = P.paa_seg > 0 ? J.m / P.paa_seg : 0;
 }
 
 // Host-to-device copies a job needs before its kernels run (item offsets, explicit seeds).
@@ -421,6 +438,7 @@ int finish_job(Job &J, double *d_cost64, Winner *d_part, Winner *d_win, hipStrea
     out->pass_ball = (int64_t)counters[1];
     out->pass_kim = (int64_t)counters[2];
     out->pass_keogh = (int64_t)counters[3];
+    out->pass_paa = J.paa_seg > 0 ? (int64_t)counters[8] : (int64_t)counters[2];
     out->dtw_abandoned = (int64_t)counters[4];
     out->dtw_cells = (int64_t)counters[5];
     out->keogh_elems = (int64_t)counters[6];
@@ -680,10 +698,12 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
              const Seeds &seeds, dtwj_result *out, clk::time_point t_entry) {
     if (opt && (opt->sweep < 0 || opt->sweep > 2)) return fail(DTWJ_EINVAL, "options.sweep must be 0, 1 or 2");
     if (opt && opt->cand_cap < 0) return fail(DTWJ_EINVAL, "options.cand_cap must be >= 0");
+    if (opt && (opt->paa < 0 || opt->paa > 2)) return fail(DTWJ_EINVAL, "options.paa must be 0, 1 or 2");
     Job J;
     plan_job(J, d_a, n, m, w_in, opt ? opt->rows_per_item : 0, opt ? opt->seed_items : 0, seeds.gpu_scan, seeds.pairs,
              (opt && opt->cand_cap > 0) ? (int64_t)opt->cand_cap : kCandCap);
     J.sweep_mode = opt ? opt->sweep : 0;
+    J.paa_seg = paa_seg_for(J.m, J.w, opt ? opt->paa : 0);
     const int32_t wpc = (opt && opt->waves_per_cu > 0) ? opt->waves_per_cu : 8;
     const int32_t grid = grid_for(c, wpc, J.n_items);
     int32_t reruns = 0;
@@ -791,6 +811,7 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
 int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const dtwj_options *opt, int32_t rows_req,
               int32_t wpc, dtwj_result *out, int32_t *status, float *join_ms) {
     if (opt && (opt->sweep < 0 || opt->sweep > 2)) return fail(DTWJ_EINVAL, "options.sweep must be 0, 1 or 2");
+    if (opt && (opt->paa < 0 || opt->paa > 2)) return fail(DTWJ_EINVAL, "options.paa must be 0, 1 or 2");
     std::vector<Job> jobs;
     std::vector<int32_t> owner;  // job -> batch index
     jobs.reserve(count);
@@ -812,6 +833,7 @@ int run_batch(dtwj_context *c, const dtwj_series *series, int32_t count, const d
         const int32_t seed_req = opt && opt->seed_items > 0 ? opt->seed_items : 4096;
         plan_job(jobs.back(), nullptr, S.n, S.m, S.w, rows_req, seed_req, true, {}, kBatchCandCap);
         jobs.back().sweep_mode = opt ? opt->sweep : 0;
+        jobs.back().paa_seg = paa_seg_for(jobs.back().m, jobs.back().w, opt ? opt->paa : 0);
         owner.push_back(k);
     }
     const int32_t nj = (int32_t)jobs.size();
@@ -1068,6 +1090,8 @@ int dtwj_plan(int64_t n, int32_t m, int32_t w, const dtwj_options *opt, dtwj_pla
     if (int rc = validate(&placeholder, n, m, w)) return rc;
     Job J;
     plan_job(J, nullptr, n, m, w, opt ? opt->rows_per_item : 0, opt ? opt->seed_items : 0, true, {}, kCandCap);
+    if (opt && (opt->paa < 0 || opt->paa > 2)) return fail(DTWJ_EINVAL, "options.paa must be 0, 1 or 2");
+    J.paa_seg = paa_seg_for(J.m, J.w, opt ? opt->paa : 0);
     const int32_t wpc = (opt && opt->waves_per_cu > 0) ? opt->waves_per_cu : 8;
     const int64_t grid = std::min<int64_t>((int64_t)256 * wpc, std::max<int64_t>(J.n_items, 1));
     auto pad = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -1112,6 +1136,7 @@ int dtwj_top1_device_noalloc(const double *d_series, int64_t n, int32_t m, int32
         dtwj_default_options(&o);
     if (o.waves_per_cu == 0) o.waves_per_cu = 4;  // 1024 survivor queues of 16 KB on 256 CUs
     if (o.cand_cap == 0) o.cand_cap = 1 << 16;
+    if (o.paa == 0) o.paa = 2;  // the segment-mean table (192 bytes per window) stays out of the workspace
     const int rc = run_join(&c, d_series, n, m, w, &o, Seeds{}, out, te);
     c.mem = nullptr;  // the workspace stays with the device
     return rc;
@@ -1523,13 +1548,14 @@ int dtwj_top1_learned(const double *series, int64_t n, int32_t m, int32_t w, con
 
 // ---------------------------------------------------------------- building blocks
 static int building_block_prep(const double *series, int64_t n, int32_t m, int32_t w, dtwj_context **cout,
-                               Prep &P, const double **d_a) {
+                               Prep &P, const double **d_a, int32_t paa_seg = 0) {
     if (int rc = with_context(0, cout)) return rc;
     dtwj_context *c = *cout;
     if (int rc = dtwj_context_upload(c, series, n, d_a)) return rc;
     Job J;
     plan_job(J, *d_a, n, m, w, 0, 1, false, {}, 1);
     J.w = w;
+    J.paa_seg = paa_seg;
     Arena dry;
     take_job(dry, J);
     if (int rc = ensure_mem(c, dry.off + 256)) return rc;
@@ -1628,6 +1654,45 @@ int dtwj_dtw_pairs(const double *series, int64_t n, int32_t m, int32_t w, const 
     if (d_pi) (void)hipFree(d_pi);
     if (d_pj) (void)hipFree(d_pj);
     if (d_cost) (void)hipFree(d_cost);
+    if (c) dtwj_context_destroy(c);
+    return rc;
+}
+
+int dtwj_paa_pairs(const double *series, int64_t n, int32_t m, int32_t w, const int64_t *pi, const int64_t *pj,
+                   int64_t npairs, double *lb) {
+    if (int rc = validate(series, n, m, w)) return rc;
+    if (m < 16) return fail(DTWJ_EINVAL, "the segment-mean bound needs m >= 16");
+    if (!pi || !pj || !lb || npairs < 0) return fail(DTWJ_EINVAL, "bad pair arguments");
+    const int64_t nw = n - m + 1;
+    for (int64_t p = 0; p < npairs; ++p)
+        if (pi[p] < 0 || pj[p] < 0 || pi[p] >= nw || pj[p] >= nw) return fail(DTWJ_EINVAL, "pair index out of range");
+    if (npairs == 0) return DTWJ_OK;
+    dtwj_context *c = nullptr;
+    Prep P;
+    const double *d_a = nullptr;
+    const int32_t nseg = paa_segments(m);
+    int rc = building_block_prep(series, n, m, std::min(w, m - 1), &c, P, &d_a, nseg);
+    int64_t *d_pi = nullptr, *d_pj = nullptr;
+    double *d_lb = nullptr;
+    if (rc == DTWJ_OK) {
+        if (hipMalloc(&d_pi, sizeof(int64_t) * npairs) != hipSuccess || hipMalloc(&d_pj, sizeof(int64_t) * npairs) != hipSuccess ||
+            hipMalloc(&d_lb, sizeof(double) * npairs) != hipSuccess)
+            rc = fail(DTWJ_ENOMEM, "device allocation failed");
+    }
+    if (rc == DTWJ_OK) {
+        hipError_t e = hipMemcpy(d_pi, pi, sizeof(int64_t) * npairs, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_pj, pj, sizeof(int64_t) * npairs, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy", __FILE__, __LINE__);
+    }
+    if (rc == DTWJ_OK) rc = launch_paa_pairs(P.paa, nseg, m / nseg, d_pi, d_pj, npairs, d_lb, c->stream);
+    if (rc == DTWJ_OK) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(lb, d_lb, sizeof(double) * npairs, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "segment-mean pairs", __FILE__, __LINE__);
+    }
+    if (d_pi) (void)hipFree(d_pi);
+    if (d_pj) (void)hipFree(d_pj);
+    if (d_lb) (void)hipFree(d_lb);
     if (c) dtwj_context_destroy(c);
     return rc;
 }

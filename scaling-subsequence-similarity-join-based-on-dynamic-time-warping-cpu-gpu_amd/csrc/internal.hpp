@@ -24,13 +24,18 @@ struct Prep {
     float *blkmax = nullptr;                            // 3 * nblk
     int32_t nblk = 0;
     float *gstat = nullptr;  // 8: max|ac|, max|mu_c|, max isig, max sig, min sig, -, -, -
+    float *paa = nullptr;  // nw x 3 paa_seg segment-mean table (null: no segment-mean stage)
+    int32_t paa_seg = 0;   // segments per window (paa_segments(m), 0: stage off)
     // scratch
-    dd_t *P = nullptr;      // 5 * (n + 1) prefix sums
-    dd_t *bsum = nullptr;   // 5 * nscan_blocks
+    dd_t *P = nullptr;      // kPrepScanChannels * (n + 1) prefix sums
+    dd_t *bsum = nullptr;   // kPrepScanChannels * nscan_blocks
     double *red = nullptr;  // 2 * 1024 min/max partials
     int32_t *flags = nullptr;  // [0] non-finite input, [1] constant window
 };
 
+// Channels of the compensated prefix sums: a - c0, (a - c0)^2, v - c0, (v - c0)^2, h^2, h, with
+// v = (U + L) / 2 and h = (U - L) / 2.
+constexpr int kPrepScanChannels = 6;
 int64_t prep_scan_blocks(int64_t n);
 // Largest envelope radius the preprocessing kernels take (their LDS tile holds 1024 + 2w doubles
 // twice); a join's radius is at most 511.
@@ -62,6 +67,9 @@ size_t join_lds_bytes(int32_t m, int32_t w);
 // DTW of an explicit pair list (dtw.hip).  prec 32: FP32 cells from ac/mu_c/isig;
 // prec 64: FP64 cells from a/mu/sig.  cost: npairs doubles (device).  Reads only the
 // series / statistics pointers and m, w of args (a host copy).
+// Segment-mean bound (FP32, unscaled) of explicit window pairs from a table built by prep_series.
+int launch_paa_pairs(const float *paa, int32_t nseg, int32_t len, const int64_t *pi, const int64_t *pj, int64_t npairs,
+                     double *out, hipStream_t s);
 int launch_dtw_pairs(const JoinArgs &args, const double *sig64, const int64_t *pi, const int64_t *pj,
                      int64_t npairs, int32_t prec, double *cost, hipStream_t s);
 

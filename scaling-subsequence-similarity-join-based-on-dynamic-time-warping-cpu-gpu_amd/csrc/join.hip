@@ -65,10 +65,13 @@ constexpr int PLCAP = 64 * KD;         // expanded pairs of one 64-entry queue b
 constexpr int Q2CAP = 256;             // LB_Keogh queue (packed pairs)
 constexpr int Q2RUN = Q2CAP - 64;      // run LB_Keogh once this many pairs are queued
 constexpr int Q3CAP = 128;             // DTW queue (packed pairs)
+constexpr int QPCAP = 128;             // segment-mean queue (packed pairs): < 64 left + one LB_Kim batch
 constexpr int QLIM = kJoinQueueCap - 8 * 64;  // drain before an 8-row group could overflow
 
-enum { C_SWEPT = 0, C_BALL, C_KIM, C_KEOGH, C_DTW_AB, C_CELLS, C_KEOGH_ELEMS, C_SWEPT_ONE, C_NCOUNTERS };
-static_assert(C_NCOUNTERS == 8, "the host reads 8 counters per series");
+enum { C_SWEPT = 0, C_BALL, C_KIM, C_KEOGH, C_DTW_AB, C_CELLS, C_KEOGH_ELEMS, C_SWEPT_ONE, C_PAA, C_NCOUNTERS };
+static_assert(C_NCOUNTERS == 9, "the host reads 9 counters per series");
+constexpr int kWcnt = 10;  // LDS stage counters (80 bytes keep the dynamic area 16-byte aligned)
+static_assert(kWcnt >= C_NCOUNTERS && kWcnt % 2 == 0, "stage counters");
 
 // LDS words of the cascade staging for window length m (overlays the column tables): series
 // and envelope of the chunk's row windows and column windows (upper and lower envelope in FP32,
@@ -267,6 +270,82 @@ struct CascadeLDS {
 static_assert((511 + 3) / 2 + 1 <= 2 * RC + 2 * (RC + DG),
               "DTW reads before a row window stay in the statistics for the widest band, w = 511");
 
+// Segment-mean bound of the window pair (gi, gj): len sum_s max(Z_j - U_i, L_i - Z_j, 0)^2 and
+// the same with i and j swapped, the larger of the two (rows of the series' table, see kPaaSeg).
+// For the LB_Keogh terms t_k of a segment's samples, sum t_k^2 >= (sum t_k)^2 / len
+// (Cauchy-Schwarz) and sum t_k >= len max(Z - U, L - Z, 0), so the bound is at most LB_Keogh
+// against the raw-series envelope, itself at most the banded DTW.  In FP32 each term costs one
+// rounded subtraction and one FMA: the result is at most 20 * 2^-24 above the bound of the
+// table values, which a caller's (1 - 2^-17) scaling covers.
+__device__ inline float paa_lb(const float *paa, int64_t gi, int64_t gj, int nseg, float len) {
+    const float4 *ti = reinterpret_cast<const float4 *>(paa + gi * 3 * nseg);
+    const float4 *tj = reinterpret_cast<const float4 *>(paa + gj * 3 * nseg);
+    float accA = 0.f, accB = 0.f;
+    auto term = [](float z, float u, float l, float &acc) __attribute__((always_inline)) {
+        const float t = fmaxf(fmaxf(z - u, l - z), 0.f);
+        acc = fmaf(t, t, acc);
+    };
+#pragma unroll
+    for (int q = 0; q < nseg / 4; ++q) {
+        const float4 ui = ti[q], li = ti[nseg / 4 + q], zi = ti[nseg / 2 + q];
+        const float4 uj = tj[q], lj = tj[nseg / 4 + q], zj = tj[nseg / 2 + q];
+        term(zj.x, ui.x, li.x, accA);
+        term(zj.y, ui.y, li.y, accA);
+        term(zj.z, ui.z, li.z, accA);
+        term(zj.w, ui.w, li.w, accA);
+        term(zi.x, uj.x, lj.x, accB);
+        term(zi.y, uj.y, lj.y, accB);
+        term(zi.z, uj.z, lj.z, accB);
+        term(zi.w, uj.w, lj.w, accB);
+    }
+    return len * fmaxf(accA, accB);
+}
+static_assert(paa_segments(8) % 8 == 0 && paa_segments(1 << 20) % 8 == 0 && kPaaSegMax >= paa_segments(1 << 20),
+              "the table is read as float4 vectors, 8 segments per round");
+
+// The same bound as a test against thr (true: the pair survives), with early exit: direction A
+// (window j's means against window i's envelope means), then B, each in rounds of 8 segments;
+// a lane stops loading as soon as its partial sum (a lower bound of the full one)
+// exceeds thr, so the table rows of pruned pairs are mostly not read.  Exits exactly when
+// paa_lb(...) (1 - 2^-17) > thr would, up to the rounding of the partial sums.
+__device__ inline bool paa_pass(const float *paa, int64_t gi, int64_t gj, int nseg, float len, float thr) {
+    const float4 *ti = reinterpret_cast<const float4 *>(paa + gi * 3 * nseg);
+    const float4 *tj = reinterpret_cast<const float4 *>(paa + gj * 3 * nseg);
+    const float scale = len * (1.0f - 0x1p-17f);
+    bool alive = true;
+#pragma unroll
+    for (int dir = 0; dir < 2; ++dir) {
+        const float4 *te = dir == 0 ? ti : tj;  // envelope side
+        const float4 *tz = dir == 0 ? tj : ti;  // series side
+        float acc = 0.f;
+#pragma unroll
+        for (int h = 0; h < kPaaSegMax / 8; ++h) {
+            if (alive && 8 * h < nseg) {
+                float4 u[2], l[2], z[2];
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    u[k] = te[2 * h + k];
+                    l[k] = te[nseg / 4 + 2 * h + k];
+                    z[k] = tz[nseg / 2 + 2 * h + k];
+                }
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const float zz[4] = {z[k].x, z[k].y, z[k].z, z[k].w};
+                    const float uu[4] = {u[k].x, u[k].y, u[k].z, u[k].w};
+                    const float ll[4] = {l[k].x, l[k].y, l[k].z, l[k].w};
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const float t = fmaxf(fmaxf(zz[c] - uu[c], ll[c] - zz[c]), 0.f);
+                        acc = fmaf(t, t, acc);
+                    }
+                }
+                alive = !(acc * scale > thr);
+            }
+        }
+    }
+    return alive;
+}
+
 // Packed (row, column) offsets of a pair inside the staged chunk (both < 2^16).
 __device__ inline uint32_t pack_pair(int li, int lj) { return (uint32_t)li | ((uint32_t)lj << 16); }
 __device__ inline int pair_row(uint32_t p) { return (int)(p & 0xffffu); }
@@ -287,8 +366,8 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     // Stage counters live in LDS (updated by lane 0 on the rare paths) so that they occupy no
     // scalar registers in the sweep; LB_Keogh positions are counted per lane.  They are added
     // to the series' counters whenever the wave moves to another series, and at the end.
-    __shared__ __align__(16) unsigned long long wcnt[8];  // 64 bytes keep the dynamic area aligned
-    if (lane < 8) wcnt[lane] = 0ull;
+    __shared__ __align__(16) unsigned long long wcnt[kWcnt];
+    if (lane < kWcnt) wcnt[lane] = 0ull;
     auto count = [&](int which, unsigned long long v) __attribute__((always_inline)) {
         if (lane == 0) wcnt[which] += v;
     };
@@ -313,11 +392,11 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
         c_kelem = 0;
         c_cells = 0;
         __syncthreads();
-        if (lane < 8) wcnt[lane] = 0ull;
+        if (lane < kWcnt) wcnt[lane] = 0ull;
         __syncthreads();
     };
 
-    int q1n = 0, q2n = 0, q3n = 0;
+    int q1n = 0, q2n = 0, q3n = 0, qpn = 0;
     int64_t i_beg = 0, d0 = 0, c0 = 0;
     int64_t cb = 0;  // first row of the RC rows staged for the cascade
     int m = 0;
@@ -328,6 +407,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     uint32_t *plist = reinterpret_cast<uint32_t *>(smem + uni);  // expanded pairs of a batch
     uint32_t *q2 = plist + PLCAP;                                // LB_Keogh queue
     uint32_t *q3 = q2 + Q2CAP;                                   // DTW queue
+    uint32_t *qp = q3 + Q3CAP;                                   // segment-mean queue
 
     // ---- stage 4: DTW of every queued pair.  Bands up to 63 run several pairs per wave
     // (segments of w + 1 lanes); wider bands one pair per wave with QP offset pairs per lane.
@@ -623,6 +703,32 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
         q2n = 0;
     };
 
+    // ---- stage 2b: the segment-mean bound over queued LB_Kim survivors (at most 64 of them),
+    // one pair per lane, from the series' table in global memory; survivors go on to LB_Keogh.
+    // The entries left (fewer than 64) move to the front of the queue.
+    auto process_qp = [&]() __attribute__((always_inline)) {
+        const JoinArgs &A = *Ap;
+        __syncthreads();  // qp entries were written by other lanes
+        const int cnt = qpn < 64 ? qpn : 64;
+        const int rest = qpn - cnt;
+        bool alive = false;
+        uint32_t pr = 0u;
+        if (lane < cnt) {
+            pr = qp[lane];
+            alive = paa_pass(A.paa, cb + pair_row(pr), cb + d0 + pair_col(pr), A.paa_seg, (float)A.paa_len, th.T2);
+        }
+        const uint32_t tail = lane < rest ? qp[64 + lane] : 0u;
+        __syncthreads();  // every lane has read its entry before the queue moves down
+        if (lane < rest) qp[lane] = tail;
+        qpn = rest;
+        const unsigned long long msk = __builtin_amdgcn_ballot_w64(alive);
+        if (msk) {
+            if (alive) q2[q2n + (int)mbcnt64(msk)] = pr;
+            q2n += __popcll(msk);
+            count(C_PAA, __popcll(msk));
+        }
+    };
+
     // ---- drain: the queue entries (sorted by row) are taken in parts of RC rows: stage the
     // part in LDS, expand its entries into pairs, LB_Kim (stage 2) on 64-lane batches, feeding
     // LB_Keogh and DTW; everything is flushed before the next part is staged.
@@ -808,12 +914,19 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                     alive = kim <= th.kim;
                 }
                 const unsigned long long msk = __builtin_amdgcn_ballot_w64(alive);
+                const bool paa = A.paa_seg > 0;
                 if (msk) {
-                    if (alive) q2[q2n + (int)mbcnt64(msk)] = pack_pair(pi, pj);
-                    q2n += __popcll(msk);
+                    if (paa) {
+                        if (alive) qp[qpn + (int)mbcnt64(msk)] = pack_pair(pi, pj);
+                        qpn += __popcll(msk);
+                    } else {
+                        if (alive) q2[q2n + (int)mbcnt64(msk)] = pack_pair(pi, pj);
+                        q2n += __popcll(msk);
+                    }
                     count(C_KIM, __popcll(msk));
                 }
                 // queued pairs refer to this part's staging: all of them run before the next part
+                if (paa && (qpn >= 64 || (last && qpn > 0))) process_qp();
                 if (q2n >= Q2RUN || (last && q2n > 0)) process_q2();
                 if (p + 64 >= total) break;
             }
@@ -1478,9 +1591,26 @@ __global__ __launch_bounds__(64) void k_seed_dtw_batch(const JoinArgs *__restric
 
 }  // namespace
 
+namespace {
+// Segment-mean bound of explicit window pairs (the building block behind dtwj_paa_pairs).
+__global__ __launch_bounds__(256) void k_paa_pairs(const float *__restrict__ paa, int32_t nseg, int32_t len,
+                                                   const int64_t *pi, const int64_t *pj, int64_t npairs, double *out) {
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k < npairs) out[k] = (double)paa_lb(paa, pi[k], pj[k], nseg, (float)len);
+}
+}  // namespace
+
+int launch_paa_pairs(const float *paa, int32_t nseg, int32_t len, const int64_t *pi, const int64_t *pj, int64_t npairs,
+                     double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_paa_pairs, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, s, paa, nseg, len, pi, pj,
+                       npairs, out);
+    DTWJ_HIP(hipGetLastError());
+    return 0;
+}
+
 size_t join_lds_bytes(int32_t m, int32_t w) {
     return sizeof(float) * (size_t)union_floats(m, packed_envelope(dtw_qp_for(w))) +
-           sizeof(uint32_t) * (PLCAP + Q2CAP + Q3CAP);
+           sizeof(uint32_t) * (PLCAP + Q2CAP + Q3CAP + QPCAP);
 }
 
 int launch_join_setup(JoinArgs *d_series, int32_t nseries, unsigned long long *d_work, hipStream_t s) {
@@ -1494,7 +1624,7 @@ int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t ns
     const int Q = dtw_qp_for(max_w);
     const size_t lds = join_lds_bytes(max_m, max_w);
     const int32_t uni = union_floats(max_m, packed_envelope(Q));
-    if (lds + 64 > 160 * 1024) {  // + the kernel's static stage counters
+    if (lds + 8 * kWcnt > 160 * 1024) {  // + the kernel's static stage counters
         set_error("window too long for the join's LDS staging");
         return DTWJ_EINVAL;
     }

@@ -144,7 +144,7 @@ __device__ inline dd_t exact_sq(double x) {
     return {p, fma(x, x, -p)};
 }
 __device__ inline void scan_values(const double *__restrict__ a, const double *__restrict__ U,
-                                   const double *__restrict__ L, double c0, int64_t t, dd_t x[5]) {
+                                   const double *__restrict__ L, double c0, int64_t t, dd_t x[kPrepScanChannels]) {
     double ad = a[t] - c0;
     double u = U[t], l = L[t];
     double vd = (0.5 * u + 0.5 * l) - c0;
@@ -154,6 +154,7 @@ __device__ inline void scan_values(const double *__restrict__ a, const double *_
     x[2] = {vd, 0.0};
     x[3] = exact_sq(vd);
     x[4] = exact_sq(h);
+    x[5] = {h, 0.0};
 }
 
 // Inclusive block scan of one dd value per thread (Hillis-Steele in LDS).
@@ -179,18 +180,18 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partial(const double *__r
                                                                int64_t n, int64_t nb, dd_t *bsum) {
     __shared__ dd_t buf[kScanThreads];
     const double c0 = *c0p;
-    dd_t acc[5];
-    for (int c = 0; c < 5; ++c) acc[c] = {0.0, 0.0};
+    dd_t acc[kPrepScanChannels];
+    for (int c = 0; c < kPrepScanChannels; ++c) acc[c] = {0.0, 0.0};
     int64_t base = (int64_t)blockIdx.x * kScanBlock + (int64_t)threadIdx.x * kScanPerThread;
     for (int k = 0; k < kScanPerThread; ++k) {
         int64_t t = base + k;
         if (t < n) {
-            dd_t x[5];
+            dd_t x[kPrepScanChannels];
             scan_values(a, U, L, c0, t, x);
-            for (int c = 0; c < 5; ++c) acc[c] = dd_add(acc[c], x[c]);
+            for (int c = 0; c < kPrepScanChannels; ++c) acc[c] = dd_add(acc[c], x[c]);
         }
     }
-    for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < kPrepScanChannels; ++c) {
         dd_t s = block_scan_dd(acc[c], buf);
         if (threadIdx.x == kScanThreads - 1) bsum[(int64_t)c * nb + blockIdx.x] = s;
     }
@@ -220,19 +221,19 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const double *__res
     __shared__ dd_t buf[kScanThreads];
     const double c0 = *c0p;
     int64_t base = (int64_t)blockIdx.x * kScanBlock + (int64_t)threadIdx.x * kScanPerThread;
-    dd_t xs[kScanPerThread][5];
-    dd_t tot[5];
-    for (int c = 0; c < 5; ++c) tot[c] = {0.0, 0.0};
+    dd_t xs[kScanPerThread][kPrepScanChannels];
+    dd_t tot[kPrepScanChannels];
+    for (int c = 0; c < kPrepScanChannels; ++c) tot[c] = {0.0, 0.0};
     for (int k = 0; k < kScanPerThread; ++k) {
         int64_t t = base + k;
         if (t < n) {
             scan_values(a, U, L, c0, t, xs[k]);
         } else {
-            for (int c = 0; c < 5; ++c) xs[k][c] = {0.0, 0.0};
+            for (int c = 0; c < kPrepScanChannels; ++c) xs[k][c] = {0.0, 0.0};
         }
-        for (int c = 0; c < 5; ++c) tot[c] = dd_add(tot[c], xs[k][c]);
+        for (int c = 0; c < kPrepScanChannels; ++c) tot[c] = dd_add(tot[c], xs[k][c]);
     }
-    for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < kPrepScanChannels; ++c) {
         dd_t inc = block_scan_dd(tot[c], buf);
         dd_t run = dd_add(bsum[(int64_t)c * nb + blockIdx.x], dd_sub(inc, tot[c]));
         dd_t *Pc = P + (int64_t)c * (n + 1);
@@ -393,6 +394,37 @@ __global__ __launch_bounds__(256) void k_positions(const double *__restrict__ a,
 
 int64_t prep_scan_blocks(int64_t n) { return (n + kScanBlock - 1) / kScanBlock; }
 
+// Segment-mean table (layout at paa_segments): one thread per (window, segment), segments
+// fastest so that the stores of a block are contiguous.  The segment means are differences of
+// the compensated prefix sums (channels a - c0, v - c0 and h), taken against the window mean in
+// double-double before the division by sigma, so a large offset of the series costs no accuracy.
+// |Z| <= sqrt(m) (a mean of z-values), so Z rounded to FP32 is within 2^-24 sqrt(m) of its FP64
+// value; U and L are widened by twice that (which also covers the FP64 rounding here) and then
+// rounded outwards, so that the FP32 table bounds every window pair's exact segment terms.
+__global__ __launch_bounds__(256) void k_paa_tables(const dd_t *__restrict__ P, int64_t n, int m, int nseg,
+                                                    const double *__restrict__ sig, float *paa, int64_t nw) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nw * nseg) return;
+    const int64_t p = t / nseg;
+    const int sg = (int)(t - p * nseg);
+    const int len = m / nseg;
+    const dd_t *P0 = P, *P2 = P + 2 * (n + 1), *P5 = P + 5 * (n + 1);
+    const dd_t mean = dd_div_d(dd_sub(P0[p + m], P0[p]), (double)m);
+    const int64_t b = p + (int64_t)sg * len;
+    const dd_t za = dd_sub(dd_div_d(dd_sub(P0[b + len], P0[b]), (double)len), mean);
+    const dd_t zv = dd_sub(dd_div_d(dd_sub(P2[b + len], P2[b]), (double)len), mean);
+    const dd_t hh = dd_div_d(dd_sub(P5[b + len], P5[b]), (double)len);
+    const double is = 1.0 / sig[p];
+    const double z = (za.hi + za.lo) * is;
+    const double mv = (zv.hi + zv.lo) * is;
+    const double h = (hh.hi + hh.lo) * is;
+    const double e = 0x1p-23 * sqrt((double)m);
+    float *row = paa + p * 3 * nseg;
+    row[sg] = __double2float_ru(mv + h + e);
+    row[nseg + sg] = __double2float_rd(mv - h - e);
+    row[2 * nseg + sg] = (float)z;
+}
+
 int prep_series(Prep &P, hipStream_t s) {
     const int64_t n = P.n;
     const int m = P.m;
@@ -408,13 +440,20 @@ int prep_series(Prep &P, hipStream_t s) {
                        sizeof(double) * 2 * (size_t)(etile + 2 * P.w), s, P.a, n, P.w, etile, P.U, P.L, P.v);
     hipLaunchKernelGGL(k_scan_partial, dim3((unsigned)nb), dim3(kScanThreads), 0, s, P.a, P.U, P.L, P.c0, n, nb,
                        P.bsum);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(5), dim3(kScanThreads), 0, s, P.bsum, nb);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(kPrepScanChannels), dim3(kScanThreads), 0, s, P.bsum, nb);
     hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(kScanThreads), 0, s, P.a, P.U, P.L, P.c0, n, nb, P.bsum,
                        P.P);
     hipLaunchKernelGGL(k_window_stats, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, P.a, P.v, P.P, P.c0, n, m,
                        P.mu, P.sig, P.muv, P.mu_c, P.win, P.blkmax, P.nblk, P.gstat, P.flags);
     hipLaunchKernelGGL(k_positions, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P.a, P.U, P.L, P.c0, n, P.ac,
                        P.uc, P.lc, P.wc, P.gstat);
+    if (P.paa) {
+        if ((P.paa_seg != 16 && P.paa_seg != 32) || m < P.paa_seg)
+            return hip_fail(hipErrorInvalidValue, "segment-mean table", __FILE__, __LINE__);
+        const int64_t cells = nw * P.paa_seg;
+        hipLaunchKernelGGL(k_paa_tables, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, P.P, n, m, P.paa_seg,
+                           P.sig, P.paa, nw);
+    }
     DTWJ_HIP(hipGetLastError());
     return 0;
 }

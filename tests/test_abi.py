@@ -208,6 +208,25 @@ def test_plan_rejects_what_the_join_rejects(dtwj):
         dtwj.plan(100000, 1024, 600)
 
 
+def test_plan_counts_the_segment_mean_table(dtwj):
+    """The segment-mean stage's table (3 S FP32 values per window, S = 16 below m = 512) is in
+    the plan when the stage is on: automatically for w >= 8 and m >= 128, or when forced; paa
+    outside 0..2 is rejected."""
+    n = 1_000_000
+    nw = n - 256 + 1
+    off = dtwj.plan(n, 256, 13, paa=2).device_bytes
+    assert dtwj.plan(n, 256, 13).device_bytes - off >= 192 * nw
+    assert dtwj.plan(n, 256, 13, paa=1).device_bytes == dtwj.plan(n, 256, 13).device_bytes
+    assert dtwj.plan(n, 256, 2).device_bytes == dtwj.plan(n, 256, 2, paa=2).device_bytes
+    assert dtwj.plan(n, 256, 2, paa=1).device_bytes - dtwj.plan(n, 256, 2).device_bytes >= 192 * nw
+    assert dtwj.plan(n, 64, 13).device_bytes == dtwj.plan(n, 64, 13, paa=2).device_bytes
+    nw = n - 1024 + 1
+    assert dtwj.plan(n, 1024, 102).device_bytes - dtwj.plan(n, 1024, 102, paa=2).device_bytes >= 384 * nw
+    for bad in (-1, 3):
+        with pytest.raises(dtwj.DTWJError):
+            dtwj.plan(n, 256, 13, paa=bad)
+
+
 def test_result_mirror_maps_every_field(dtwj):
     """Every field of dtwj_result reaches the Python result object (no GPU involved)."""
     r = dtwj.Result()
