@@ -419,10 +419,15 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
         const int per_wave = 64 / seg_len;
         const int seg = lane / seg_len;
         const bool leader = lane == seg * seg_len;
+        // The suffix sums of pair s are formed by the lanes [s slen, (s + 1) slen), slen =
+        // 64 / per_wave >= seg_len: the lanes a segment leaves idle in the DTW share the block
+        // sums (w = 51: 64 lanes instead of 52, one round for m = 512 instead of two).
+        const int slen = 64 / per_wave;
+        const int sseg = lane / slen;  // pair whose block sums this lane forms
         // suffix-bound abandon for bands of 16 and more (narrower bands end their DTWs within a
         // few checks anyway) when the blocks fit SUF_Q rounds and m keeps the FP32 margin valid
         const int suf_blocks = (m - 8 - ((w + 1) >> 1) + 7) / 8;  // blocks with a row below m
-        const int suf_rounds = suf_blocks > 0 ? (suf_blocks + seg_len - 1) / seg_len : 0;
+        const int suf_rounds = suf_blocks > 0 ? (suf_blocks + slen - 1) / slen : 0;
         const bool use_suffix = w >= 16 && m <= 8192 && suf_rounds <= SUF_Q;
         // the first batch uses the current thresholds; each later one the best-so-far word
         // loaded one batch earlier (an older word is only larger: still valid)
@@ -448,21 +453,37 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
             int t_stop = 0;
             float res;
             // block sums of the LB_Keogh terms of the lane's blocks and their reverse prefix sums
-            // over the segment's blocks (rounds q from last to first, carrying the totals)
+            // over the pair's blocks (rounds q from last to first, carrying the totals); the
+            // check of a DTW segment reads the sums of its pair from lanes seg slen + ...
             LaneSuffix suf;
-            suf.base = seg * seg_len;
-            suf.seg_len = seg_len;
+            suf.base = seg * slen;
+            suf.seg_len = slen;
             suf.nq = use_suffix ? suf_rounds : 0;
-            {
-                const int x = lane - seg * seg_len;
+#pragma unroll
+            for (int q = 0; q < SUF_Q; ++q) suf.reg[q] = 0.f;
+            if (suf.nq > 0) {
+                const int x = lane - sseg * slen;
+                const int sbase = sseg * slen;
+                int si = 0, sj = 0;
+                float sii = 0.f, snmi = 0.f, sij = 0.f, snmj = 0.f;
+                if (sseg < nseg) {
+                    const uint32_t pr = q3[e0 + sseg];
+                    si = pair_row(pr);
+                    sj = pair_col(pr);
+                    sii = L.ris[si];
+                    snmi = -L.rmu[si] * sii;
+                    sij = L.cis[sj];
+                    snmj = -L.cmu[sj] * sij;
+                }
+                const StagedZ zs{L.sai + si, L.saj + sj, sii, snmi, sij, snmj};
                 float cA = 0.f, cB = 0.f;
 #pragma unroll
                 for (int q = SUF_Q - 1; q >= 0; --q) {
                     suf.reg[q] = 0.f;
                     if (q < suf.nq) {
                         float vA = 0.f, vB = 0.f;
-                        const int r0 = 8 * (x + seg_len * q + 1) + ((w + 1) >> 1);
-                        if (seg < nseg && r0 < m) {
+                        const int r0 = 8 * (x + slen * q + 1) + ((w + 1) >> 1);
+                        if (sseg < nseg && r0 < m) {
                             // the lane's 8 rows in an order skewed by the lane index: at each
                             // step the lanes' addresses differ by 9 (odd), so no bank conflicts
 #pragma unroll
@@ -470,13 +491,13 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                                 const int r = r0 + ((rr + x) & 7);
                                 if (r < m) {
                                     // envelope z-values by the same map as the series'
-                                    const float xi = L.sai[li + r], xj = L.saj[lj + r];
+                                    const float xi = L.sai[si + r], xj = L.saj[sj + r];
                                     float Ui, Li, Uj, Lj;
-                                    L.env_i(li + r, xi, Ui, Li);
-                                    L.env_j(lj + r, xj, Uj, Lj);
-                                    const float zi = z.row(r), zj = z.col(r);
-                                    const float uj = fmaf(Uj, ij, nmj), lwj = fmaf(Lj, ij, nmj);
-                                    const float ui = fmaf(Ui, ii, nmi), lwi = fmaf(Li, ii, nmi);
+                                    L.env_i(si + r, xi, Ui, Li);
+                                    L.env_j(sj + r, xj, Uj, Lj);
+                                    const float zi = zs.row(r), zj = zs.col(r);
+                                    const float uj = fmaf(Uj, sij, snmj), lwj = fmaf(Lj, sij, snmj);
+                                    const float ui = fmaf(Ui, sii, snmi), lwi = fmaf(Li, sii, snmi);
                                     const float ta = fmaxf(fmaxf(zi - uj, lwj - zi), 0.f);
                                     const float tb = fmaxf(fmaxf(zj - ui, lwi - zj), 0.f);
                                     vA = fmaf(ta, ta, vA);
@@ -487,15 +508,15 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
 #pragma unroll
                         for (int off = 1; off < 64; off <<= 1) {
                             const float yA = __shfl_down(vA, off, 64), yB = __shfl_down(vB, off, 64);
-                            if (x + off < seg_len) {
+                            if (x + off < slen) {
                                 vA += yA;
                                 vB += yB;
                             }
                         }
                         vA += cA;
                         vB += cB;
-                        cA = __shfl(vA, suf.base, 64);
-                        cB = __shfl(vB, suf.base, 64);
+                        cA = __shfl(vA, sbase, 64);
+                        cB = __shfl(vB, sbase, 64);
                         suf.reg[q] = fmaxf(vA, vB);
                     }
                 }
