@@ -50,8 +50,9 @@ typedef struct dtwj_options {
     int32_t verbose;       /* print per-stage counters to stderr */
     int32_t graph;         /* batch: capture prep + seed + join in one HIP graph (default 0 = on,
                               -1 = launch kernels directly) */
-    int32_t autotune;      /* batch: 1 = pick rows_per_item / waves_per_cu by timing probe joins
-                              on a sample of the batch (0 = use the fields above or defaults) */
+    int32_t autotune;      /* batch: 1 = pick rows_per_item / waves_per_cu by timing one join of
+                              the whole batch per candidate, kept per context and batch shape
+                              (0 = use the fields above or defaults) */
     int32_t sweep;         /* diagonal sweep test: 0 = adaptive per work item (default), 1 = always
                               the two-sided ball bound, 2 = always the one-sided one (both exact) */
     int32_t cand_cap;      /* near-tie candidate slots of a single-series join (0 = 2^20; a batch
