@@ -355,10 +355,9 @@ __device__ inline int pair_col(uint32_t p) { return (int)(p >> 16); }
 // device work queue is the range [0, item_base[nseries]) claimed through work_counter), then
 // the claim-order multiplier;
 // uni: floats of the LDS union area, sized for the longest window of the launch.
-template <int Q>
+template <int Q, bool PW>
 __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base,
                                           int32_t nseries, int32_t uni, unsigned long long *work_counter, uint32_t *queue) {
-    constexpr bool PW = packed_envelope(Q);
     extern __shared__ __align__(16) float smem[];  // 16-byte aligned: the sweep reads float4 rows
     const int lane = (int)threadIdx.x;
     uint32_t *q1 = queue + (size_t)blockIdx.x * kJoinQueueCap;  // this wave's survivor queue
@@ -1405,10 +1404,10 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     flush_counters();
 }
 
-template <int Q>
+template <int Q, bool PW>
 __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base,
                                              int32_t nseries, int32_t uni, unsigned long long *work_counter, uint32_t *queue) {
-    join_body<Q>(series, item_base, nseries, uni, work_counter, queue);
+    join_body<Q, PW>(series, item_base, nseries, uni, work_counter, queue);
 }
 
 // The same join held to the register budget of two waves per SIMD (a few spills to scratch):
@@ -1418,7 +1417,7 @@ template <int Q>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_join_occ2(
     const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base, int32_t nseries, int32_t uni,
     unsigned long long *work_counter, uint32_t *queue) {
-    join_body<Q>(series, item_base, nseries, uni, work_counter, queue);
+    join_body<Q, true>(series, item_base, nseries, uni, work_counter, queue);
 }
 
 // ---- set-up of every series of a join: the FP32 error model from the preprocessing's value
@@ -1629,10 +1628,21 @@ int launch_paa_pairs(const float *paa, int32_t nseg, int32_t len, const int64_t 
     return 0;
 }
 
-size_t join_lds_bytes(int32_t m, int32_t w) {
-    return sizeof(float) * (size_t)union_floats(m, packed_envelope(dtw_qp_for(w))) +
-           sizeof(uint32_t) * (PLCAP + Q2CAP + Q3CAP + QPCAP);
+namespace {
+size_t join_lds_bytes_pw(int32_t m, bool pw) {
+    return sizeof(float) * (size_t)union_floats(m, pw) + sizeof(uint32_t) * (PLCAP + Q2CAP + Q3CAP + QPCAP);
 }
+// Packed FP16 envelope staging: always for two offset pairs per lane (k_join_occ2), and for one
+// pair per lane when FP32 staging of windows of length m would leave fewer than the 8 waves per
+// CU its registers allow (m > 402: at m = 512, 7 waves instead of 8).
+bool join_packed(int Q, int32_t m) {
+    if (packed_envelope(Q)) return true;
+    if (Q != 1) return false;
+    return 160 * 1024 / (join_lds_bytes_pw(m, false) + 8 * kWcnt) < 8;
+}
+}  // namespace
+
+size_t join_lds_bytes(int32_t m, int32_t w) { return join_lds_bytes_pw(m, join_packed(dtw_qp_for(w), m)); }
 
 int launch_join_setup(JoinArgs *d_series, int32_t nseries, unsigned long long *d_work, hipStream_t s) {
     hipLaunchKernelGGL(k_join_setup, dim3((unsigned)((nseries + 63) / 64)), dim3(64), 0, s, d_series, nseries, d_work);
@@ -1643,23 +1653,29 @@ int launch_join_setup(JoinArgs *d_series, int32_t nseries, unsigned long long *d
 int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t nseries, int32_t max_m, int32_t max_w,
                 unsigned long long *d_work, uint32_t *d_queue, int32_t grid, hipStream_t s) {
     const int Q = dtw_qp_for(max_w);
+    const bool pw = join_packed(Q, max_m);
     const size_t lds = join_lds_bytes(max_m, max_w);
-    const int32_t uni = union_floats(max_m, packed_envelope(Q));
+    const int32_t uni = union_floats(max_m, pw);
     if (lds + 8 * kWcnt > 160 * 1024) {  // + the kernel's static stage counters
         set_error("window too long for the join's LDS staging");
         return DTWJ_EINVAL;
     }
-#define DTWJ_LAUNCH_JOIN(QQ)                                                                                  \
-    hipLaunchKernelGGL(k_join<QQ>, dim3(grid), dim3(64), lds, s, d_series, d_item_base, nseries, uni, d_work, \
-                       d_queue)
+#define DTWJ_LAUNCH_JOIN(QQ, PP)                                                                                  \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_join<QQ, PP>), dim3(grid), dim3(64), lds, s, d_series, d_item_base, nseries, \
+                       uni, d_work, d_queue)
     switch (Q) {
-        case 1: DTWJ_LAUNCH_JOIN(1); break;
+        case 1:
+            if (pw)
+                DTWJ_LAUNCH_JOIN(1, true);
+            else
+                DTWJ_LAUNCH_JOIN(1, false);
+            break;
         case 2:  // two waves per SIMD (packed envelope staging)
             hipLaunchKernelGGL(k_join_occ2<2>, dim3(grid), dim3(64), lds, s, d_series, d_item_base, nseries, uni,
                                d_work, d_queue);
             break;
-        case 4: DTWJ_LAUNCH_JOIN(4); break;
-        case 8: DTWJ_LAUNCH_JOIN(8); break;
+        case 4: DTWJ_LAUNCH_JOIN(4, false); break;
+        case 8: DTWJ_LAUNCH_JOIN(8, false); break;
         default: set_error("band too wide: 2w+1 must be <= 1024"); return DTWJ_EINVAL;
     }
 #undef DTWJ_LAUNCH_JOIN
