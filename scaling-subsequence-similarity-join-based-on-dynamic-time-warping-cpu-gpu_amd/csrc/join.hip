@@ -171,16 +171,41 @@ __device__ inline float range_max(const JoinArgs &A, int which, int64_t lo, int6
     return r;
 }
 
-// z-values of a DTW pair normalised on the fly from the staged chunk, x / sigma - mean / sigma
-// in one FMA (nmi = -mean / sigma).  Indices may leave the window by up to (w + 3) / 2 + 1 on
-// either side: the cascade layout keeps finite staged values there (see CascadeLDS).
+// A DTW pair read from the staged chunk as raw values: the difference of a cell is formed as
+// fma(x_c, nij, fma(x_r, ii, c)) with ii = 1/sigma_i, nij = -1/sigma_j and c = the difference
+// of the two windows' -mean/sigma (pair_bias), i.e. z_i(r) - z_j(c) without normalising the
+// columns.  Its error stays within the 2 eps_z the DTW error model allows per difference:
+// 2^-24 (6 max|x| max(1/sigma) + 8 max|mean| max(1/sigma) + |d|) <= 2^-21 (max|x| +
+// max|mean|) max(1/sigma) + 2^-23 sqrt(m).  Indices may leave the window by up to
+// (w + 3) / 2 + 1 on either side: the cascade layout keeps finite staged values there (see
+// CascadeLDS).
 struct StagedZ {
     static constexpr bool kPadded = true;
+    static constexpr bool kFused = true;
+    const float *xi, *xj;
+    float ii, nij, c;
+    __device__ float xrow(int r) const { return xi[r]; }
+    __device__ float xcol(int q) const { return xj[q]; }
+};
+// The same pair as z-values, each normalised on its own, x / sigma - mean / sigma in one FMA
+// (nmi = -mean / sigma): the form of the kernels with one offset pair per lane (bands up to
+// 63), where the fused form measured slower through the code generated around the sweep.
+struct StagedZPlain {
+    static constexpr bool kPadded = true;
+    static constexpr bool kFused = false;
     const float *xi, *xj;
     float ii, nmi, ij, nmj;
     __device__ float row(int r) const { return fmaf(xi[r], ii, nmi); }
     __device__ float col(int c) const { return fmaf(xj[c], ij, nmj); }
 };
+
+// c = (-mu_i / sigma_i) - (-mu_j / sigma_j) of a staged pair, with the products rounded on
+// their own: the DTW and the suffix bound of the same pair must obtain the same bits.
+__device__ inline float pair_bias(float mi, float ii, float mj, float ij) {
+    DTWJ_NO_CONTRACT
+    const float nmi = -mi * ii, nmj = -mj * ij;
+    return nmi - nmj;
+}
 
 // Suffix bound of the DTW of each segment's pair (see dtw_device.hpp): block kappa holds the
 // LB_Keogh terms (both directions) of rows / columns [8 (kappa + 1) + ceil(w/2), + 8), and the
@@ -437,6 +462,8 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 th = threshold_of(key, A);
                 key = load_bsf(A);
             }
+            // fused cell differences in the kernels with several offset pairs per lane
+            constexpr bool FZ = Q >= 2;
             int li = 0, lj = 0;
             float ii = 0.f, nmi = 0.f, ij = 0.f, nmj = 0.f;
             if (seg < nseg) {
@@ -448,7 +475,12 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 ij = L.cis[lj];
                 nmj = -L.cmu[lj] * ij;
             }
-            const StagedZ z{L.sai + li, L.saj + lj, ii, nmi, ij, nmj};
+            using ZT = std::conditional_t<FZ, StagedZ, StagedZPlain>;
+            ZT z;
+            if constexpr (FZ)
+                z = StagedZ{L.sai + li, L.saj + lj, ii, -ij, seg < nseg ? pair_bias(L.rmu[li], ii, L.cmu[lj], ij) : 0.f};
+            else
+                z = StagedZPlain{L.sai + li, L.saj + lj, ii, nmi, ij, nmj};
             int t_stop = 0;
             float res;
             // block sums of the LB_Keogh terms of the lane's blocks and their reverse prefix sums
@@ -474,7 +506,11 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                     sij = L.cis[sj];
                     snmj = -L.cmu[sj] * sij;
                 }
-                const StagedZ zs{L.sai + si, L.saj + sj, sii, snmi, sij, snmj};
+                const StagedZPlain zs{L.sai + si, L.saj + sj, sii, snmi, sij, snmj};
+                float sc = 0.f;  // the fused form's pair bias, as the DTW forms it
+                if constexpr (FZ) {
+                    if (sseg < nseg) sc = pair_bias(L.rmu[si], sii, L.cmu[sj], sij);
+                }
                 float cA = 0.f, cB = 0.f;
 #pragma unroll
                 for (int q = SUF_Q - 1; q >= 0; --q) {
@@ -489,16 +525,32 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                             for (int rr = 0; rr < 8; ++rr) {
                                 const int r = r0 + ((rr + x) & 7);
                                 if (r < m) {
-                                    // envelope z-values by the same map as the series'
+                                    // Terms by the DTW's own arithmetic (StagedZ, or the
+                                    // z-values of StagedZPlain): a cell's FP32 difference is
+                                    // monotone in x_c and in x_r, so with the envelope values in
+                                    // place of x_c (row r, direction A) or of x_r (column r,
+                                    // direction B) the results bound every cell of that row /
+                                    // column.
                                     const float xi = L.sai[si + r], xj = L.saj[sj + r];
                                     float Ui, Li, Uj, Lj;
                                     L.env_i(si + r, xi, Ui, Li);
                                     L.env_j(sj + r, xj, Uj, Lj);
-                                    const float zi = zs.row(r), zj = zs.col(r);
-                                    const float uj = fmaf(Uj, sij, snmj), lwj = fmaf(Lj, sij, snmj);
-                                    const float ui = fmaf(Ui, sii, snmi), lwi = fmaf(Li, sii, snmi);
-                                    const float ta = fmaxf(fmaxf(zi - uj, lwj - zi), 0.f);
-                                    const float tb = fmaxf(fmaxf(zj - ui, lwi - zj), 0.f);
+                                    float ta, tb;
+                                    if constexpr (FZ) {
+                                        const float zr = fmaf(xi, sii, sc);
+                                        const float dU = fmaf(Uj, -sij, zr), dL = fmaf(Lj, -sij, zr);
+                                        const float dlo = fmaf(xj, -sij, fmaf(Li, sii, sc));
+                                        const float dhi = fmaf(xj, -sij, fmaf(Ui, sii, sc));
+                                        ta = fmaxf(fmaxf(dU, -dL), 0.f);
+                                        tb = fmaxf(fmaxf(dlo, -dhi), 0.f);
+                                    } else {
+                                        // z-values (StagedZPlain): one FMA each, monotone
+                                        const float zi = zs.row(r), zj = zs.col(r);
+                                        const float uj = fmaf(Uj, sij, snmj), lwj = fmaf(Lj, sij, snmj);
+                                        const float ui = fmaf(Ui, sii, snmi), lwi = fmaf(Li, sii, snmi);
+                                        ta = fmaxf(fmaxf(zi - uj, lwj - zi), 0.f);
+                                        tb = fmaxf(fmaxf(zj - ui, lwi - zj), 0.f);
+                                    }
                                     vA = fmaf(ta, ta, vA);
                                     vB = fmaf(tb, tb, vB);
                                 }
