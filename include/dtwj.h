@@ -302,6 +302,33 @@ int dtwj_dtw_pairs(const double *series, int64_t n, int32_t m, int32_t w, const 
 int dtwj_paa_pairs(const double *series, int64_t n, int32_t m, int32_t w, const int64_t *pi,
                    const int64_t *pj, int64_t npairs, double *lb);
 
+/* The reference's global occupancy mask of the series' positions, replacing
+ * compute_shared_data_global + compute_MASK_global (CPU_version1/compute_shared_data_global.cpp:12-46,
+ * CPU_version1/compute_MASK_global.cpp:11-93; GPU_version1/src/compute_shared_data_global.cu:48-81,
+ * GPU_version1/src/compute_MASK_global.cu:92-171; table length as
+ * CPU_version1/new_dtw_motifGUI.cpp:151-176), computed on the GPU from the library's FP64 window
+ * statistics with the reference's operations in its order: per window offset a histogram of
+ * the z-values in bins of 0.01, per (window, offset) the number of windows inside that window's
+ * clipped envelope there, summed per position over its windows and divided by its coverage,
+ * MASK = 1 where that is at most its mean.  mask: n bytes (0 / 1).  Optional (may be NULL):
+ * occupancy (n doubles, the statistic compared), threshold (its mean), len_of_table (histogram
+ * bins per offset).  Needs 8 <= m <= 16384 and w <= 256.  The join does not use the mask: the
+ * masked ball bound's running sums would cost its sweep more per pair than the cascade work
+ * they remove (the cascade's own bounds after the ball bound are cheap on the GPU). */
+int dtwj_global_mask(const double *series, int64_t n, int32_t m, int32_t w, uint8_t *mask,
+                     double *occupancy, double *threshold, int32_t *len_of_table);
+
+/* The reference's masked ball bound (CPU_version1/diag_mask_global.cpp:11-226,
+ * GPU_version1/lib/diag_mask_global.cu:12-230) for an explicit pair list, both directions:
+ * lb[2p] with the envelope of window pi[p] against window pj[p], lb[2p + 1] the reverse.  Over the
+ * offsets K = {k in [3, m-4] : MASK[e + k] = 1} of the envelope window e (MASK as
+ * dtwj_global_mask), dist = sum_K (z_d - Uhat_e)^2 + (z_d - Lhat_e)^2 and DUL2 = the squared norm
+ * of (U - L) / sigma_e over the masked positions of the whole window; lb = 0 if dist < DUL2, else
+ * (sqrt(2 dist - DUL2) - sqrt(DUL2)) / 2: a lower bound of the banded DTW distance (not squared).
+ * FP64, direct sums (the reference keeps running sums along a diagonal).  Same limits. */
+int dtwj_masked_ball_pairs(const double *series, int64_t n, int32_t m, int32_t w, const int64_t *pi,
+                           const int64_t *pj, int64_t npairs, double *lb);
+
 /* ---- synthetic workloads ------------------------------------------------------------ */
 typedef struct dtwj_gen_spec {
     int64_t n;             /* series length */

@@ -586,7 +586,7 @@ static void lb_kim(join_state *st, double threshold, int64_t temp_1, int diag, d
 }
 
 /* CPU_version1/diag_mask_global.cpp:11-226 -- masked ball bound + LB_Kim end points */
-static void diag_mask_global(join_state *st, int diag, double bsf, double *cnt) {
+static void diag_mask_global(join_state *st, int diag, double bsf, double *cnt, double *out_fir, double *out_sec) {
     const double *ts = st->a, *mu = st->mu, *invsig = st->invsig;
     const double *sumMASK = st->sumMASK_g, *sumU_sumL = st->sumU_sumL_g, *nUL = st->normUL_g;
     const double *DUL2 = st->DUL2_g, *DUL = st->DUL_g;
@@ -615,6 +615,7 @@ static void diag_mask_global(join_state *st, int diag, double bsf, double *cnt) 
         else
             lb = 0.5 * (sqrt(2 * dist - DUL2[row]) - DUL[row]);
         st->lb_vector_new[row] = lb;
+        if (out_fir) out_fir[0] = lb;
         if (lb > bsf) {
             st->lb_vector[row] = 1;
             (*cnt)++;
@@ -632,6 +633,7 @@ static void diag_mask_global(join_state *st, int diag, double bsf, double *cnt) 
             else
                 lb = 0.5 * (sqrt(2 * dist - DUL2[row]) - DUL[row]);
             st->lb_vector_new[col] = lb;
+            if (out_sec) out_sec[0] = lb;
             if (lb > bsf) {
                 st->lb_vector[col] = 1;
                 (*cnt)++;
@@ -653,8 +655,10 @@ static void diag_mask_global(join_state *st, int diag, double bsf, double *cnt) 
         dist = 2 * M_NORM - 2 * t + nUL[lo];
         if (dist < DUL2[lo]) {
             lb = 0;
+            if (out_fir) out_fir[lo] = lb;
         } else {
             lb = 0.5 * (sqrt(2 * dist - DUL2[lo]) - DUL[lo]);
+            if (out_fir) out_fir[lo] = lb;
             st->lb_vector_new[lo] = lb;
             if (lb > bsf) {
                 st->lb_vector[lo] = 1;
@@ -668,8 +672,10 @@ static void diag_mask_global(join_state *st, int diag, double bsf, double *cnt) 
         dist = 2 * M_NORM - 2 * t + nUL[hi];
         if (dist < DUL2[hi]) {
             lb = 0;
+            if (out_sec) out_sec[lo] = lb;
         } else {
             lb = 0.5 * (sqrt(2 * dist - DUL2[hi]) - DUL[hi]);
+            if (out_sec) out_sec[lo] = lb;
             st->lb_vector_new[lo] = smax(lb, st->lb_vector_new[lo]);
             if (lb > bsf) {
                 st->lb_vector[lo] = 1;
@@ -687,7 +693,7 @@ static void diag_mask_global(join_state *st, int diag, double bsf, double *cnt) 
 }
 
 /* CPU_version1/compute_shared_data_global.cpp:12-46 and CPU_version1/compute_MASK_global.cpp:11-93 */
-static void mask_global(join_state *st, int len_of_table) {
+static void mask_global(join_state *st, int len_of_table, double *diff_out, double *ths_out) {
     int m = st->m;
     int64_t subcount = st->subcount, len = st->len;
     double *ct = (double *)calloc((size_t)m * len_of_table, sizeof(double));
@@ -744,11 +750,184 @@ static void mask_global(join_state *st, int len_of_table) {
     int adjust_factor = 0;
     double ths = avg + adjust_factor * sd;
     for (int64_t t = 0; t < len; t++) st->MASK[t] = (nd[t] <= ths) ? 1 : 0;
+    if (diff_out) memcpy(diff_out, nd, sizeof(double) * (size_t)len);
+    if (ths_out) *ths_out = ths;
     free(ct);
     free(nd);
 }
 
 static void *xcalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
+
+/* CPU_version1/new_dtw_motifGUI.cpp:151-297: the global occupancy mask (table length from the
+ * largest |z-value|) and the masked tables of diag_mask_global; needs the windows (subs) and
+ * the raw-series envelope (UTS, LTS).  Optional outputs: the occupancy statistic per position,
+ * the threshold and the table length. */
+static void global_tables(join_state *st, double *diff_out, double *ths_out, int *len_out) {
+    const double *a = st->a;
+    const int m = st->m, subseqLen = st->m;
+    const int64_t len = st->len, subcount = st->subcount;
+    double *tmpv = (double *)xcalloc(len, sizeof(double));
+    double *nL = (double *)xcalloc(subseqLen, sizeof(double)), *nU = (double *)xcalloc(subseqLen, sizeof(double));
+    double real_max = SUB(st, 0)[0], real_min = SUB(st, 0)[0];
+    for (int64_t r = 0; r < subcount; r++)
+        for (int q = 0; q < m; q++) {
+            double v2 = SUB(st, r)[q];
+            if (v2 > real_max) real_max = v2;
+            if (v2 < real_min) real_min = v2;
+        }
+    double max_real_value = floor(ORC_MAX(fabs(real_max), fabs(real_min))) * 2 * 100;
+    int len_of_table_global = (int)(ORC_MAX(600, max_real_value));
+    st->MASK = (double *)xcalloc(len, sizeof(double));
+    mask_global(st, len_of_table_global, diff_out, ths_out);
+    if (len_out) *len_out = len_of_table_global;
+    st->TS2 = (double *)xcalloc(len, sizeof(double));
+    for (int64_t x = 0; x < len; x++) st->TS2[x] = a[x] * a[x];
+    st->UTS_g = (double *)xcalloc(len, sizeof(double));
+    st->LTS_g = (double *)xcalloc(len, sizeof(double));
+    for (int64_t x = 0; x < len; x++) {
+        st->UTS_g[x] = st->UTS[x] * st->MASK[x];
+        st->LTS_g[x] = st->LTS[x] * st->MASK[x];
+    }
+    /* shifted tables (BIAS = 3): index x >= 1 reads position x + 2 (backward) and
+       x + m - 4 (forward) */
+    st->dr_bwd_g = (double *)xcalloc(len, sizeof(double));
+    st->dc_bwd_g = (double *)xcalloc(len, sizeof(double));
+    st->dr_fwd_g = (double *)xcalloc(len, sizeof(double));
+    st->dc_fwd_g = (double *)xcalloc(len, sizeof(double));
+    st->dr_bwdM = (double *)xcalloc(len, sizeof(double));
+    st->dr_fwdM = (double *)xcalloc(len, sizeof(double));
+    st->dc_bwdT2 = (double *)xcalloc(len, sizeof(double));
+    st->dc_fwdT2 = (double *)xcalloc(len, sizeof(double));
+    st->dr_bwd_g[0] = 0.0 + 0.0;
+    for (int64_t x = 1; x + 2 < len; x++) {
+        st->dr_bwd_g[x] = st->UTS_g[x + 2] + st->LTS_g[x + 2];
+        st->dc_bwd_g[x] = a[x + 2];
+        st->dr_bwdM[x] = st->MASK[x + 2];
+        st->dc_bwdT2[x] = st->TS2[x + 2];
+    }
+    for (int64_t x = 0; x + m - 4 < len; x++) {
+        st->dr_fwd_g[x] = st->UTS_g[x + m - 4] + st->LTS_g[x + m - 4];
+        st->dc_fwd_g[x] = a[x + m - 4];
+        st->dr_fwdM[x] = st->MASK[x + m - 4];
+        st->dc_fwdT2[x] = st->TS2[x + m - 4];
+    }
+    int b6 = subseqLen - 1 - 6;
+    double *sUg = (double *)xcalloc(subcount, sizeof(double)), *sLg = (double *)xcalloc(subcount, sizeof(double));
+    double *sU2g = (double *)xcalloc(subcount, sizeof(double)), *sL2g = (double *)xcalloc(subcount, sizeof(double));
+    st->sumMASK_g = (double *)xcalloc(subcount, sizeof(double));
+    movsum(st->UTS_g + 3, len - 6, b6, sUg);
+    movsum(st->LTS_g + 3, len - 6, b6, sLg);
+    for (int64_t x = 0; x < len - 6; x++) tmpv[x] = (st->UTS_g + 3)[x] * (st->UTS_g + 3)[x];
+    movsum(tmpv, len - 6, b6, sU2g);
+    for (int64_t x = 0; x < len - 6; x++) tmpv[x] = (st->LTS_g + 3)[x] * (st->LTS_g + 3)[x];
+    movsum(tmpv, len - 6, b6, sL2g);
+    movsum(st->MASK + 3, len - 6, b6, st->sumMASK_g);
+    st->sumU_sumL_g = (double *)xcalloc(subcount, sizeof(double));
+    double *sU2sL2g = (double *)xcalloc(subcount, sizeof(double));
+    for (int64_t x = 0; x < subcount; x++) {
+        sU2sL2g[x] = sU2g[x] + sL2g[x];
+        st->sumU_sumL_g[x] = sUg[x] + sLg[x];
+    }
+    st->normUL_g = (double *)xcalloc(subcount, sizeof(double));
+    for (int64_t row = 1; row <= subcount; row++) {
+        double mu_ = st->mu[row - 1];
+        st->normUL_g[row - 1] = (sU2sL2g[row - 1] - 2 * st->sumU_sumL_g[row - 1] * mu_ +
+                                 2 * st->sumMASK_g[row - 1] * mu_ * mu_) *
+                                st->invsig[row - 1] * st->invsig[row - 1];
+    }
+    st->DUL2_g = (double *)xcalloc(subcount, sizeof(double));
+    st->DUL_g = (double *)xcalloc(subcount, sizeof(double));
+    for (int64_t row = 0; row < subcount; row++) {
+        for (int64_t i = row; i <= row + subseqLen - 1; i++) {
+            nL[i - row] = (st->LTS_g[i] - st->mu[row]) * st->invsig[row];
+            nU[i - row] = (st->UTS_g[i] - st->mu[row]) * st->invsig[row];
+        }
+        for (int q = 0; q < subseqLen; q++) tmpv[q] = nL[q] - nU[q];
+        st->DUL2_g[row] = pow(norm2(tmpv, subseqLen), 2);
+        st->DUL_g[row] = sqrt(ORC_MAX(st->DUL2_g[row], 0));
+    }
+    free(tmpv); free(nL); free(nU);
+    free(sUg); free(sLg); free(sU2g); free(sL2g); free(sU2sL2g);
+}
+
+/* The state global_tables and diag_mask_global need, with the caller's window statistics (or
+ * the reference's movmean when mu is NULL): z-normalised windows and their clipped envelopes
+ * (CPU_version1/new_dtw_motifGUI.cpp:31-52) and the raw-series envelope (:96-98). */
+static int mask_state(join_state *st, const double *a, int64_t n, int32_t m, int32_t w, const double *mu,
+                      const double *sig) {
+    memset(st, 0, sizeof(*st));
+    if (m < 8 || w < 0 || n < 2 * (int64_t)m + 1) return 1;
+    st->len = n;
+    st->subcount = n - m + 1;
+    st->m = m;
+    st->w = w;
+    st->a = a;
+    st->mu = (double *)xcalloc(st->subcount, sizeof(double));
+    st->sig = (double *)xcalloc(st->subcount, sizeof(double));
+    st->invsig = (double *)xcalloc(st->subcount, sizeof(double));
+    if (mu && sig) {
+        memcpy(st->mu, mu, sizeof(double) * (size_t)st->subcount);
+        memcpy(st->sig, sig, sizeof(double) * (size_t)st->subcount);
+    } else {
+        orc_mvmean(a, n, m, st->mu, st->sig);
+    }
+    for (int64_t i = 0; i < st->subcount; i++) {
+        if (!(st->sig[i] > 0)) return 2;
+        st->invsig[i] = 1 / st->sig[i];
+    }
+    st->subs = (double *)xcalloc((size_t)st->subcount * m, sizeof(double));
+    st->subsU = (double *)xcalloc((size_t)st->subcount * m, sizeof(double));
+    st->subsL = (double *)xcalloc((size_t)st->subcount * m, sizeof(double));
+    for (int64_t i = 0; i < st->subcount; i++) {
+        for (int j = 0; j < m; j++) SUB(st, i)[j] = (a[i + j] - st->mu[i]) / st->sig[i];
+        orc_lemire(SUB(st, i), m, w, SUBL(st, i), SUBU(st, i));
+    }
+    st->UTS = (double *)xcalloc(n, sizeof(double));
+    st->LTS = (double *)xcalloc(n, sizeof(double));
+    orc_lemire(a, n, w, st->LTS, st->UTS);
+    return 0;
+}
+
+static void mask_state_free(join_state *st) {
+    free(st->mu); free(st->sig); free(st->invsig); free(st->subs); free(st->subsU); free(st->subsL);
+    free(st->UTS); free(st->LTS); free(st->MASK); free(st->TS2); free(st->UTS_g); free(st->LTS_g);
+    free(st->dr_bwd_g); free(st->dc_bwd_g); free(st->dr_fwd_g); free(st->dc_fwd_g); free(st->dr_bwdM);
+    free(st->dr_fwdM); free(st->dc_bwdT2); free(st->dc_fwdT2); free(st->sumMASK_g); free(st->sumU_sumL_g);
+    free(st->normUL_g); free(st->DUL2_g); free(st->DUL_g); free(st->special); free(st->lb_vector);
+    free(st->lb_vector_new);
+}
+
+int orc_global_mask(const double *a, int64_t n, int32_t m, int32_t w, const double *mu, const double *sig,
+                    unsigned char *mask, double *diff, double *ths, int32_t *len_of_table) {
+    join_state S;
+    int rc = mask_state(&S, a, n, m, w, mu, sig);
+    if (rc == 0) {
+        int len = 0;
+        global_tables(&S, diff, ths, &len);
+        for (int64_t t = 0; t < n; t++) mask[t] = S.MASK[t] != 0.0;
+        if (len_of_table) *len_of_table = len;
+    }
+    mask_state_free(&S);
+    return rc;
+}
+
+int orc_masked_ball_diag(const double *a, int64_t n, int32_t m, int32_t w, const double *mu, const double *sig,
+                         int64_t diag, double *lb_fir, double *lb_sec) {
+    join_state S;
+    int rc = mask_state(&S, a, n, m, w, mu, sig);
+    if (rc == 0 && (diag <= m || diag > S.subcount)) rc = 3;
+    if (rc == 0) {
+        global_tables(&S, NULL, NULL, NULL);
+        S.special = (double *)xcalloc((size_t)S.subcount * 2, sizeof(double));
+        S.lb_vector = (unsigned char *)xcalloc(S.subcount, 1);
+        S.lb_vector_new = (double *)xcalloc(S.subcount, sizeof(double));
+        double cnt = 0;
+        /* with an infinite best-so-far nothing is pruned, so both directions run on every pair */
+        diag_mask_global(&S, (int)diag, INFINITY, &cnt, lb_fir, lb_sec);
+    }
+    mask_state_free(&S);
+    return rc;
+}
 
 /* CPU_version1/new_dtw_motifGUI.cpp:21-586 */
 int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_ed, const int32_t *mp_ed_index,
@@ -860,83 +1039,7 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
     for (int64_t row = 0; row < subcount; row++)
         st->normUL_trans[row] = (sumU2_sumL2[row] - 2 * st->sumU_sumL[row] * st->mu[row]) * st->invsig[row];
     /* global occupancy mask and masked tables, :151-297 */
-    double real_max = SUB(st, 0)[0], real_min = SUB(st, 0)[0];
-    for (int64_t r = 0; r < subcount; r++)
-        for (int q = 0; q < m; q++) {
-            double v2 = SUB(st, r)[q];
-            if (v2 > real_max) real_max = v2;
-            if (v2 < real_min) real_min = v2;
-        }
-    double max_real_value = floor(ORC_MAX(fabs(real_max), fabs(real_min))) * 2 * 100;
-    int len_of_table_global = (int)(ORC_MAX(600, max_real_value));
-    st->MASK = (double *)xcalloc(len, sizeof(double));
-    mask_global(st, len_of_table_global);
-    st->TS2 = (double *)xcalloc(len, sizeof(double));
-    for (int64_t x = 0; x < len; x++) st->TS2[x] = a[x] * a[x];
-    st->UTS_g = (double *)xcalloc(len, sizeof(double));
-    st->LTS_g = (double *)xcalloc(len, sizeof(double));
-    for (int64_t x = 0; x < len; x++) {
-        st->UTS_g[x] = st->UTS[x] * st->MASK[x];
-        st->LTS_g[x] = st->LTS[x] * st->MASK[x];
-    }
-    /* shifted tables (BIAS = 3): index x >= 1 reads position x + 2 (backward) and
-       x + m - 4 (forward) */
-    st->dr_bwd_g = (double *)xcalloc(len, sizeof(double));
-    st->dc_bwd_g = (double *)xcalloc(len, sizeof(double));
-    st->dr_fwd_g = (double *)xcalloc(len, sizeof(double));
-    st->dc_fwd_g = (double *)xcalloc(len, sizeof(double));
-    st->dr_bwdM = (double *)xcalloc(len, sizeof(double));
-    st->dr_fwdM = (double *)xcalloc(len, sizeof(double));
-    st->dc_bwdT2 = (double *)xcalloc(len, sizeof(double));
-    st->dc_fwdT2 = (double *)xcalloc(len, sizeof(double));
-    st->dr_bwd_g[0] = 0.0 + 0.0;
-    for (int64_t x = 1; x + 2 < len; x++) {
-        st->dr_bwd_g[x] = st->UTS_g[x + 2] + st->LTS_g[x + 2];
-        st->dc_bwd_g[x] = a[x + 2];
-        st->dr_bwdM[x] = st->MASK[x + 2];
-        st->dc_bwdT2[x] = st->TS2[x + 2];
-    }
-    for (int64_t x = 0; x + m - 4 < len; x++) {
-        st->dr_fwd_g[x] = st->UTS_g[x + m - 4] + st->LTS_g[x + m - 4];
-        st->dc_fwd_g[x] = a[x + m - 4];
-        st->dr_fwdM[x] = st->MASK[x + m - 4];
-        st->dc_fwdT2[x] = st->TS2[x + m - 4];
-    }
-    int b6 = subseqLen - 1 - 6;
-    double *sUg = (double *)xcalloc(subcount, sizeof(double)), *sLg = (double *)xcalloc(subcount, sizeof(double));
-    double *sU2g = (double *)xcalloc(subcount, sizeof(double)), *sL2g = (double *)xcalloc(subcount, sizeof(double));
-    st->sumMASK_g = (double *)xcalloc(subcount, sizeof(double));
-    movsum(st->UTS_g + 3, len - 6, b6, sUg);
-    movsum(st->LTS_g + 3, len - 6, b6, sLg);
-    for (int64_t x = 0; x < len - 6; x++) tmpv[x] = (st->UTS_g + 3)[x] * (st->UTS_g + 3)[x];
-    movsum(tmpv, len - 6, b6, sU2g);
-    for (int64_t x = 0; x < len - 6; x++) tmpv[x] = (st->LTS_g + 3)[x] * (st->LTS_g + 3)[x];
-    movsum(tmpv, len - 6, b6, sL2g);
-    movsum(st->MASK + 3, len - 6, b6, st->sumMASK_g);
-    st->sumU_sumL_g = (double *)xcalloc(subcount, sizeof(double));
-    double *sU2sL2g = (double *)xcalloc(subcount, sizeof(double));
-    for (int64_t x = 0; x < subcount; x++) {
-        sU2sL2g[x] = sU2g[x] + sL2g[x];
-        st->sumU_sumL_g[x] = sUg[x] + sLg[x];
-    }
-    st->normUL_g = (double *)xcalloc(subcount, sizeof(double));
-    for (int64_t row = 1; row <= subcount; row++) {
-        double mu_ = st->mu[row - 1];
-        st->normUL_g[row - 1] = (sU2sL2g[row - 1] - 2 * st->sumU_sumL_g[row - 1] * mu_ +
-                                 2 * st->sumMASK_g[row - 1] * mu_ * mu_) *
-                                st->invsig[row - 1] * st->invsig[row - 1];
-    }
-    st->DUL2_g = (double *)xcalloc(subcount, sizeof(double));
-    st->DUL_g = (double *)xcalloc(subcount, sizeof(double));
-    for (int64_t row = 0; row < subcount; row++) {
-        for (int64_t i = row; i <= row + subseqLen - 1; i++) {
-            nL[i - row] = (st->LTS_g[i] - st->mu[row]) * st->invsig[row];
-            nU[i - row] = (st->UTS_g[i] - st->mu[row]) * st->invsig[row];
-        }
-        for (int q = 0; q < subseqLen; q++) tmpv[q] = nL[q] - nU[q];
-        st->DUL2_g[row] = pow(norm2(tmpv, subseqLen), 2);
-        st->DUL_g[row] = sqrt(ORC_MAX(st->DUL2_g[row], 0));
-    }
+    global_tables(st, NULL, NULL, NULL);
     st->special = (double *)xcalloc((size_t)subcount * 2, sizeof(double));
     st->lb_vector = (unsigned char *)xcalloc(subcount, 1);
     st->lb_vector_new = (double *)xcalloc(subcount, sizeof(double));
@@ -964,7 +1067,7 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
             double pruning_rate = cnt_of_purn / (double)(subcount - diag + 1);
             if (pruning_rate < 0.98) {
                 out->global_rounds++;
-                diag_mask_global(st, (int)diag, bsf, &cnt_of_purn);
+                diag_mask_global(st, (int)diag, bsf, &cnt_of_purn, NULL, NULL);
             } else {
                 lb_kim(st, bsf, subcount - diag + 1, (int)diag, &cnt_of_purn);
             }
@@ -1038,8 +1141,8 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
     free(st->sumU_sumL); free(sumU2_sumL2); free(st->DUL2_fast); free(st->DUL_fast); free(nL); free(nU);
     free(st->normUL_trans); free(st->MASK); free(st->TS2); free(st->UTS_g); free(st->LTS_g);
     free(st->dr_bwd_g); free(st->dc_bwd_g); free(st->dr_fwd_g); free(st->dc_fwd_g); free(st->dr_bwdM);
-    free(st->dr_fwdM); free(st->dc_bwdT2); free(st->dc_fwdT2); free(sUg); free(sLg); free(sU2g); free(sL2g);
-    free(st->sumMASK_g); free(st->sumU_sumL_g); free(sU2sL2g); free(st->normUL_g); free(st->DUL2_g);
+    free(st->dr_fwdM); free(st->dc_bwdT2); free(st->dc_fwdT2);
+    free(st->sumMASK_g); free(st->sumU_sumL_g); free(st->normUL_g); free(st->DUL2_g);
     free(st->DUL_g); free(st->special); free(st->lb_vector); free(st->lb_vector_new); free(cb); free(cb1);
     free(cb2); free(proj); free(proj_u); free(proj_l);
     return 0;

@@ -72,6 +72,21 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
 /* CPU_version1/DTWMotifDiscoveryGUI.cpp:12-27: orc_mpx then orc_join */
 int orc_motif_discovery(const double *a, int64_t n, int32_t m, int32_t w, orc_join_result *out);
 
+/* CPU_version1/compute_shared_data_global.cpp:12-46 + CPU_version1/compute_MASK_global.cpp:11-93
+   (table length as CPU_version1/new_dtw_motifGUI.cpp:151-176): the global occupancy mask of
+   every series position, from the caller's window statistics (mu, sig; NULL: the reference's
+   movmean).  Optional: diff (n: the occupancy statistic after the coverage ramp), ths (the
+   threshold, its sequential mean), len_of_table.  0 on success. */
+int orc_global_mask(const double *a, int64_t n, int32_t m, int32_t w, const double *mu, const double *sig,
+                    unsigned char *mask, double *diff, double *ths, int32_t *len_of_table);
+/* CPU_version1/diag_mask_global.cpp:11-226 with an infinite best-so-far: the masked ball bound
+   of both directions along the reference's diagonal diag (pair lo, lo + diag - 1 for lo = 0 ..
+   n - m + 1 - diag; m < diag), running sums and all.  lb_fir: envelope of window lo against
+   window lo + diag - 1; lb_sec: the reverse.  At lo = 0 the reference's second direction reads
+   the envelope norm of window 0 (CPU_version1/diag_mask_global.cpp:79), kept as is. */
+int orc_masked_ball_diag(const double *a, int64_t n, int32_t m, int32_t w, const double *mu, const double *sig,
+                         int64_t diag, double *lb_fir, double *lb_sec);
+
 /* ---- independent oracle ---- */
 /* textbook Sakoe-Chiba DTW: squared cost D[m-1][m-1] */
 double orc_dtw_sq(const double *x, const double *y, int32_t m, int32_t w);

@@ -56,6 +56,10 @@ def lib():
         L.orc_bruteforce_top1.argtypes = [dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, lp, lp, dp]
         L.orc_lb_keogh_sq.argtypes = [dp, dp, ctypes.c_int32, ctypes.c_int32]
         L.orc_lb_keogh_sq.restype = ctypes.c_double
+        u8p = ctypes.POINTER(ctypes.c_uint8)
+        L.orc_global_mask.argtypes = [dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, dp, dp, u8p, dp, dp, ip]
+        L.orc_masked_ball_diag.argtypes = [dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, dp, dp,
+                                           ctypes.c_int64, dp, dp]
         _L = L
     return _L
 
@@ -152,3 +156,37 @@ def mon_dtw(lines, cols, cb, w, bsf2):
     cols = _arr(cols)
     cb = _arr(cb)
     return lib().orc_mon_dtw(_d(lines), _d(cols), _d(cb), lines.size, w, bsf2)
+
+
+def global_mask(a, m, w, mu=None, sig=None):
+    """Global occupancy mask (uint8, n), occupancy statistic (n), threshold, table length.
+    mu, sig: window statistics to use (None: the reference's running sums)."""
+    a = _arr(a)
+    n = a.size
+    mask = np.zeros(n, dtype=np.uint8)
+    diff = np.zeros(n)
+    ths = ctypes.c_double(0)
+    ln = ctypes.c_int32(0)
+    mp = _d(_arr(mu)) if mu is not None else None
+    sp = _d(_arr(sig)) if sig is not None else None
+    rc = lib().orc_global_mask(_d(a), n, m, w, mp, sp, mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _d(diff),
+                               ctypes.byref(ths), ctypes.byref(ln))
+    if rc:
+        raise ValueError("orc_global_mask failed: %d" % rc)
+    return mask, diff, ths.value, ln.value
+
+
+def masked_ball_diag(a, m, w, diag, mu=None, sig=None):
+    """The reference's masked ball bound along diagonal diag (pairs (lo, lo + diag - 1)), both
+    directions, nothing pruned: (lb_fir, lb_sec), each of length n - m + 2 - diag."""
+    a = _arr(a)
+    n = a.size
+    cnt = n - m + 1 - diag + 1
+    fir = np.zeros(max(cnt, 1))
+    sec = np.zeros(max(cnt, 1))
+    mp = _d(_arr(mu)) if mu is not None else None
+    sp = _d(_arr(sig)) if sig is not None else None
+    rc = lib().orc_masked_ball_diag(_d(a), n, m, w, mp, sp, int(diag), _d(fir), _d(sec))
+    if rc:
+        raise ValueError("orc_masked_ball_diag failed: %d" % rc)
+    return fir[:cnt], sec[:cnt]

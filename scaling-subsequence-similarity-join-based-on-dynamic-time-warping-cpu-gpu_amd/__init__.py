@@ -112,7 +112,7 @@ EXPORTED = [
     "dtwj_gen_spec_default", "dtwj_generate", "dtwj_top1_batch",
     "dtwj_context_top1_batch", "dtwj_proposal_candidates", "dtwj_mlp_forward", "dtwj_mlp_gradient",
     "dtwj_mlp_train", "dtwj_top1_learned", "dtwj_plan", "dtwj_top1_device_noalloc", "dtwj_matrix_profile",
-    "dtwj_paa_pairs",
+    "dtwj_paa_pairs", "dtwj_global_mask", "dtwj_masked_ball_pairs",
 ]
 
 _lib = None
@@ -150,6 +150,10 @@ def lib():
     L.dtwj_dtw_pairs.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, i64_p, i64_p, ctypes.c_int64,
                                  ctypes.c_int32, d_p]
     L.dtwj_paa_pairs.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, i64_p, i64_p, ctypes.c_int64, d_p]
+    L.dtwj_global_mask.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, P(ctypes.c_uint8), d_p, d_p,
+                                   i32_p]
+    L.dtwj_masked_ball_pairs.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, i64_p, i64_p,
+                                         ctypes.c_int64, d_p]
     L.dtwj_top1_batch.argtypes = [P(Series), ctypes.c_int32, P(Options), P(Result), i32_p]
     L.dtwj_context_top1_batch.argtypes = [ctypes.c_void_p, P(Series), ctypes.c_int32, P(Options), P(Result), i32_p]
     f_p = P(ctypes.c_float)
@@ -516,6 +520,32 @@ def paa_pairs(series, m, w, pi, pj):
     i64 = ctypes.POINTER(ctypes.c_int64)
     _check(lib().dtwj_paa_pairs(_dptr(a), a.size, int(m), int(w), pi.ctypes.data_as(i64), pj.ctypes.data_as(i64),
                                 pi.size, _dptr(out)))
+    return out
+
+
+def global_mask(series, m, w):
+    """The reference's global occupancy mask on the GPU (compute_shared_data_global +
+    compute_MASK_global): (mask uint8[n], occupancy statistic float64[n], threshold, table length)."""
+    a = _as_series(series)
+    mask = np.zeros(a.size, dtype=np.uint8)
+    occ = np.zeros(a.size)
+    ths = ctypes.c_double(0)
+    ln = ctypes.c_int32(0)
+    _check(lib().dtwj_global_mask(_dptr(a), a.size, int(m), int(w), mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                  _dptr(occ), ctypes.byref(ths), ctypes.byref(ln)))
+    return mask, occ, ths.value, ln.value
+
+
+def masked_ball_pairs(series, m, w, pi, pj):
+    """The reference's masked ball bound (diag_mask_global) of explicit pairs on the GPU, FP64:
+    array (npairs, 2): envelope of window pi against window pj, then the reverse."""
+    a = _as_series(series)
+    pi = np.ascontiguousarray(pi, dtype=np.int64)
+    pj = np.ascontiguousarray(pj, dtype=np.int64)
+    out = np.zeros((pi.size, 2))
+    P = ctypes.POINTER
+    _check(lib().dtwj_masked_ball_pairs(_dptr(a), a.size, int(m), int(w), pi.ctypes.data_as(P(ctypes.c_int64)),
+                                        pj.ctypes.data_as(P(ctypes.c_int64)), pi.size, _dptr(out)))
     return out
 
 

@@ -28,7 +28,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", o
           "-I", CSRC]
 
 LIB_SOURCES = ["dtwj_api.cpp", "series_gen.cpp", "loader.cpp", "prep.hip", "join.hip", "dtw.hip", "proposal.hip",
-               "noalloc.hip", "mpx.hip"]
+               "noalloc.hip", "mpx.hip", "mask.hip"]
 CLI_SOURCES = ["gpu_dtw_cli.cpp"]
 TOOL_SOURCES = ["copy_calib.hip"]   # standalone executables (traffic-counter calibration)
 

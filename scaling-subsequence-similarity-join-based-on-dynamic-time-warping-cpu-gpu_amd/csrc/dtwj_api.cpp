@@ -1703,4 +1703,120 @@ int dtwj_paa_pairs(const double *series, int64_t n, int32_t m, int32_t w, const 
     return rc;
 }
 
+// The global occupancy mask of a preprocessed series (P: statistics, envelope of radius w) into
+// buffers of one device allocation (*mem, freed by the caller).
+static int run_global_mask(dtwj_context *c, const Prep &P, int64_t n, int32_t m, int32_t w, MaskBuffers &B,
+                           void **mem) {
+    B.a = P.a;
+    B.U = P.U;
+    B.L = P.L;
+    B.mu = P.mu;
+    B.sig = P.sig;
+    B.n = n;
+    B.m = m;
+    B.w = w;
+    B.lmax = mask_table_bins(m);
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t b_table = up(sizeof(uint32_t) * (size_t)B.lmax * m), b_diff = up(sizeof(double) * (size_t)n),
+                 b_mask = up((size_t)n), b_small = 256;
+    char *base = nullptr;
+    if (hipMalloc(&base, b_table + b_diff + b_mask + b_small) != hipSuccess)
+        return fail(DTWJ_ENOMEM, "device allocation failed (global mask)");
+    *mem = base;
+    B.table = reinterpret_cast<uint32_t *>(base);
+    B.diff = reinterpret_cast<double *>(base + b_table);
+    B.mask = reinterpret_cast<uint8_t *>(base + b_table + b_diff);
+    char *small = base + b_table + b_diff + b_mask;
+    B.ths = reinterpret_cast<double *>(small);
+    B.zmax_bits = reinterpret_cast<unsigned long long *>(small + 8);
+    B.status = reinterpret_cast<int32_t *>(small + 16);
+    if (int rc = launch_global_mask(B, c->stream)) return rc;
+    DTWJ_HIP(hipStreamSynchronize(c->stream));
+    int32_t status[2];
+    DTWJ_HIP(hipMemcpy(status, B.status, sizeof(status), hipMemcpyDeviceToHost));
+    if (status[1]) return fail(DTWJ_EINVAL, "a z-value exceeds the histogram range (window statistics out of range)");
+    return DTWJ_OK;
+}
+
+static int mask_checks(const double *series, int64_t n, int32_t m, int32_t w) {
+    if (int rc = validate(series, n, m, w)) return rc;
+    if (w > kMaskMaxRadius) return fail(DTWJ_EINVAL, "the global mask needs w <= 256");
+    if (m > kMaskMaxWindow) return fail(DTWJ_EINVAL, "the global mask needs m <= 16384");
+    return DTWJ_OK;
+}
+
+// Preprocessing of a building block that needs non-constant windows.
+static int mask_prep(const double *series, int64_t n, int32_t m, int32_t w, dtwj_context **c, Prep &P) {
+    const double *d_a = nullptr;
+    int rc = building_block_prep(series, n, m, w, c, P, &d_a);
+    if (rc == DTWJ_OK) {
+        int32_t flags[4];
+        DTWJ_HIP(hipMemcpy(flags, P.flags, sizeof(flags), hipMemcpyDeviceToHost));
+        if (flags[1]) rc = fail(DTWJ_EINVAL, "a window of length m is constant (standard deviation 0)");
+    }
+    return rc;
+}
+
+int dtwj_global_mask(const double *series, int64_t n, int32_t m, int32_t w, uint8_t *mask, double *occupancy,
+                     double *threshold, int32_t *len_of_table) {
+    if (int rc = mask_checks(series, n, m, w)) return rc;
+    if (!mask) return fail(DTWJ_EINVAL, "mask is NULL");
+    dtwj_context *c = nullptr;
+    Prep P;
+    void *mem = nullptr;
+    MaskBuffers B;
+    int rc = mask_prep(series, n, m, w, &c, P);
+    if (rc == DTWJ_OK) rc = run_global_mask(c, P, n, m, w, B, &mem);
+    if (rc == DTWJ_OK) {
+        hipError_t e = hipMemcpy(mask, B.mask, (size_t)n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && occupancy) e = hipMemcpy(occupancy, B.diff, sizeof(double) * n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && threshold) e = hipMemcpy(threshold, B.ths, sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && len_of_table) e = hipMemcpy(len_of_table, B.status, sizeof(int32_t), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "global mask", __FILE__, __LINE__);
+    }
+    if (mem) (void)hipFree(mem);
+    if (c) dtwj_context_destroy(c);
+    return rc;
+}
+
+int dtwj_masked_ball_pairs(const double *series, int64_t n, int32_t m, int32_t w, const int64_t *pi,
+                           const int64_t *pj, int64_t npairs, double *lb) {
+    if (int rc = mask_checks(series, n, m, w)) return rc;
+    if (!pi || !pj || !lb || npairs < 0) return fail(DTWJ_EINVAL, "bad pair arguments");
+    const int64_t nw = n - m + 1;
+    for (int64_t p = 0; p < npairs; ++p)
+        if (pi[p] < 0 || pj[p] < 0 || pi[p] >= nw || pj[p] >= nw) return fail(DTWJ_EINVAL, "pair index out of range");
+    if (npairs == 0) return DTWJ_OK;
+    dtwj_context *c = nullptr;
+    Prep P;
+    void *mem = nullptr;
+    MaskBuffers B;
+    int64_t *d_pi = nullptr, *d_pj = nullptr;
+    double *d_lb = nullptr;
+    int rc = mask_prep(series, n, m, w, &c, P);
+    if (rc == DTWJ_OK) rc = run_global_mask(c, P, n, m, w, B, &mem);
+    if (rc == DTWJ_OK) {
+        if (hipMalloc(&d_pi, sizeof(int64_t) * npairs) != hipSuccess || hipMalloc(&d_pj, sizeof(int64_t) * npairs) != hipSuccess ||
+            hipMalloc(&d_lb, sizeof(double) * 2 * npairs) != hipSuccess)
+            rc = fail(DTWJ_ENOMEM, "device allocation failed");
+    }
+    if (rc == DTWJ_OK) {
+        hipError_t e = hipMemcpy(d_pi, pi, sizeof(int64_t) * npairs, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_pj, pj, sizeof(int64_t) * npairs, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy", __FILE__, __LINE__);
+    }
+    if (rc == DTWJ_OK) rc = launch_masked_ball(B, d_pi, d_pj, npairs, d_lb, c->stream);
+    if (rc == DTWJ_OK) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(lb, d_lb, sizeof(double) * 2 * npairs, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "masked ball pairs", __FILE__, __LINE__);
+    }
+    if (d_pi) (void)hipFree(d_pi);
+    if (d_pj) (void)hipFree(d_pj);
+    if (d_lb) (void)hipFree(d_lb);
+    if (mem) (void)hipFree(mem);
+    if (c) dtwj_context_destroy(c);
+    return rc;
+}
+
 }  // extern "C"

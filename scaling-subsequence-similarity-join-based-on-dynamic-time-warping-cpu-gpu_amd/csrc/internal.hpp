@@ -98,6 +98,32 @@ size_t mp_lds_bytes(int32_t m);
 int launch_matrix_profile(const double *d_a, int64_t n, int32_t m, double *scratch, int32_t *flags, double *d_mp,
                           int32_t *d_mpi, hipStream_t s);
 
+// The reference's global occupancy mask and masked ball bound (mask.hip).  Inputs from the
+// preprocessing (series, envelope, FP64 window statistics); buffers: table m x lmax counts,
+// diff (n), mask (n), ths, zmax_bits (1 each), status[2] (table length, error flag: the table
+// would need more than lmax bins).
+struct MaskBuffers {
+    const double *a = nullptr, *U = nullptr, *L = nullptr, *mu = nullptr, *sig = nullptr;
+    int64_t n = 0;
+    int32_t m = 0, w = 0;
+    int64_t lmax = 0;
+    uint32_t *table = nullptr;
+    double *diff = nullptr;
+    uint8_t *mask = nullptr;
+    double *ths = nullptr;
+    unsigned long long *zmax_bits = nullptr;
+    int32_t *status = nullptr;
+};
+// limits of the mask kernels: the envelope radius (sparse tables of a block's span in LDS) and
+// the window length (a histogram row of max(600, 200 (floor(sqrt(m - 1)) + 1)) bins in LDS)
+constexpr int32_t kMaskMaxRadius = 256;
+constexpr int32_t kMaskMaxWindow = 16384;
+int64_t mask_table_bins(int32_t m);
+size_t mask_sum_lds_bytes(int32_t m, int32_t w);
+int launch_global_mask(const MaskBuffers &B, hipStream_t s);
+int launch_masked_ball(const MaskBuffers &B, const int64_t *pi, const int64_t *pj, int64_t npairs, double *lb,
+                       hipStream_t s);
+
 // Fixed per-device workspace of the no-allocation entry point (noalloc.hip): its size, and the
 // device address of `device`'s copy (looked up on first use, then cached).
 size_t noalloc_scratch_bytes();
