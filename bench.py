@@ -242,8 +242,7 @@ def reference_cpu_measured(cfg):
 
 def cpu_baseline(series, m, w, budget_s, cfg=None):
     """Times the oracle (the repository's FP64 restatement of the reference CPU join with its ED
-    matrix-profile seed; it leaves out the reference's alpha-tuned local masks, which only prune)
-    on the largest prefix that fits the budget, single-threaded on this host, and extrapolates to
+    matrix-profile seed, every pruning stage included) on the largest prefix that fits the budget, single-threaded on this host, and extrapolates to
     the full series by pair count.  Where the fixtures hold full-size runs of the same workload
     (reference and oracle, build container), they are reported beside it: at n = 1M the
     extrapolation is about 4 times below the oracle's measured full run, whose tables no longer
@@ -263,9 +262,9 @@ def cpu_baseline(series, m, w, budget_s, cfg=None):
     model, ncpu = host_cpu()
     out = {"value": full, "unit": "s (extrapolated to the full workload by pair count)", "cores": 1,
            "kind": "port", "sample": "first %d samples of the workload series, m=%d, w=%d: %.2f s measured "
-           "(ED matrix-profile seed + join, FP64, single thread; the restatement omits the reference's "
-           "alpha-tuned local masks; extrapolated by pair count, which leaves out the cache misses of the "
-           "full tables: see reference_measured)" % (n_s, m, w, dt),
+           "(ED matrix-profile seed + join with every reference stage, FP64, single thread; extrapolated by "
+           "pair count, which leaves out the cache misses of the full tables: see reference_measured)"
+           % (n_s, m, w, dt),
            "sample_seconds": dt, "sample_pairs_per_s": pairs_of(n_s, m) / dt,
            "sample_ns_per_pair": 1e9 * dt / pairs_of(n_s, m), "host_cpu": model, "host_logical_cpus": ncpu,
            "reference_ns_per_pair_container": 16.0}

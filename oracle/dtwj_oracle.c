@@ -850,6 +850,391 @@ static void global_tables(join_state *st, double *diff_out, double *ths_out, int
     free(sUg); free(sLg); free(sU2g); free(sL2g); free(sU2sL2g);
 }
 
+/* ---------------- the alpha-tuned local masks (CPU_version1/new_dtw_motifGUI.cpp:150-176,
+ * 311-498, CPU_version1/compute_shared_data.cpp:10-135, CPU_version1/diag_mask_local_up.cpp:12-239,
+ * CPU_version1/diag_mask_local_down.cpp:12-227) ---------------- */
+
+/* CPU_version1/compute_shared_data.cpp:10-28: bins of 0.01 within [-3, 3], of 0.1 outside; the
+ * double is truncated to int on assignment */
+static int find_pos_local(double input, int length) {
+    int pos;
+    if (input > 3)
+        pos = (int)(300 + (input - 3) * 10);
+    else if (input < -3)
+        pos = (int)(-300 + (input + 3) * 10);
+    else
+        pos = (int)(input * 100);
+    pos += length / 2;
+    return pos;
+}
+
+typedef struct {
+    int len_of_cdf;
+    short *ct;               /* len x len_of_cdf: cumulative counts of the z-values at each position */
+    double *pos_UU, *pos_LL; /* len: bins of the mean envelope at each position */
+    double *invsig_2;        /* subcount */
+    /* vectors the reference allocates once per join (CPU_version1/new_dtw_motifGUI.cpp:311-326) and
+       every local-mask call overwrites in part: rows a call does not write keep what an earlier call
+       left there (zeros if none did) */
+    double *sumU_sumL, *sumU2_sumL2, *sumMASK, *normUL;     /* subcount */
+    double *raw_DIFF_UL, *raw_DIFF_UL2, *raw_DIFF_UL2_temp; /* len */
+    double *DUL2_raw, *DUL2;                                /* subcount */
+    double bad_bin;          /* reads of a cumulative count at bin 0 (index -1 in the reference) */
+} local_state;
+
+/* CPU_version1/new_dtw_motifGUI.cpp:150-176 with CPU_version1/compute_shared_data.cpp:30-135 */
+static void local_tables(join_state *st, local_state *ls) {
+    const int m = st->m;
+    const int64_t len = st->len, subcount = st->subcount;
+    memset(ls, 0, sizeof(*ls));
+    double real_max = SUB(st, 0)[0], real_min = SUB(st, 0)[0];
+    for (int64_t r = 0; r < subcount; r++)
+        for (int q = 0; q < m; q++) {
+            double v2 = SUB(st, r)[q];
+            if (v2 > real_max) real_max = v2;
+            if (v2 < real_min) real_min = v2;
+        }
+    double MAX_REAL_VALUE = ORC_MAX(fabs(real_max), fabs(real_min));
+    int L;
+    if (MAX_REAL_VALUE > 3)
+        L = (int)((MAX_REAL_VALUE - 3) * 20 + 600);
+    else
+        L = 600;
+    ls->len_of_cdf = L;
+    short *ct = (short *)calloc((size_t)len * L, sizeof(short));
+    ls->ct = ct;
+    for (int64_t j = 0; j < subcount; j++)
+        for (int i = 0; i < m; i++) {
+            double value = SUB(st, j)[i];
+            int pp = find_pos_local(value, L);
+            if (pp < 0) pp = 0;
+            if (pp >= L) pp = L - 1;
+            ct[(size_t)(i + j) * L + pp] = (short)(ct[(size_t)(i + j) * L + pp] + 1);
+        }
+    for (int64_t i = 0; i < len; i++) {
+        int sum1 = 0;
+        for (int j = 0; j < L; j++) {
+            sum1 = sum1 + ct[(size_t)i * L + j];
+            ct[(size_t)i * L + j] = (short)sum1;
+        }
+    }
+    double *sum_UU = (double *)calloc(len, sizeof(double)), *sum_LL = (double *)calloc(len, sizeof(double));
+    double *cnt = (double *)calloc(len, sizeof(double));
+    for (int64_t i = 0; i < subcount; i++)
+        for (int j = 0; j < m; j++) {
+            sum_UU[i + j] = sum_UU[i + j] + SUBU(st, i)[j];
+            sum_LL[i + j] = sum_LL[i + j] + SUBL(st, i)[j];
+            cnt[i + j] = cnt[i + j] + 1;
+        }
+    for (int64_t i = 0; i < len; i++) {
+        sum_UU[i] = sum_UU[i] / cnt[i];
+        sum_LL[i] = sum_LL[i] / cnt[i];
+    }
+    ls->pos_UU = (double *)calloc(len, sizeof(double));
+    ls->pos_LL = (double *)calloc(len, sizeof(double));
+    for (int64_t i = 0; i < len; i++) {
+        double posLL = find_pos_local(sum_LL[i], L), posUU = find_pos_local(sum_UU[i], L);
+        /* :113-135, including the check of posUU that sets posLL */
+        if (posLL < 0) posLL = 1;
+        if (posLL > L) posLL = L;
+        if (posUU < 0) posLL = 1;
+        if (posLL > L) posLL = L;
+        if (posUU < 0) posUU = 1;
+        if (posUU > L) posUU = L;
+        ls->pos_UU[i] = posUU;
+        ls->pos_LL[i] = posLL;
+    }
+    free(sum_UU);
+    free(sum_LL);
+    free(cnt);
+    ls->invsig_2 = (double *)xcalloc(subcount, sizeof(double));
+    for (int64_t i = 0; i < subcount; i++) ls->invsig_2[i] = st->invsig[i] * st->invsig[i];
+    ls->sumU_sumL = (double *)xcalloc(subcount, sizeof(double));
+    ls->sumU2_sumL2 = (double *)xcalloc(subcount, sizeof(double));
+    ls->sumMASK = (double *)xcalloc(subcount, sizeof(double));
+    ls->normUL = (double *)xcalloc(subcount, sizeof(double));
+    ls->raw_DIFF_UL = (double *)xcalloc(len, sizeof(double));
+    ls->raw_DIFF_UL2 = (double *)xcalloc(len, sizeof(double));
+    ls->raw_DIFF_UL2_temp = (double *)xcalloc(len, sizeof(double));
+    ls->DUL2_raw = (double *)xcalloc(subcount, sizeof(double));
+    ls->DUL2 = (double *)xcalloc(subcount, sizeof(double));
+}
+
+static void local_free(local_state *ls) {
+    free(ls->ct); free(ls->pos_UU); free(ls->pos_LL); free(ls->invsig_2); free(ls->sumU_sumL);
+    free(ls->sumU2_sumL2); free(ls->sumMASK); free(ls->normUL); free(ls->raw_DIFF_UL); free(ls->raw_DIFF_UL2);
+    free(ls->raw_DIFF_UL2_temp); free(ls->DUL2_raw); free(ls->DUL2);
+}
+
+/* count_table[pos][p - 1] - count_table[pos][q - 1]; a bin of 0 (the reference then reads index
+ * -1, outside the row) is counted in bad_bin and read as a count of 0 */
+static double local_count_diff(local_state *ls, int64_t pos, double p, double q) {
+    const int L = ls->len_of_cdf;
+    size_t pu = (size_t)p, pl = (size_t)q;
+    int cu = 0, cl = 0;
+    if (pu == 0) ls->bad_bin++; else cu = ls->ct[(size_t)pos * L + pu - 1];
+    if (pl == 0) ls->bad_bin++; else cl = ls->ct[(size_t)pos * L + pl - 1];
+    return (double)(cu - cl);
+}
+
+/* CPU_version1/diag_mask_local_up.cpp:12-239: the masked ball bound of the envelope of window
+ * lo against window lo + diag - 1, the mask from the occupancy of each position's mean envelope
+ * at the partner position, threshold (mean + 0.5) / 2 + alpha */
+static void local_up(join_state *st, local_state *ls, double bsf, int diag, unsigned char *lb_vector,
+                     double *cnt, double alpha, int64_t diag_bias) {
+    const int subseqlen = st->m;
+    const int64_t len = st->len, subcount = st->subcount;
+    const double *ts = st->a, *mu = st->mu, *invsig = st->invsig, *TS2 = st->TS2;
+    const double *UTS_ = st->UTS, *LTS_ = st->LTS;
+    const int64_t nlen = len - diag_bias;
+    double *normal_DIFF = (double *)xcalloc(nlen, sizeof(double));
+    for (int64_t i = 0; i < nlen; i++) {
+        int64_t pos = i + diag - 1;
+        if (pos >= len) continue;
+        double diff2 = local_count_diff(ls, pos, ls->pos_UU[i], ls->pos_LL[i]);
+        double diff_ratio = diff2 / subseqlen;
+        normal_DIFF[i] = diff_ratio;
+    }
+    double sum = 0.0;
+    int count = 0;
+    for (int64_t i = 0; i < nlen; i++) {
+        if (!isnan(normal_DIFF[i])) {
+            sum += normal_DIFF[i];
+            count++;
+        }
+    }
+    double avg_diff = (count == 0) ? NAN : sum / count;
+    double ths_diff = (avg_diff + 0.5) / 2 + alpha;
+    for (int64_t i = 0; i < nlen; i++) ls->raw_DIFF_UL[i] = UTS_[i] - LTS_[i];
+    for (int64_t i = 0; i < nlen; i++) ls->raw_DIFF_UL2_temp[i] = ls->raw_DIFF_UL[i] * ls->raw_DIFF_UL[i];
+    double *MASK = (double *)xcalloc(nlen, sizeof(double));
+    for (int64_t i = 0; i < nlen; i++) MASK[i] = (normal_DIFF[i] <= ths_diff) ? 1 : 0;
+    double *UTS = (double *)xcalloc(nlen, sizeof(double)), *LTS = (double *)xcalloc(nlen, sizeof(double));
+    for (int64_t i = 0; i < nlen; i++) {
+        UTS[i] = UTS_[i] * MASK[i];
+        LTS[i] = LTS_[i] * MASK[i];
+    }
+    /* dr_bwd: (U + L)[3 + i]; dr_fwd: (U + L)[m - 4 + i] (the reference's arrays reach two entries
+       past UTS at their ends, entries no pair reads) */
+    const int64_t nbwd = len - 1 - diag_bias, nfwd = len - subseqlen + 1 - diag_bias;
+    double *dr_bwd_less = (double *)xcalloc(nbwd, sizeof(double));
+    double *dr_fwd = (double *)xcalloc(nfwd, sizeof(double));
+    for (int64_t i = 0; i < nbwd && i + 3 < nlen; i++) dr_bwd_less[i] = UTS[3 + i] + LTS[3 + i];
+    for (int64_t i = 0; i < nfwd; i++) dr_fwd[i] = UTS[subseqlen - 4 + i] + LTS[subseqlen - 4 + i];
+    double sum_sumU = 0, sum_sumL = 0, sum_sumU2 = 0, sum_sumL2 = 0, sum_sumMASK = 0;
+    const double *U_pos = UTS + 3, *L_pos = LTS + 3, *MASK_pos = MASK + 3;
+    double fU, fL, sU, sL;
+    for (int i = 0; i < subseqlen - 6; i++) {
+        fU = U_pos[i];
+        fL = L_pos[i];
+        sum_sumU += fU;
+        sum_sumL += fL;
+        sum_sumU2 += fU * fU;
+        sum_sumL2 += fL * fL;
+        sum_sumMASK += MASK_pos[i];
+    }
+    for (int64_t i = 0; i < subcount - 6 - diag_bias; i++) {
+        ls->sumU_sumL[i] = sum_sumU + sum_sumL;
+        ls->sumU2_sumL2[i] = sum_sumU2 + sum_sumL2;
+        ls->sumMASK[i] = sum_sumMASK;
+        fU = U_pos[i];
+        fL = L_pos[i];
+        sU = U_pos[i + subseqlen - 6];
+        sL = L_pos[i + subseqlen - 6];
+        sum_sumU = sum_sumU - fU + sU;
+        sum_sumL = sum_sumL - fL + sL;
+        sum_sumU2 = sum_sumU2 - fU * fU + sU * sU;
+        sum_sumL2 = sum_sumL2 - fL * fL + sL * sL;
+        sum_sumMASK = sum_sumMASK - MASK_pos[i] + MASK_pos[i + subseqlen - 6];
+    }
+    for (int64_t row = 0; row < subcount - diag_bias; row++)
+        ls->normUL[row] = (ls->sumU2_sumL2[row] - 2 * ls->sumU_sumL[row] * mu[row] +
+                           2 * ls->sumMASK[row] * mu[row] * mu[row]) * invsig[row] * invsig[row];
+    for (int64_t i = 0; i < nlen; i++) ls->raw_DIFF_UL2[i] = ls->raw_DIFF_UL2_temp[i] * MASK[i];
+    movsum(ls->raw_DIFF_UL2 + 3, nlen - 6, subseqlen - 7, ls->DUL2_raw);
+    for (int64_t i = 0; i < ORC_MIN(subcount, nlen); i++) ls->DUL2[i] = ls->DUL2_raw[i] * ls->invsig_2[i];
+    const double *dc_bwd_less = ts + 3, *dc_fwd = ts + subseqlen - 4;
+    const double *dr_bwdMASK_less = MASK + 3, *dr_fwdMASK = MASK + subseqlen - 4;
+    const double *dc_bwdTS2_less = TS2 + 3, *dc_fwdTS2 = TS2 + subseqlen - 4;
+    double cov = 0, dTS = 0, dTS2 = 0;
+    for (int k = 0; k < subseqlen - 6; k++) cov += (ts + diag - 1 + 3)[k] * ((UTS + 3)[k] + (LTS + 3)[k]);
+    for (int k = 0; k < subseqlen - 6; k++) dTS += (ts + diag - 1 + 3)[k] * (MASK + 3)[k];
+    for (int k = 0; k < subseqlen - 6; k++) dTS2 += (TS2 + diag - 1 + 3)[k] * (MASK + 3)[k];
+    const double *sumMASK = ls->sumMASK, *sumU_sumL = ls->sumU_sumL, *nUL = ls->normUL, *DUL2 = ls->DUL2;
+    double M_NORM, t, dist, lb;
+    if (!lb_vector[0]) {
+        int64_t row = 0, col = diag - 1;
+        M_NORM = dTS2 - 2 * mu[col] * dTS + sumMASK[row] * mu[col] * mu[col];
+        M_NORM = M_NORM * invsig[col] * invsig[col];
+        t = (cov - mu[col] * sumU_sumL[row] + 2 * mu[row] * (sumMASK[row] * mu[col] - dTS)) * (invsig[col] * invsig[row]);
+        dist = 2 * M_NORM - 2 * t + nUL[row];
+        if (dist < DUL2[row])
+            lb = 0;
+        else
+            lb = 0.5 * (sqrt(2 * dist - DUL2[row]) - sqrt(ORC_MAX(DUL2[row], 0)));
+        if (lb > bsf || (lb * lb + st->special[2 * row] + st->special[2 * row + 1] > bsf * bsf)) {
+            lb_vector[row] = 1;
+            (*cnt)++;
+        }
+    }
+    double bsf_2 = bsf * bsf;
+    int64_t temp_1 = subcount - diag + 1;
+    for (int64_t lo = 1; lo < temp_1; lo++) {
+        int64_t hi = diag + lo - 1, lo_l = lo - 1, hi_l = hi - 1;
+        cov = cov - dr_bwd_less[lo_l] * dc_bwd_less[hi_l] + dr_fwd[lo] * dc_fwd[hi];
+        dTS = dTS - dr_bwdMASK_less[lo_l] * dc_bwd_less[hi_l] + dr_fwdMASK[lo] * dc_fwd[hi];
+        dTS2 = dTS2 - dr_bwdMASK_less[lo_l] * dc_bwdTS2_less[hi_l] + dr_fwdMASK[lo] * dc_fwdTS2[hi];
+        if (lb_vector[lo]) continue;
+        M_NORM = dTS2 - 2 * mu[hi] * dTS + sumMASK[lo] * mu[hi] * mu[hi];
+        M_NORM = M_NORM * invsig[hi] * invsig[hi];
+        t = (cov - mu[hi] * sumU_sumL[lo] + 2 * mu[lo] * (sumMASK[lo] * mu[hi] - dTS)) * (invsig[hi] * invsig[lo]);
+        dist = 2 * M_NORM - 2 * t + nUL[lo];
+        if (dist < DUL2[lo]) {
+            lb = 0;
+        } else {
+            lb = 0.5 * (sqrt(2 * dist - DUL2[lo]) - sqrt(ORC_MAX(DUL2[lo], 0)));
+            if (lb > bsf || (lb * lb + st->special[2 * lo] + st->special[2 * lo + 1] > bsf_2)) {
+                (*cnt)++;
+                lb_vector[lo] = 1;
+            }
+        }
+    }
+    free(normal_DIFF); free(MASK); free(UTS); free(LTS); free(dr_bwd_less); free(dr_fwd);
+}
+
+/* CPU_version1/diag_mask_local_down.cpp:12-227: the same with the envelope of window lo + diag - 1
+ * against window lo, the mask from positions from diag_bias on (positions below count as 0 in
+ * its mean) */
+static void local_down(join_state *st, local_state *ls, double bsf, int diag, unsigned char *lb_vector,
+                       double *cnt, double alpha, int64_t diag_bias) {
+    const int subseqlen = st->m;
+    const int64_t len = st->len, subcount = st->subcount;
+    const double *ts = st->a, *mu = st->mu, *invsig = st->invsig, *TS2 = st->TS2;
+    const double *UTS_ = st->UTS, *LTS_ = st->LTS;
+    double *normal_DIFF = (double *)xcalloc(len, sizeof(double));
+    for (int64_t i = diag_bias; i < len; i++) {
+        int64_t pos = i - diag + 1;
+        if (pos < 0) continue;
+        double diff2 = local_count_diff(ls, pos, ls->pos_UU[i], ls->pos_LL[i]);
+        double diff_ratio = diff2 / subseqlen;
+        normal_DIFF[i] = diff_ratio;
+    }
+    double sum = 0.0;
+    int count = 0;
+    for (int64_t i = 0; i < len; i++) {
+        if (!isnan(normal_DIFF[i])) {
+            sum += normal_DIFF[i];
+            count++;
+        }
+    }
+    double avg_diff = (count == 0) ? NAN : sum / count;
+    double ths_diff = (avg_diff + 0.5) / 2 + alpha;
+    for (int64_t i = diag_bias; i < len; i++) ls->raw_DIFF_UL[i] = UTS_[i] - LTS_[i];
+    for (int64_t i = diag_bias; i < len; i++) ls->raw_DIFF_UL2_temp[i] = ls->raw_DIFF_UL[i] * ls->raw_DIFF_UL[i];
+    double *MASK = (double *)xcalloc(len, sizeof(double));
+    for (int64_t i = diag_bias; i < len; i++) MASK[i] = (normal_DIFF[i] <= ths_diff) ? 1 : 0;
+    double *UTS = (double *)xcalloc(len, sizeof(double)), *LTS = (double *)xcalloc(len, sizeof(double));
+    for (int64_t i = diag_bias; i < len; i++) {
+        UTS[i] = UTS_[i] * MASK[i];
+        LTS[i] = LTS_[i] * MASK[i];
+    }
+    const int64_t nbwd = len - 1, nfwd = len - subseqlen + 1;
+    double *dr_bwd_less = (double *)xcalloc(nbwd, sizeof(double));
+    double *dr_fwd = (double *)xcalloc(nfwd, sizeof(double));
+    for (int64_t i = diag_bias; i < nbwd && i + 3 < len; i++) dr_bwd_less[i] = UTS[3 + i] + LTS[3 + i];
+    for (int64_t i = diag_bias; i < nfwd; i++) dr_fwd[i] = UTS[subseqlen - 4 + i] + LTS[subseqlen - 4 + i];
+    double sum_sumU = 0, sum_sumL = 0, sum_sumU2 = 0, sum_sumL2 = 0, sum_sumMASK = 0;
+    const double *U_pos = UTS + 3, *L_pos = LTS + 3, *MASK_pos = MASK + 3;
+    double fU, fL, sU, sL;
+    for (int64_t i = diag_bias; i < diag_bias + subseqlen - 6; i++) {
+        fU = U_pos[i];
+        fL = L_pos[i];
+        sum_sumU += fU;
+        sum_sumL += fL;
+        sum_sumU2 += fU * fU;
+        sum_sumL2 += fL * fL;
+        sum_sumMASK += MASK_pos[i];
+    }
+    for (int64_t i = diag_bias; i < subcount - 6; i++) {
+        ls->sumU_sumL[i] = sum_sumU + sum_sumL;
+        ls->sumU2_sumL2[i] = sum_sumU2 + sum_sumL2;
+        ls->sumMASK[i] = sum_sumMASK;
+        fU = U_pos[i];
+        fL = L_pos[i];
+        sU = U_pos[i + subseqlen - 6];
+        sL = L_pos[i + subseqlen - 6];
+        sum_sumU = sum_sumU - fU + sU;
+        sum_sumL = sum_sumL - fL + sL;
+        sum_sumU2 = sum_sumU2 - fU * fU + sU * sU;
+        sum_sumL2 = sum_sumL2 - fL * fL + sL * sL;
+        sum_sumMASK = sum_sumMASK - MASK_pos[i] + MASK_pos[i + subseqlen - 6];
+    }
+    for (int64_t row = diag_bias; row < subcount; row++)
+        ls->normUL[row] = (ls->sumU2_sumL2[row] - 2 * ls->sumU_sumL[row] * mu[row] +
+                           2 * ls->sumMASK[row] * mu[row] * mu[row]) * invsig[row] * invsig[row];
+    for (int64_t i = diag_bias; i < len; i++) ls->raw_DIFF_UL2[i] = ls->raw_DIFF_UL2_temp[i] * MASK[i];
+    {
+        /* movsum_p_from_pos (CPU_version1/allmatrixop/movsum.cpp): sums from position diag_bias */
+        const double *x = ls->raw_DIFF_UL2 + 3;
+        const int64_t b = subseqlen - 7, n2 = (len - 6) - b;
+        double sm = 0;
+        for (int64_t i = diag_bias; i <= b + diag_bias; i++) sm += x[i];
+        for (int64_t i = diag_bias; i < n2 - 1; i++) {
+            ls->DUL2_raw[i] = sm;
+            sm = sm - x[i] + x[i + b + 1];
+        }
+        ls->DUL2_raw[n2 - 1] = sm;
+    }
+    for (int64_t i = diag_bias; i < subcount; i++) ls->DUL2[i] = ls->DUL2_raw[i] * ls->invsig_2[i];
+    const double *dc_bwd_less = ts + 3, *dc_fwd = ts + subseqlen - 4;
+    const double *dr_bwdMASK_less = MASK + 3, *dr_fwdMASK = MASK + subseqlen - 4;
+    const double *dc_bwdTS2_less = TS2 + 3, *dc_fwdTS2 = TS2 + subseqlen - 4;
+    double cov = 0, dTS = 0, dTS2 = 0;
+    for (int k = 0; k < subseqlen - 6; k++) cov += (ts + 3)[k] * ((UTS + diag - 1 + 3)[k] + (LTS + diag - 1 + 3)[k]);
+    for (int k = 0; k < subseqlen - 6; k++) dTS += (ts + 3)[k] * (MASK + diag - 1 + 3)[k];
+    for (int k = 0; k < subseqlen - 6; k++) dTS2 += (TS2 + 3)[k] * (MASK + diag - 1 + 3)[k];
+    const double *sumMASK = ls->sumMASK, *sumU_sumL = ls->sumU_sumL, *nUL = ls->normUL, *DUL2 = ls->DUL2;
+    double M_NORM, t, dist, lb;
+    if (!lb_vector[0]) {
+        int64_t col = 0, row = diag - 1;
+        M_NORM = dTS2 - 2 * mu[col] * dTS + sumMASK[row] * mu[col] * mu[col];
+        M_NORM = M_NORM * invsig[col] * invsig[col];
+        t = (cov - mu[col] * sumU_sumL[row] + 2 * mu[row] * (sumMASK[row] * mu[col] - dTS)) * (invsig[col] * invsig[row]);
+        dist = 2 * M_NORM - 2 * t + nUL[row];
+        if (dist < DUL2[row])
+            lb = 0;
+        else
+            lb = 0.5 * (sqrt(2 * dist - DUL2[row]) - sqrt(ORC_MAX(DUL2[row], 0)));
+        if (lb > bsf || (lb * lb + st->special[2 * col] + st->special[2 * col + 1] > bsf * bsf)) {
+            lb_vector[col] = 1;
+            (*cnt)++;
+        }
+    }
+    double bsf_2 = bsf * bsf;
+    int64_t temp_1 = subcount - diag + 1;
+    for (int64_t lo = 1; lo < temp_1; lo++) {
+        int64_t hi = diag + lo - 1, lo_l = lo - 1, hi_l = hi - 1;
+        cov = cov - dr_bwd_less[hi_l] * dc_bwd_less[lo_l] + dr_fwd[hi] * dc_fwd[lo];
+        dTS = dTS - dr_bwdMASK_less[hi_l] * dc_bwd_less[lo_l] + dr_fwdMASK[hi] * dc_fwd[lo];
+        dTS2 = dTS2 - dr_bwdMASK_less[hi_l] * dc_bwdTS2_less[lo_l] + dr_fwdMASK[hi] * dc_fwdTS2[lo];
+        if (lb_vector[lo]) continue;
+        M_NORM = dTS2 - 2 * mu[lo] * dTS + sumMASK[hi] * mu[lo] * mu[lo];
+        M_NORM = M_NORM * invsig[lo] * invsig[lo];
+        t = (cov - mu[lo] * sumU_sumL[hi] + 2 * mu[hi] * (sumMASK[hi] * mu[lo] - dTS)) * (invsig[lo] * invsig[hi]);
+        dist = 2 * M_NORM - 2 * t + nUL[hi];
+        if (dist < DUL2[hi]) {
+            lb = 0;
+        } else {
+            lb = 0.5 * (sqrt(2 * dist - DUL2[hi]) - sqrt(ORC_MAX(DUL2[hi], 0)));
+            if (lb > bsf || (lb * lb + st->special[2 * lo] + st->special[2 * lo + 1] > bsf_2)) {
+                (*cnt)++;
+                lb_vector[lo] = 1;
+            }
+        }
+    }
+    free(normal_DIFF); free(MASK); free(UTS); free(LTS); free(dr_bwd_less); free(dr_fwd);
+}
+
 /* The state global_tables and diag_mask_global need, with the caller's window statistics (or
  * the reference's movmean when mu is NULL): z-normalised windows and their clipped envelopes
  * (CPU_version1/new_dtw_motifGUI.cpp:31-52) and the raw-series envelope (:96-98). */
@@ -1053,10 +1438,22 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
     double *cb2 = (double *)xcalloc(subseqLen + maxwarp, sizeof(double));
     double *proj = (double *)xcalloc(subseqLen, sizeof(double));
     double *proj_u = (double *)xcalloc(subseqLen, sizeof(double)), *proj_l = (double *)xcalloc(subseqLen, sizeof(double));
-    double pe_cnt = 0, local_rounds = 0;
+    double pe_cnt = 0, local_rounds = 0, local_down_rounds = 0, local_prune_cnt = 0;
     int found = 0;
+    /* the alpha search of the local masks, :332-357 */
+    local_state LS;
+    local_tables(st, &LS);
+    int alpha_interval = ORC_MAX(subseqLen * 5, 1000);
+    int buffer_div_alpha = (int)ceil(1.0 * alpha_interval / buffer_length);
+    static const double alpha_list[15] = {-0.7, -0.6, -0.5, -0.4, -0.3, -0.2, -0.1, 0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7};
+    const double ALPHA_BIAS = 0.1; /* CPU_version1/alldef/matrix.h:16 */
+    double *alpha_table = (double *)xcalloc(subcount, sizeof(double));
+    unsigned char *lb_vector_for_next = (unsigned char *)xcalloc(subcount, 1);
+    double best_alpha = 0;
     for (int i_buffer = 0; i_buffer < buffer_length; i_buffer++) {
+        int i_table_num = 0;
         for (int64_t diag = minlag + 1 + i_buffer; diag <= subcount; diag += buffer_length) {
+            i_table_num++;
             out->fast_stage_rounds++;
             memset(st->lb_vector_new, 0, sizeof(double) * subcount);
             memset(st->lb_vector, 0, subcount);
@@ -1071,12 +1468,64 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
             } else {
                 lb_kim(st, bsf, subcount - diag + 1, (int)diag, &cnt_of_purn);
             }
-            global_prune_cnt += (cnt_of_purn - cnt1);
-            /* :405, :465, :484: the reference's alpha-tuned local masks (diag_mask_local_up /
-               _down) run on this diagonal only when fewer than 80 % of its pairs are pruned by
-               now; they are not restated here, so such a diagonal is counted (local_rounds):
-               with none, the stage sequence is the reference's */
-            if (cnt_of_purn / (double)(subcount - diag + 1) < 0.80) local_rounds++;
+            double cnt2 = cnt_of_purn;
+            global_prune_cnt += (cnt2 - cnt1);
+            pruning_rate = cnt_of_purn / (double)(subcount - diag + 1);
+            /* :400-498: alpha of this diagonal (a search over alpha_list every buffer_div_alpha
+               diagonals of the first pass, a local search or an extrapolation in between, the
+               table interpolated on later passes), then the local masks, up and down, each while
+               fewer than 80 % of the diagonal's pairs are pruned */
+            {
+                const int64_t diag_bias = diag - minlag - 1;
+                double temp_pruning_cnt = 0;
+                int best_pruning_cnt = 0;
+                const double temp_alpha_list[5] = {best_alpha - 2 * ALPHA_BIAS, best_alpha - ALPHA_BIAS, best_alpha,
+                                                   best_alpha + ALPHA_BIAS, best_alpha + 2 * ALPHA_BIAS};
+                if (i_buffer == 0) {
+                    if (i_table_num % buffer_div_alpha == 0) {
+                        for (int q = 0; q < 15; q++) {
+                            temp_pruning_cnt = 0;
+                            memcpy(lb_vector_for_next, st->lb_vector, subcount);
+                            local_up(st, &LS, bsf, (int)diag, lb_vector_for_next, &temp_pruning_cnt, alpha_list[q],
+                                     diag_bias);
+                            if (temp_pruning_cnt > best_pruning_cnt) {
+                                best_pruning_cnt = (int)temp_pruning_cnt;
+                                best_alpha = alpha_list[q];
+                            }
+                        }
+                        alpha_table[i_table_num] = best_alpha;
+                    } else {
+                        if (pruning_rate < 0.85) {
+                            for (int q = 0; q < 5; q++) {
+                                temp_pruning_cnt = 0;
+                                memcpy(lb_vector_for_next, st->lb_vector, subcount);
+                                local_up(st, &LS, bsf, (int)diag, lb_vector_for_next, &temp_pruning_cnt,
+                                         temp_alpha_list[q], diag_bias);
+                                if (temp_pruning_cnt > best_pruning_cnt) {
+                                    best_pruning_cnt = (int)temp_pruning_cnt;
+                                    best_alpha = temp_alpha_list[q];
+                                }
+                            }
+                            alpha_table[i_table_num] = best_alpha;
+                        } else {
+                            alpha_table[i_table_num] = 2 * alpha_table[ORC_MAX(i_table_num - 1, 0)] -
+                                                       alpha_table[ORC_MAX(i_table_num - 2, 0)];
+                        }
+                    }
+                }
+                double alpha = alpha_table[i_table_num] +
+                               (alpha_table[i_table_num + 1] - alpha_table[i_table_num]) / buffer_length * (i_buffer);
+                if (pruning_rate < 0.80) {
+                    local_rounds++;
+                    local_up(st, &LS, bsf, (int)diag, st->lb_vector, &cnt_of_purn, alpha, diag_bias);
+                }
+                pruning_rate = cnt_of_purn / (double)(subcount - diag + 1);
+                if (pruning_rate < 0.80) {
+                    local_down_rounds++;
+                    local_down(st, &LS, bsf, (int)diag, st->lb_vector, &cnt_of_purn, 0, diag_bias);
+                }
+                local_prune_cnt += (cnt_of_purn - cnt2);
+            }
             for (int64_t i = 1; i <= subcount - diag + 1; i++) {
                 all_cnt++;
                 if (st->lb_vector[i - 1]) continue;
@@ -1134,6 +1583,12 @@ int orc_join(const double *a, int64_t n, int32_t m, int32_t w, const double *mp_
     out->dtw_cnt = dtw_cnt;
     out->pe_cnt = pe_cnt;
     out->local_rounds = local_rounds;
+    out->local_down_rounds = local_down_rounds;
+    out->local_cnt = local_prune_cnt;
+    out->bad_bin = LS.bad_bin;
+    local_free(&LS);
+    free(alpha_table);
+    free(lb_vector_for_next);
     /* free everything */
     free(st->mu); free(st->sig); free(st->invsig); free(st->subs); free(st->subsU); free(st->subsL);
     free(min_index); free(cb_temp); free(st->UTS); free(st->LTS); free(st->dr_bwd); free(st->dc_bwd);

@@ -9,16 +9,11 @@
  *  (1) orc_join / orc_motif_discovery: a plain-C restatement of the reference's FP64 CPU
  *      join (CPU_version1/new_dtw_motifGUI.cpp:21-586 and the files it calls), in the same
  *      floating-point operation order, so the seed, the pruning stages, LB_KK, LB_Petitjean
- *      (CPU_version1/LB_PETITJEAN.cpp:9-123), the number of pairs that reach the DTW and the
- *      answer agree with the compiled reference (checked against golden vectors from
- *      oracle/_ref).  Left out: the alpha-tuned local masks
- *      (CPU_version1/new_dtw_motifGUI.cpp:411-498, diag_mask_local_up / _down).  They only
- *      prune pairs that LB_KK would prune as well (a ball bound over masked positions of the
- *      raw-series envelope, which contains the per-window envelope LB_KK uses), so they move
- *      pairs between stages without changing the DTW count, the best-so-far trajectory or
- *      the result; the diagonals on which the reference would run them are counted
- *      (local_rounds), and the DTW count matches the reference's on every fixture, including
- *      those with such diagonals.
+ *      (CPU_version1/LB_PETITJEAN.cpp:9-123), the alpha-tuned local masks with their alpha
+ *      search (CPU_version1/new_dtw_motifGUI.cpp:332-498, diag_mask_local_up / _down,
+ *      compute_shared_data_local), the per-stage prune counts, the number of pairs that reach
+ *      the DTW and the answer agree with the compiled reference (checked against golden vectors
+ *      from oracle/_ref).
  *  (2) orc_bruteforce_top1: textbook banded DTW over every admissible pair (tiny n), with
  *      its own two-pass window statistics; shares no code with (1).
  */
@@ -49,8 +44,12 @@ typedef struct orc_join_result {
     int64_t seed_i, seed_j;     /* 0-based pair that set the seed's best distance (not reported
                                    by the reference, which prints the last seed visited) */
     double pe_cnt;              /* pairs reaching LB_Petitjean (both LB_KK passed) */
-    double local_rounds;        /* diagonals on which the reference would run its local masks
-                                   (not restated: 0 means the stage sequence is the reference's) */
+    double local_rounds;        /* diagonals with diag_mask_local_up ("local_up times") */
+    double local_down_rounds;   /* diagonals with diag_mask_local_down ("local_down times") */
+    double local_cnt;           /* pairs pruned by the two local masks ("local prunes" x all_cnt) */
+    double bad_bin;             /* local-mask count reads at bin 0, where the reference reads index
+                                   -1 of a count row (outside it); read here as 0 (none on the
+                                   fixtures) */
 } orc_join_result;
 
 /* CPU_version1/allmatrixop/movmean.cpp:9-33 */

@@ -20,7 +20,8 @@ class JoinResult(ctypes.Structure):
                 ("global_cnt", ctypes.c_double), ("all_cnt", ctypes.c_int64), ("kk_cnt", ctypes.c_double),
                 ("dtw_cnt", ctypes.c_double), ("fast_stage_rounds", ctypes.c_double),
                 ("global_rounds", ctypes.c_double), ("seed_i", ctypes.c_int64), ("seed_j", ctypes.c_int64),
-                ("pe_cnt", ctypes.c_double), ("local_rounds", ctypes.c_double)]
+                ("pe_cnt", ctypes.c_double), ("local_rounds", ctypes.c_double),
+                ("local_down_rounds", ctypes.c_double), ("local_cnt", ctypes.c_double), ("bad_bin", ctypes.c_double)]
 
 
 def build():

@@ -8,7 +8,9 @@ deterministic generator; its printed result lines are parsed and stored with the
                               is written with 17 significant digits)
     tests/golden/golden.json  per case: generator parameters, m, w and the reference outputs
                               (1-based first/second index, final distance, seed distance,
-                              diag_fast / global prune fractions, its own timer).
+                              the prune fractions of every stage -- diag_fast, global, local
+                              masks, LB_KK, LB_Petitjean -- the DTW count, the number of
+                              diagonals each stage ran on, its own timer).
     tests/golden/grid.json    the reference CPU main's grid (CPU_version1/AAAA.cpp:30-38: band
                               ratios 0.01, 0.02, 0.05, 0.1 outer, m = 32 .. 1024 inner,
                               w = MAX(round(m r), 1)) on one series, one result per configuration.
@@ -87,7 +89,10 @@ def parse(out):
     m = re.search(r"total time is (\S+)s\s+input of bsf:\s*(\S+)\s*,final distance :(\S+)", out)
     r["ref_seconds"], r["input_bsf"], r["final_distance"] = float(m.group(1)), float(m.group(2)), float(m.group(3))
     for key, pat in (("fast_prunes", r"fast prunes (\S+)"), ("global_prunes", r"global prunes (\S+)"),
-                     ("dtw_times", r"DTW times (\S+)")):
+                     ("dtw_times", r"DTW times (\S+)"), ("local_prunes", r"local prunes (\S+)"),
+                     ("kk_prunes", r"KK prunes (\S+)"), ("pe_prunes", r"Pe prunes (\S+)"),
+                     ("fast_times", r"fast times :(\S+)"), ("global_times", r"global times :(\S+)"),
+                     ("local_up_times", r"local_up times :(\S+)"), ("local_down_times", r"local_down times :(\S+)")):
         m = re.search(pat, out)
         r[key] = float(m.group(1)) if m else None
     return r

@@ -2,9 +2,10 @@
 
 tests/golden/golden.json holds, per case, what CPU_version1 printed for the stored series:
 1-based first/second index (CPU_version1/new_dtw_motifGUI.cpp:575), final distance and seed
-distance (:578, printed with %f / %5.3f), the diag_fast / global-stage prune fractions
-(:569-570, printed with 6 significant digits) and the number of pairs that reached the DTW
-("DTW times", :565): the pairs that passed both LB_KK directions and LB_Petitjean.
+distance (:578, printed with %f / %5.3f), the prune fraction of every stage -- diag_fast, the
+global mask stage, the alpha-tuned local masks, LB_KK, LB_Petitjean (:569-574, printed with 6
+significant digits) --, the number of diagonals each stage ran on ("fast / global / local_up /
+local_down times", :561-564) and the number of pairs that reached the DTW ("DTW times", :565).
 """
 import math
 
@@ -26,17 +27,31 @@ def test_oracle_matches_reference(name, oracle):
     assert r.first_min == int(ref["first_min"]) and r.sec_min == int(ref["sec_min"])
     assert abs(r.bsf - ref["final_distance"]) <= 5e-7
     assert abs(r.seed_bsf - ref["input_bsf"]) <= 5e-4
+    check_stage_counts(r, ref, name)
+    assert r.pe_cnt >= r.dtw_cnt
+
+
+def check_stage_counts(r, ref, what):
+    """Every stage's prune fraction (6 significant digits, as printed), the number of diagonals
+    each stage ran on and the DTW count (exact)."""
     all_cnt = r.all_cnt
-    for key, val in (("fast_prunes", r.fast_cnt), ("global_prunes", r.global_cnt)):
+    for key, val in (("fast_prunes", r.fast_cnt), ("global_prunes", r.global_cnt),
+                     ("local_prunes", r.local_cnt), ("kk_prunes", r.kk_cnt - r.pe_cnt),
+                     ("pe_prunes", r.pe_cnt - r.dtw_cnt)):
         got = val / all_cnt
         want = ref[key]
-        assert math.isclose(got, want, rel_tol=6e-6, abs_tol=1e-12), (key, got, want)
-    # the DTW count, exactly: it pins LB_Petitjean (which removes pairs that passed both LB_KK
-    # directions) and every bound before it; the reference's alpha-tuned local masks, which the
-    # oracle does not restate, run on some diagonals of three of these cases (local_rounds) and
-    # only move pairs between stages there
-    assert r.dtw_cnt == ref["dtw_times"], (r.dtw_cnt, ref["dtw_times"], r.pe_cnt, r.local_rounds)
-    assert r.pe_cnt >= r.dtw_cnt
+        assert math.isclose(got, want, rel_tol=6e-6, abs_tol=1e-12), (what, key, got, want)
+    for key, val in (("fast_times", r.fast_stage_rounds), ("global_times", r.global_rounds),
+                     ("local_up_times", r.local_rounds), ("local_down_times", r.local_down_rounds),
+                     ("dtw_times", r.dtw_cnt)):
+        assert val == ref[key], (what, key, val, ref[key])
+    assert r.bad_bin == 0
+
+
+def test_local_masks_prune_on_the_fixtures():
+    """The fixtures exercise the alpha-tuned local masks: some cases prune with them."""
+    pruning = [n for n, c in CASES.items() if c["ref"]["local_prunes"] > 0]
+    assert len(pruning) >= 3, pruning
 
 
 def test_golden_pairs_are_admissible():
@@ -77,4 +92,4 @@ def test_oracle_matches_reference_grid(oracle):
         assert (r.first_min, r.sec_min) == (int(ref["first_min"]), int(ref["sec_min"])), c
         assert abs(r.bsf - ref["final_distance"]) <= 5e-7, c
         assert abs(r.seed_bsf - ref["input_bsf"]) <= 5e-4, c
-        assert r.dtw_cnt == ref["dtw_times"], (c, r.dtw_cnt, r.local_rounds)
+        check_stage_counts(r, ref, (c["m"], c["w"]))
