@@ -137,7 +137,7 @@ inline int dtw_qp_for(int w) {
 // is the band's last offset w needs one.  Same contract and results as dtw_band otherwise.
 template <typename T, int QP, typename Z, typename SUF>
 __device__ __forceinline__ T dtw_band_fused(int m, int w, int seg_len, int nseg, T ub, const Z &z, int &t_stop,
-                                            const SUF &suf) {
+                                            SUF suf) {
     const int lane = (int)(threadIdx.x & 63);
     const T INF = dev_inf<T>();
     const int seg = lane / seg_len;

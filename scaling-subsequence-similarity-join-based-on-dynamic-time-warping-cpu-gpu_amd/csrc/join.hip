@@ -295,6 +295,65 @@ struct CascadeLDS {
 static_assert((511 + 3) / 2 + 1 <= 2 * RC + 2 * (RC + DG),
               "DTW reads before a row window stay in the statistics for the widest band, w = 511");
 
+// Suffix bound of a DTW with one pair per wave (the kernels with several offset pairs per lane)
+// from the pair's LB_Keogh totals, which LB_Keogh formed term by term with the DTW's own
+// arithmetic (fz_term): the sum over the rows (columns) from R on is the total minus the prefix
+// below R, and the prefix is formed only as far as the checks reach, 64 rows at a time (one
+// per lane, then a wave prefix sum).  An FP32 sum of n non-negative terms lies within
+// (1 +- 1.01 n u) of the exact sum (u = 2^-24), so total (1 - 2^-10) - prefix (1 + 2^-10) is at
+// most the exact sum of the suffix's terms for m <= 16000; each term bounds the FP32 costs of its
+// row's (column's) cells (see StagedZ), as the DTW's check requires.
+template <bool PW>
+struct TotSuffix {
+    static constexpr bool enabled = true;
+    const CascadeLDS<PW> *L;
+    int si, sj, m, w2;    // staged windows of the pair, window length, ceil(w / 2)
+    float ii, nij, c;     // the pair's StagedZ coefficients
+    float totC, totR;     // LB_Keogh totals: columns of j against env_i, rows of i against env_j
+    int h;                // last 64-row chunk of the prefix formed (-1: none)
+    float preC, preR;     // inclusive prefix sums up to row 64 h + lane
+    __device__ float at(int t) {
+        const int R = 8 * ((t >> 4) + 1) + w2;  // first row of the suffix at this check
+        if (R >= m) return 0.f;
+        const int q = R - 1;
+        const int lane = (int)(threadIdx.x & 63);
+        while (h < (q >> 6)) {
+            const float cC = h >= 0 ? __shfl(preC, 63, 64) : 0.f;
+            const float cR = h >= 0 ? __shfl(preR, 63, 64) : 0.f;
+            ++h;
+            const int r = 64 * h + lane;
+            float sC = 0.f, sR = 0.f;
+            if (r < m) {
+                const float xi = L->sai[si + r], xj = L->saj[sj + r];
+                float Ui, Li, Uj, Lj;
+                L->env_i(si + r, xi, Ui, Li);
+                L->env_j(sj + r, xj, Uj, Lj);
+                const float dlo = fmaf(xj, nij, fmaf(Li, ii, c)), dhi = fmaf(xj, nij, fmaf(Ui, ii, c));
+                const float ta = fmaxf(fmaxf(dlo, -dhi), 0.f);
+                const float zr = fmaf(xi, ii, c);
+                const float dU = fmaf(Uj, nij, zr), dL = fmaf(Lj, nij, zr);
+                const float tb = fmaxf(fmaxf(dU, -dL), 0.f);
+                sC = ta * ta;
+                sR = tb * tb;
+            }
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const float yC = __shfl_up(sC, off, 64), yR = __shfl_up(sR, off, 64);
+                if (lane >= off) {
+                    sC += yC;
+                    sR += yR;
+                }
+            }
+            preC = sC + cC;
+            preR = sR + cR;
+        }
+        const float pC = __shfl(preC, q & 63, 64), pR = __shfl(preR, q & 63, 64);
+        const float vC = totC * (1.f - 0x1p-10f) - pC * (1.f + 0x1p-10f);
+        const float vR = totR * (1.f - 0x1p-10f) - pR * (1.f + 0x1p-10f);
+        return fmaxf(fmaxf(vC, vR), 0.f);
+    }
+};
+
 // Segment-mean bound of the window pair (gi, gj): len sum_s max(Z_j - U_i, L_i - Z_j, 0)^2 and
 // the same with i and j swapped, the larger of the two (rows of the series' table, see kPaaSeg).
 // For the LB_Keogh terms t_k of a segment's samples, sum t_k^2 >= (sum t_k)^2 / len
@@ -432,6 +491,11 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     uint32_t *q2 = plist + PLCAP;                                // LB_Keogh queue
     uint32_t *q3 = q2 + Q2CAP;                                   // DTW queue
     uint32_t *qp = q3 + Q3CAP;                                   // segment-mean queue
+    // kernels with several offset pairs per lane: the LB_Keogh totals of each DTW queue entry
+    // (columns of window j against the envelope of window i, rows of i against that of j)
+    float *q3ta = reinterpret_cast<float *>(qp + QPCAP), *q3tb = q3ta + Q3CAP;
+    // the cascade's DTW and LB_Keogh form cell differences from raw values (see StagedZ)
+    constexpr bool FZ = Q >= 2;
 
     // ---- stage 4: DTW of every queued pair.  Bands up to 63 run several pairs per wave
     // (segments of w + 1 lanes); wider bands one pair per wave with QP offset pairs per lane.
@@ -462,8 +526,6 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 th = threshold_of(key, A);
                 key = load_bsf(A);
             }
-            // fused cell differences in the kernels with several offset pairs per lane
-            constexpr bool FZ = Q >= 2;
             int li = 0, lj = 0;
             float ii = 0.f, nmi = 0.f, ij = 0.f, nmj = 0.f;
             if (seg < nseg) {
@@ -489,7 +551,8 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
             LaneSuffix suf;
             suf.base = seg * slen;
             suf.seg_len = slen;
-            suf.nq = use_suffix ? suf_rounds : 0;
+            // (the kernels with several offset pairs per lane take TotSuffix below instead)
+            suf.nq = (use_suffix && !FZ) ? suf_rounds : 0;
 #pragma unroll
             for (int q = 0; q < SUF_Q; ++q) suf.reg[q] = 0.f;
             if (suf.nq > 0) {
@@ -572,10 +635,27 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                     }
                 }
             }
-            if (Q == 1 || w <= 63)  // Q = 1 launches hold bands up to 63 only (2w + 1 <= 128)
-                res = dtw_band<float, 1>(m, w, seg_len, nseg, th.dtw_ub, z, t_stop, suf);
-            else
-                res = dtw_band<float, Q>(m, w, 64, 1, th.dtw_ub, z, t_stop, suf);
+            if constexpr (FZ) {
+                if (w > 63) {
+                    // one pair per wave: the suffix bound from its LB_Keogh totals (the margins
+                    // hold up to m = 8192)
+                    if (m <= 8192) {
+                        TotSuffix<PW> ts{&L, li, lj, m, (w + 1) >> 1, ii, -ij, z.c, q3ta[e0], q3tb[e0], -1, 0.f, 0.f};
+                        res = dtw_band<float, Q>(m, w, 64, 1, th.dtw_ub, z, t_stop, ts);
+                    } else {
+                        res = dtw_band<float, Q>(m, w, 64, 1, th.dtw_ub, z, t_stop);
+                    }
+                } else {
+                    // a narrower member of a batch whose widest band needs this kernel: several
+                    // pairs per wave, without the suffix bound
+                    res = dtw_band<float, 1>(m, w, seg_len, nseg, th.dtw_ub, z, t_stop);
+                }
+            } else {
+                if (Q == 1 || w <= 63)  // Q = 1 launches hold bands up to 63 only (2w + 1 <= 128)
+                    res = dtw_band<float, 1>(m, w, seg_len, nseg, th.dtw_ub, z, t_stop, suf);
+                else
+                    res = dtw_band<float, Q>(m, w, 64, 1, th.dtw_ub, z, t_stop, suf);
+            }
             const bool lead = leader && seg < nseg;
             const bool ab = lead && !(res <= th.dtw_ub);
             if (lead) {
@@ -605,6 +685,13 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
         // th: the drain batch's thresholds (at most one batch old)
         const float sm = sqrtf((float)m);
         float T2 = th.T2, sqmT2 = sqrtf(m * T2);  // updated after each DTW run
+        // FZ: LB_Keogh by the DTW's own arithmetic (a term bounds, exactly, every FP32 cell cost
+        // of its row / column, and the FP32 sum of the m terms of a direction is at most
+        // (1 + 1.01 m u) times their exact sum while the DTW's FMA chain loses at most 2 m u:
+        // together at most 1 + 3.1 m u, u = 2^-24), so a pair is dropped only when its FP32 DTW
+        // would exceed the DTW threshold
+        const float zfac = 1.f + 4.f * (float)m * 0x1p-24f;
+        float thrZ = th.dtw_ub * zfac;
         int taken = 0;     // queue entries handed out (uniform)
         bool act = false;  // this lane holds a pair
         int li = 0, lj = 0, k = 0;
@@ -633,18 +720,26 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 lj = pair_col(pr);
                 const float isi = L.ris[li], isj = L.cis[lj];
                 const float mci = L.rmu[li], mcj = L.cmu[lj];
-                // raw envelope of window i mapped into the units of window j, and the reverse
-                rA = isi / isj;  // sigma_j / sigma_i
-                bA = mcj - mci * rA;
-                rB = isj / isi;  // sigma_i / sigma_j
-                bB = mci - mcj * rB;
-                const float eA = 0x1p-22f * ((A.amax * (1.f + rA) + fabsf(bA)) * isj + sm);
-                const float eB = 0x1p-22f * ((A.amax * (1.f + rB) + fabsf(bB)) * isi + sm);
-                // sigma^2 from the hardware reciprocal (1 ulp): the 2^-20 inflation covers it
-                const float rsj = __builtin_amdgcn_rcpf(isj), rsi = __builtin_amdgcn_rcpf(isi);
-                const float sj2 = rsj * rsj, si2 = rsi * rsi;
-                thrA = (T2 + 2.f * eA * sqmT2 + m * eA * eA + m * 0x1p-22f * T2) * sj2 * (1.f + 0x1p-20f);
-                thrB = (T2 + 2.f * eB * sqmT2 + m * eB * eB + m * 0x1p-22f * T2) * si2 * (1.f + 0x1p-20f);
+                if constexpr (FZ) {
+                    // rA = 1/sigma_i, rB = -1/sigma_j, bA = the pair's bias (StagedZ)
+                    rA = isi;
+                    rB = -isj;
+                    bA = pair_bias(mci, isi, mcj, isj);
+                    thrA = thrB = thrZ;
+                } else {
+                    // raw envelope of window i mapped into the units of window j, and the reverse
+                    rA = isi / isj;  // sigma_j / sigma_i
+                    bA = mcj - mci * rA;
+                    rB = isj / isi;  // sigma_i / sigma_j
+                    bB = mci - mcj * rB;
+                    const float eA = 0x1p-22f * ((A.amax * (1.f + rA) + fabsf(bA)) * isj + sm);
+                    const float eB = 0x1p-22f * ((A.amax * (1.f + rB) + fabsf(bB)) * isi + sm);
+                    // sigma^2 from the hardware reciprocal (1 ulp): the 2^-20 inflation covers it
+                    const float rsj = __builtin_amdgcn_rcpf(isj), rsi = __builtin_amdgcn_rcpf(isi);
+                    const float sj2 = rsj * rsj, si2 = rsi * rsi;
+                    thrA = (T2 + 2.f * eA * sqmT2 + m * eA * eA + m * 0x1p-22f * T2) * sj2 * (1.f + 0x1p-20f);
+                    thrB = (T2 + 2.f * eB * sqmT2 + m * eB * eB + m * 0x1p-22f * T2) * si2 * (1.f + 0x1p-20f);
+                }
                 accA = accB = 0.f;
                 k = 0;
                 kb = kc = 0;
@@ -660,11 +755,28 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 // x - (U rA + bA) = (x - bA) - U rA and (L rA + bA) - x = L rA - (x - bA): one
                 // subtraction and two FMAs per direction (rounding error <= 2u (|x| + |U rA| +
                 // |bA|), inside the 4u of eA / eB)
+                // FZ terms of one position: column value xj against the envelope of window i
+                // (accA) and row value xi against that of window j (accB), exactly as the DTW
+                // suffix bound forms them
+                auto fz_term = [&](float xj, float xi, float Ui, float Li, float Uj, float Lj)
+                                   __attribute__((always_inline)) {
+                    const float dlo = fmaf(xj, rB, fmaf(Li, rA, bA)), dhi = fmaf(xj, rB, fmaf(Ui, rA, bA));
+                    const float ta = fmaxf(fmaxf(dlo, -dhi), 0.f);
+                    accA = fmaf(ta, ta, accA);
+                    const float zr = fmaf(xi, rA, bA);
+                    const float dU = fmaf(Uj, rB, zr), dL = fmaf(Lj, rB, zr);
+                    const float tb = fmaxf(fmaxf(dU, -dL), 0.f);
+                    accB = fmaf(tb, tb, accB);
+                };
                 auto term = [&](int kk) __attribute__((always_inline)) {
                     const float xj = L.saj[lj + kk], xi = L.sai[li + kk];
                     float Ui, Li, Uj, Lj;
                     L.env_i(li + kk, xi, Ui, Li);
                     L.env_j(lj + kk, xj, Uj, Lj);
+                    if constexpr (FZ) {
+                        fz_term(xj, xi, Ui, Li, Uj, Lj);
+                        return;
+                    }
                     const float cx = xj - bA;
                     const float t = fmaxf(fmaxf(fmaf(-Ui, rA, cx), fmaf(Li, rA, -cx)), 0.f);
                     accA = fmaf(t, t, accA);
@@ -698,6 +810,10 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                                 float Ui, Li, Uj, Lj;
                                 L.env(wa[t], vw[t], Ui, Li);
                                 L.env(va[t], ww[t], Uj, Lj);
+                                if constexpr (FZ) {
+                                    fz_term(va[t], wa[t], Ui, Li, Uj, Lj);
+                                    continue;
+                                }
                                 const float cx = va[t] - bA;
                                 const float ta = fmaxf(fmaxf(fmaf(-Ui, rA, cx), fmaf(Li, rA, -cx)), 0.f);
                                 accA = fmaf(ta, ta, accA);
@@ -724,6 +840,10 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                             for (int t = 0; t < HB; ++t) {
+                                if constexpr (FZ) {
+                                    fz_term(va[t], wa[t], vu[t], vl[t], wu[t], wl[t]);
+                                    continue;
+                                }
                                 const float cx = va[t] - bA;
                                 const float ta = fmaxf(fmaxf(fmaf(-vu[t], rA, cx), fmaf(vl[t], rA, -cx)), 0.f);
                                 accA = fmaf(ta, ta, accA);
@@ -758,7 +878,14 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
             if (dead) act = false;
             const unsigned long long dk = __builtin_amdgcn_ballot_w64(done);
             if (dk) {
-                if (done) q3[q3n + (int)mbcnt64(dk)] = pack_pair(li, lj);
+                if (done) {
+                    const int slot = q3n + (int)mbcnt64(dk);
+                    q3[slot] = pack_pair(li, lj);
+                    if constexpr (FZ) {
+                        q3ta[slot] = accA;
+                        q3tb[slot] = accB;
+                    }
+                }
                 q3n += __popcll(dk);
                 count(C_KEOGH, __popcll(dk));
                 if (done) act = false;
@@ -770,6 +897,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
             process_q3();
             T2 = th.T2;
             sqmT2 = sqrtf(m * T2);
+            thrZ = th.dtw_ub * zfac;
         }
         }
         q2n = 0;
@@ -1681,8 +1809,10 @@ int launch_paa_pairs(const float *paa, int32_t nseg, int32_t len, const int64_t 
 }
 
 namespace {
-size_t join_lds_bytes_pw(int32_t m, bool pw) {
-    return sizeof(float) * (size_t)union_floats(m, pw) + sizeof(uint32_t) * (PLCAP + Q2CAP + Q3CAP + QPCAP);
+// tot: the kernel carries LB_Keogh totals with its DTW queue (several offset pairs per lane)
+size_t join_lds_bytes_pw(int32_t m, bool pw, bool tot = false) {
+    return sizeof(float) * (size_t)union_floats(m, pw) + sizeof(uint32_t) * (PLCAP + Q2CAP + Q3CAP + QPCAP) +
+           (tot ? sizeof(float) * 2 * Q3CAP : 0);
 }
 // Packed FP16 envelope staging: always for two offset pairs per lane (k_join_occ2), and for one
 // pair per lane when FP32 staging of windows of length m would leave fewer than the 8 waves per
@@ -1694,7 +1824,10 @@ bool join_packed(int Q, int32_t m) {
 }
 }  // namespace
 
-size_t join_lds_bytes(int32_t m, int32_t w) { return join_lds_bytes_pw(m, join_packed(dtw_qp_for(w), m)); }
+size_t join_lds_bytes(int32_t m, int32_t w) {
+    const int Q = dtw_qp_for(w);
+    return join_lds_bytes_pw(m, join_packed(Q, m), Q >= 2);
+}
 
 int launch_join_setup(JoinArgs *d_series, int32_t nseries, unsigned long long *d_work, hipStream_t s) {
     hipLaunchKernelGGL(k_join_setup, dim3((unsigned)((nseries + 63) / 64)), dim3(64), 0, s, d_series, nseries, d_work);
