@@ -119,6 +119,24 @@ __device__ inline Thr threshold_of(unsigned long long key, const JoinArgs &A) {
 
 __device__ inline Thr load_threshold(const JoinArgs &A) { return threshold_of(load_bsf(A), A); }
 
+// DTW threshold of one pair (windows with FP32 centred means mi, mj and 1/sigma ii, ij): a pair
+// whose exact cost is at most T2 has an FP32 DTW of at most (1 + rel_dtw) (sqrt(T2) + e
+// sqrt(2m))^2, e the pair's own bound on the error of one cell difference.  Each difference, in
+// either form (StagedZ, StagedZPlain), rounds the centred sample, mean and 1/sigma once and
+// takes at most five roundings more; with |z| <= sqrt(m) inside a window that gives
+// e <= 2^-24 (max|x| (ii + ij) + 4 (|mi| ii + |mj| ij) + 6 sqrt(m)) (FP32 inputs; the 2^-20
+// covers the products' second-order terms and this evaluation, rel_dtw the rest), by
+// Minkowski's inequality over the at most 2m cells of the path and the path sum's FP32
+// accumulation (rel_dtw).  With the pair's own 1/sigma and means in place of the series maxima
+// of eps_z it is far below dtw_ub on most pairs; the smaller of the two is used.  sT2 =
+// sqrt(T2) as FP32 (its rounding is within rel_dtw), c6m = 6 sqrt(m), s2m = sqrt(2m).
+__device__ inline float pair_dtw_ub(const Thr &t, float sT2, const JoinArgs &A, float c6m, float s2m, float mi,
+                                    float ii, float mj, float ij) {
+    const float e = 0x1p-24f * (1.f + 0x1p-20f) * fmaf(A.amax, ii + ij, fmaf(4.f, fabsf(mi) * ii + fabsf(mj) * ij, c6m));
+    const float r = fmaf(e, s2m, sT2);
+    return fminf((1.f + A.rel_dtw) * (r * r), t.dtw_ub);
+}
+
 // Records a pair whose FP32 cost is within the error margin of the best-so-far (re-scored in
 // FP64 after the join) and lowers the best-so-far.  The push counter is 64-bit, so it keeps
 // counting past cand_cap without wrapping: the host sees how many slots a re-run needs.
@@ -295,35 +313,45 @@ struct CascadeLDS {
 static_assert((511 + 3) / 2 + 1 <= 2 * RC + 2 * (RC + DG),
               "DTW reads before a row window stay in the statistics for the widest band, w = 511");
 
-// Suffix bound of a DTW with one pair per wave (the kernels with several offset pairs per lane)
-// from the pair's LB_Keogh totals, which LB_Keogh formed term by term with the DTW's own
-// arithmetic (fz_term): the sum over the rows (columns) from R on is the total minus the prefix
-// below R, and the prefix is formed only as far as the checks reach, 64 rows at a time (one
-// per lane, then a wave prefix sum).  An FP32 sum of n non-negative terms lies within
-// (1 +- 1.01 n u) of the exact sum (u = 2^-24), so total (1 - 2^-10) - prefix (1 + 2^-10) is at
-// most the exact sum of the suffix's terms for m <= 16000; each term bounds the FP32 costs of its
-// row's (column's) cells (see StagedZ), as the DTW's check requires.
-template <bool PW>
+// Suffix bound of a DTW from the pair's LB_Keogh totals, which LB_Keogh formed term by term with
+// the DTW's own arithmetic (fz_term): the sum over the rows (columns) from R on is the total minus
+// the prefix below R, and the prefix is formed only as far as the checks reach, one chunk of rows
+// at a time.  One pair per wave (G = false): 64-row chunks, lane x forms the terms of row
+// 64 h + x.  Several pairs per wave (G): the pair of DTW segment s has the lane group
+// [s slen, (s + 1) slen), slen = 64 / (pairs per wave), whose lane x forms the terms of row
+// slen h + x of that pair, and a segmented prefix sum runs over each group.  An FP32 sum of n
+// non-negative terms lies within (1 +- 1.01 n u) of the exact sum (u = 2^-24), so total
+// (1 - 2^-10) - prefix (1 + 2^-10) is at most the exact sum of the suffix's terms for
+// m <= 16000; each term bounds the FP32 costs of its row's (column's) cells (see StagedZ), as
+// the DTW's check requires.
+template <bool PW, bool G>
 struct TotSuffix {
     static constexpr bool enabled = true;
     const CascadeLDS<PW> *L;
-    int si, sj, m, w2;    // staged windows of the pair, window length, ceil(w / 2)
-    float ii, nij, c;     // the pair's StagedZ coefficients
-    float totC, totR;     // LB_Keogh totals: columns of j against env_i, rows of i against env_j
-    int h;                // last 64-row chunk of the prefix formed (-1: none)
-    float preC, preR;     // inclusive prefix sums up to row 64 h + lane
+    int si, sj, m, w2;    // staged windows of the lane group's pair, window length, ceil(w / 2)
+    float ii, nij, c;     // that pair's StagedZ coefficients
+    float totC, totR;     // LB_Keogh totals of the lane's DTW pair: columns of j against env_i,
+                          // rows of i against env_j
+    int h;                // last chunk of the prefix formed (-1: none)
+    float preC, preR;     // inclusive prefix sums up to row slen h + x of the group's pair
+    int slen, gx, gbase, rbase;  // (G) lanes per group, index in the group, the group's first
+                                 // lane, first lane of the DTW pair's group
+    bool gvalid;                 // (G) the lane's group has a pair
     __device__ float at(int t) {
         const int R = 8 * ((t >> 4) + 1) + w2;  // first row of the suffix at this check
         if (R >= m) return 0.f;
         const int q = R - 1;
         const int lane = (int)(threadIdx.x & 63);
-        while (h < (q >> 6)) {
-            const float cC = h >= 0 ? __shfl(preC, 63, 64) : 0.f;
-            const float cR = h >= 0 ? __shfl(preR, 63, 64) : 0.f;
+        const int sl = G ? slen : 64;
+        const int x = G ? gx : lane;
+        const int last = G ? min(gbase + sl - 1, 63) : 63;
+        while (G ? (h + 1) * sl <= q : h < (q >> 6)) {
+            const float cC = h >= 0 ? __shfl(preC, last, 64) : 0.f;
+            const float cR = h >= 0 ? __shfl(preR, last, 64) : 0.f;
             ++h;
-            const int r = 64 * h + lane;
+            const int r = sl * h + x;
             float sC = 0.f, sR = 0.f;
-            if (r < m) {
+            if ((!G || gvalid) && r < m) {
                 const float xi = L->sai[si + r], xj = L->saj[sj + r];
                 float Ui, Li, Uj, Lj;
                 L->env_i(si + r, xi, Ui, Li);
@@ -339,7 +367,7 @@ struct TotSuffix {
 #pragma unroll
             for (int off = 1; off < 64; off <<= 1) {
                 const float yC = __shfl_up(sC, off, 64), yR = __shfl_up(sR, off, 64);
-                if (lane >= off) {
+                if (x >= off) {
                     sC += yC;
                     sR += yR;
                 }
@@ -347,7 +375,8 @@ struct TotSuffix {
             preC = sC + cC;
             preR = sR + cR;
         }
-        const float pC = __shfl(preC, q & 63, 64), pR = __shfl(preR, q & 63, 64);
+        const int src = G ? rbase + (q - h * sl) : (q & 63);
+        const float pC = __shfl(preC, src, 64), pR = __shfl(preR, src, 64);
         const float vC = totC * (1.f - 0x1p-10f) - pC * (1.f + 0x1p-10f);
         const float vR = totR * (1.f - 0x1p-10f) - pR * (1.f + 0x1p-10f);
         return fmaxf(fmaxf(vC, vR), 0.f);
@@ -439,7 +468,7 @@ __device__ inline int pair_col(uint32_t p) { return (int)(p >> 16); }
 // device work queue is the range [0, item_base[nseries]) claimed through work_counter), then
 // the claim-order multiplier;
 // uni: floats of the LDS union area, sized for the longest window of the launch.
-template <int Q, bool PW>
+template <int Q, bool PW, bool SW>
 __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base,
                                           int32_t nseries, int32_t uni, unsigned long long *work_counter, uint32_t *queue) {
     extern __shared__ __align__(16) float smem[];  // 16-byte aligned: the sweep reads float4 rows
@@ -494,8 +523,10 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     // kernels with several offset pairs per lane: the LB_Keogh totals of each DTW queue entry
     // (columns of window j against the envelope of window i, rows of i against that of j)
     float *q3ta = reinterpret_cast<float *>(qp + QPCAP), *q3tb = q3ta + Q3CAP;
-    // the cascade's DTW and LB_Keogh form cell differences from raw values (see StagedZ)
-    constexpr bool FZ = Q >= 2;
+    // the cascade's DTW and LB_Keogh form cell differences from raw values (see StagedZ): in
+    // the kernels with several offset pairs per lane, and in the one-pair-per-lane kernels for
+    // batches whose bands reach 16 (SW), where the DTW has a suffix bound
+    constexpr bool FZ = Q >= 2 || SW;
 
     // ---- stage 4: DTW of every queued pair.  Bands up to 63 run several pairs per wave
     // (segments of w + 1 lanes); wider bands one pair per wave with QP offset pairs per lane.
@@ -503,6 +534,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
         const JoinArgs &A = *Ap;
         const CascadeLDS<PW> L(smem, m);
         const int w = A.w;
+        const float c6m = 6.f * sqrtf((float)m), s2m = sqrtf(2.f * (float)m);
         const int seg_len = w <= 63 ? w + 1 : 64;
         const int per_wave = 64 / seg_len;
         const int seg = lane / seg_len;
@@ -537,6 +569,9 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 ij = L.cis[lj];
                 nmj = -L.cmu[lj] * ij;
             }
+            // the pair's own DTW threshold (the same value on every lane of its segment)
+            const float ub = seg < nseg ? pair_dtw_ub(th, sqrtf(th.T2), A, c6m, s2m, L.rmu[li], ii, L.cmu[lj], ij)
+                                        : th.dtw_ub;
             using ZT = std::conditional_t<FZ, StagedZ, StagedZPlain>;
             ZT z;
             if constexpr (FZ)
@@ -636,28 +671,50 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 }
             }
             if constexpr (FZ) {
-                if (w > 63) {
-                    // one pair per wave: the suffix bound from its LB_Keogh totals (the margins
-                    // hold up to m = 8192)
-                    if (m <= 8192) {
-                        TotSuffix<PW> ts{&L, li, lj, m, (w + 1) >> 1, ii, -ij, z.c, q3ta[e0], q3tb[e0], -1, 0.f, 0.f};
-                        res = dtw_band<float, Q>(m, w, 64, 1, th.dtw_ub, z, t_stop, ts);
-                    } else {
-                        res = dtw_band<float, Q>(m, w, 64, 1, th.dtw_ub, z, t_stop);
+                // suffix bound from the LB_Keogh totals (the margins hold up to m = 8192); the
+                // one-pair-per-lane kernels without SW run narrow bands without it
+                if (Q >= 2 && w > 63 && m <= 8192) {
+                    TotSuffix<PW, false> ts{&L, li, lj, m, (w + 1) >> 1, ii, -ij, z.c, q3ta[e0], q3tb[e0], -1, 0.f, 0.f};
+                    res = dtw_band<float, Q>(m, w, 64, 1, ub, z, t_stop, ts);
+                } else if (SW && w >= 16 && m <= 8192) {
+                    TotSuffix<PW, true> ts;
+                    ts.L = &L;
+                    ts.m = m;
+                    ts.w2 = (w + 1) >> 1;
+                    ts.slen = slen;
+                    ts.gbase = sseg * slen;
+                    ts.gx = lane - ts.gbase;
+                    ts.gvalid = sseg < nseg;
+                    ts.si = ts.sj = 0;
+                    ts.ii = ts.nij = ts.c = 0.f;
+                    if (ts.gvalid) {
+                        const uint32_t pr = q3[e0 + sseg];
+                        ts.si = pair_row(pr);
+                        ts.sj = pair_col(pr);
+                        ts.ii = L.ris[ts.si];
+                        const float gij = L.cis[ts.sj];
+                        ts.nij = -gij;
+                        ts.c = pair_bias(L.rmu[ts.si], ts.ii, L.cmu[ts.sj], gij);
                     }
+                    ts.rbase = seg * slen;
+                    ts.totC = seg < nseg ? q3ta[e0 + seg] : 0.f;
+                    ts.totR = seg < nseg ? q3tb[e0 + seg] : 0.f;
+                    ts.h = -1;
+                    ts.preC = ts.preR = 0.f;
+                    res = dtw_band<float, 1>(m, w, seg_len, nseg, ub, z, t_stop, ts);
+                } else if (w > 63) {
+                    res = dtw_band<float, Q>(m, w, 64, 1, ub, z, t_stop);
                 } else {
-                    // a narrower member of a batch whose widest band needs this kernel: several
-                    // pairs per wave, without the suffix bound
-                    res = dtw_band<float, 1>(m, w, seg_len, nseg, th.dtw_ub, z, t_stop);
+                    res = dtw_band<float, 1>(m, w, seg_len, nseg, ub, z, t_stop);
                 }
             } else {
                 if (Q == 1 || w <= 63)  // Q = 1 launches hold bands up to 63 only (2w + 1 <= 128)
-                    res = dtw_band<float, 1>(m, w, seg_len, nseg, th.dtw_ub, z, t_stop, suf);
+                    res = dtw_band<float, 1>(m, w, seg_len, nseg, ub, z, t_stop, suf);
                 else
-                    res = dtw_band<float, Q>(m, w, 64, 1, th.dtw_ub, z, t_stop, suf);
+                    res = dtw_band<float, Q>(m, w, 64, 1, ub, z, t_stop, suf);
             }
             const bool lead = leader && seg < nseg;
-            const bool ab = lead && !(res <= th.dtw_ub);
+            const bool ab = lead && !(res <= ub);
             if (lead) {
                 const unsigned long long cells = (unsigned long long)dtw_cells_upto(m, w, t_stop);
                 if (cells + c_cells >= (1ull << 31)) {
@@ -684,14 +741,14 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
         __syncthreads();  // q2 entries were written by other lanes
         // th: the drain batch's thresholds (at most one batch old)
         const float sm = sqrtf((float)m);
-        float T2 = th.T2, sqmT2 = sqrtf(m * T2);  // updated after each DTW run
+        float T2 = th.T2, sqmT2 = sqrtf(m * T2), sT2 = sqrtf(T2);  // updated after each DTW run
         // FZ: LB_Keogh by the DTW's own arithmetic (a term bounds, exactly, every FP32 cell cost
         // of its row / column, and the FP32 sum of the m terms of a direction is at most
         // (1 + 1.01 m u) times their exact sum while the DTW's FMA chain loses at most 2 m u:
         // together at most 1 + 3.1 m u, u = 2^-24), so a pair is dropped only when its FP32 DTW
-        // would exceed the DTW threshold
+        // would exceed the pair's DTW threshold (pair_dtw_ub)
         const float zfac = 1.f + 4.f * (float)m * 0x1p-24f;
-        float thrZ = th.dtw_ub * zfac;
+        const float c6m = 6.f * sm, s2m = sqrtf(2.f * (float)m);
         int taken = 0;     // queue entries handed out (uniform)
         bool act = false;  // this lane holds a pair
         int li = 0, lj = 0, k = 0;
@@ -725,7 +782,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                     rA = isi;
                     rB = -isj;
                     bA = pair_bias(mci, isi, mcj, isj);
-                    thrA = thrB = thrZ;
+                    thrA = thrB = pair_dtw_ub(th, sT2, A, c6m, s2m, mci, isi, mcj, isj) * zfac;
                 } else {
                     // raw envelope of window i mapped into the units of window j, and the reverse
                     rA = isi / isj;  // sigma_j / sigma_i
@@ -897,7 +954,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
             process_q3();
             T2 = th.T2;
             sqmT2 = sqrtf(m * T2);
-            thrZ = th.dtw_ub * zfac;
+            sT2 = sqrtf(T2);
         }
         }
         q2n = 0;
@@ -1584,10 +1641,10 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     flush_counters();
 }
 
-template <int Q, bool PW>
+template <int Q, bool PW, bool SW>
 __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base,
                                              int32_t nseries, int32_t uni, unsigned long long *work_counter, uint32_t *queue) {
-    join_body<Q, PW>(series, item_base, nseries, uni, work_counter, queue);
+    join_body<Q, PW, SW>(series, item_base, nseries, uni, work_counter, queue);
 }
 
 // The same join held to the register budget of two waves per SIMD (a few spills to scratch):
@@ -1597,7 +1654,7 @@ template <int Q>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_join_occ2(
     const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base, int32_t nseries, int32_t uni,
     unsigned long long *work_counter, uint32_t *queue) {
-    join_body<Q, true>(series, item_base, nseries, uni, work_counter, queue);
+    join_body<Q, true, false>(series, item_base, nseries, uni, work_counter, queue);
 }
 
 // ---- set-up of every series of a join: the FP32 error model from the preprocessing's value
@@ -1817,16 +1874,20 @@ size_t join_lds_bytes_pw(int32_t m, bool pw, bool tot = false) {
 // Packed FP16 envelope staging: always for two offset pairs per lane (k_join_occ2), and for one
 // pair per lane when FP32 staging of windows of length m would leave fewer than the 8 waves per
 // CU its registers allow (m > 402: at m = 512, 7 waves instead of 8).
-bool join_packed(int Q, int32_t m) {
+bool join_packed(int Q, int32_t m, bool tot = false) {
     if (packed_envelope(Q)) return true;
     if (Q != 1) return false;
-    return 160 * 1024 / (join_lds_bytes_pw(m, false) + 8 * kWcnt) < 8;
+    return 160 * 1024 / (join_lds_bytes_pw(m, false, tot) + 8 * kWcnt) < 8;
 }
+// The one-pair-per-lane join with the suffix bound from LB_Keogh totals: launches whose widest
+// band has a suffix bound (w >= 16) and fits one offset pair per lane.
+bool join_sw(int Q, int32_t max_w) { return Q == 1 && max_w >= 16; }
 }  // namespace
 
 size_t join_lds_bytes(int32_t m, int32_t w) {
     const int Q = dtw_qp_for(w);
-    return join_lds_bytes_pw(m, join_packed(Q, m), Q >= 2);
+    const bool tot = Q >= 2 || join_sw(Q, w);
+    return join_lds_bytes_pw(m, join_packed(Q, m, tot), tot);
 }
 
 int launch_join_setup(JoinArgs *d_series, int32_t nseries, unsigned long long *d_work, hipStream_t s) {
@@ -1838,29 +1899,37 @@ int launch_join_setup(JoinArgs *d_series, int32_t nseries, unsigned long long *d
 int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t nseries, int32_t max_m, int32_t max_w,
                 unsigned long long *d_work, uint32_t *d_queue, int32_t grid, hipStream_t s) {
     const int Q = dtw_qp_for(max_w);
-    const bool pw = join_packed(Q, max_m);
+    const bool sw = join_sw(Q, max_w);
+    const bool pw = join_packed(Q, max_m, Q >= 2 || sw);
     const size_t lds = join_lds_bytes(max_m, max_w);
     const int32_t uni = union_floats(max_m, pw);
     if (lds + 8 * kWcnt > 160 * 1024) {  // + the kernel's static stage counters
         set_error("window too long for the join's LDS staging");
         return DTWJ_EINVAL;
     }
-#define DTWJ_LAUNCH_JOIN(QQ, PP)                                                                                  \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_join<QQ, PP>), dim3(grid), dim3(64), lds, s, d_series, d_item_base, nseries, \
-                       uni, d_work, d_queue)
+#define DTWJ_LAUNCH_JOIN(QQ, PP, SS)                                                                                  \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_join<QQ, PP, SS>), dim3(grid), dim3(64), lds, s, d_series, d_item_base, \
+                       nseries, uni, d_work, d_queue)
     switch (Q) {
         case 1:
-            if (pw)
-                DTWJ_LAUNCH_JOIN(1, true);
-            else
-                DTWJ_LAUNCH_JOIN(1, false);
+            if (sw) {
+                if (pw)
+                    DTWJ_LAUNCH_JOIN(1, true, true);
+                else
+                    DTWJ_LAUNCH_JOIN(1, false, true);
+            } else {
+                if (pw)
+                    DTWJ_LAUNCH_JOIN(1, true, false);
+                else
+                    DTWJ_LAUNCH_JOIN(1, false, false);
+            }
             break;
         case 2:  // two waves per SIMD (packed envelope staging)
             hipLaunchKernelGGL(k_join_occ2<2>, dim3(grid), dim3(64), lds, s, d_series, d_item_base, nseries, uni,
                                d_work, d_queue);
             break;
-        case 4: DTWJ_LAUNCH_JOIN(4, false); break;
-        case 8: DTWJ_LAUNCH_JOIN(8, false); break;
+        case 4: DTWJ_LAUNCH_JOIN(4, false, false); break;
+        case 8: DTWJ_LAUNCH_JOIN(8, false, false); break;
         default: set_error("band too wide: 2w+1 must be <= 1024"); return DTWJ_EINVAL;
     }
 #undef DTWJ_LAUNCH_JOIN
