@@ -751,10 +751,22 @@ int run_join(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w
         if (int rc = launch_join_setup(d_args, 1, d_work, s)) return rc;
         if (int rc = fetch_error_model(J, d_args, s)) return rc;
         if (int rc = enqueue_seeds(J, d_args, s)) return rc;
-        unsigned long long seed_key = 0;
-        DTWJ_HIP(hipMemcpyAsync(&seed_key, J.d_bsf, sizeof(seed_key), hipMemcpyDeviceToHost, s));
-        DTWJ_HIP(hipStreamSynchronize(s));
-        if (seed_dist < 0) seed_dist = std::sqrt((double)bsf_cost(seed_key));
+        if (seed_dist < 0) {
+            // the seeds' best FP32 DTW cost, from their candidate records (the best-so-far word
+            // holds a bound on that pair's exact cost, slightly above it)
+            unsigned long long nseed = 0;
+            DTWJ_HIP(hipMemcpyAsync(&nseed, J.d_ncands, sizeof(nseed), hipMemcpyDeviceToHost, s));
+            DTWJ_HIP(hipStreamSynchronize(s));
+            nseed = std::min<unsigned long long>(nseed, (unsigned long long)J.cand_cap);
+            std::vector<int4> sc((size_t)nseed);
+            if (nseed > 0) {
+                DTWJ_HIP(hipMemcpyAsync(sc.data(), J.d_cands, nseed * sizeof(int4), hipMemcpyDeviceToHost, s));
+                DTWJ_HIP(hipStreamSynchronize(s));
+            }
+            float best = 3.0e38f;
+            for (const int4 &c : sc) best = std::min(best, __builtin_bit_cast(float, c.z));
+            seed_dist = std::sqrt((double)best);
+        }
         const auto t2 = clk::now();
         // ---- fused join
         if (c->ev0) DTWJ_HIP(hipEventRecord(c->ev0, s));

@@ -106,10 +106,12 @@ __device__ inline unsigned long long load_bsf(const JoinArgs &A) {
 }
 
 // Thresholds of every stage from a best-so-far word (an older word is larger: still valid).
+// The word's cost is at most that of a pair the join has seen as an exact (FP64) DTW cost
+// (push_candidate records that bound, not the FP32 cost), so it serves as T2 as it is.
 __device__ inline Thr threshold_of(unsigned long long key, const JoinArgs &A) {
     const float cost = bsf_cost(key);
     Thr t;
-    t.T2 = cost + dtw_err(cost, A);
+    t.T2 = cost;
     t.T = sqrtf(t.T2) * (1.0f + 0x1p-20f);
     const float e = A.eps_z;
     t.kim = t.T2 + 4.0f * e * sqrtf(12.0f * t.T2) + 48.0f * e * e + 0x1p-20f * t.T2;
@@ -119,31 +121,44 @@ __device__ inline Thr threshold_of(unsigned long long key, const JoinArgs &A) {
 
 __device__ inline Thr load_threshold(const JoinArgs &A) { return threshold_of(load_bsf(A), A); }
 
-// DTW threshold of one pair (windows with FP32 centred means mi, mj and 1/sigma ii, ij): a pair
-// whose exact cost is at most T2 has an FP32 DTW of at most (1 + rel_dtw) (sqrt(T2) + e
-// sqrt(2m))^2, e the pair's own bound on the error of one cell difference.  Each difference, in
-// either form (StagedZ, StagedZPlain), rounds the centred sample, mean and 1/sigma once and
-// takes at most five roundings more; with |z| <= sqrt(m) inside a window that gives
-// e <= 2^-24 (max|x| (ii + ij) + 4 (|mi| ii + |mj| ij) + 6 sqrt(m)) (FP32 inputs; the 2^-20
-// covers the products' second-order terms and this evaluation, rel_dtw the rest), by
-// Minkowski's inequality over the at most 2m cells of the path and the path sum's FP32
-// accumulation (rel_dtw).  With the pair's own 1/sigma and means in place of the series maxima
-// of eps_z it is far below dtw_ub on most pairs; the smaller of the two is used.  sT2 =
-// sqrt(T2) as FP32 (its rounding is within rel_dtw), c6m = 6 sqrt(m), s2m = sqrt(2m).
-__device__ inline float pair_dtw_ub(const Thr &t, float sT2, const JoinArgs &A, float c6m, float s2m, float mi,
-                                    float ii, float mj, float ij) {
-    const float e = 0x1p-24f * (1.f + 0x1p-20f) * fmaf(A.amax, ii + ij, fmaf(4.f, fabsf(mi) * ii + fabsf(mj) * ij, c6m));
-    const float r = fmaf(e, s2m, sT2);
-    return fminf((1.f + A.rel_dtw) * (r * r), t.dtw_ub);
+// Bound on the error of one FP32 cell difference of the pair of windows with FP32 centred means
+// mi, mj and 1/sigma ii, ij, in each form the kernels use (StagedZ, StagedZPlain, the seed DTW's
+// (x - mean) (1/sigma)): each rounds the centred sample, the mean and 1/sigma once and takes at
+// most five roundings more, so with |z| <= sqrt(m) inside a window
+// e <= 2^-24 (max|x| (ii + ij) + 4 (|mi| ii + |mj| ij) + 8 sqrt(m)); the 2^-20 covers the
+// second-order terms and this evaluation.  c8m = 8 sqrt(m).  With the pair's own 1/sigma and
+// means in place of the series maxima it is far below the 2 eps_z of the series-wide model.
+__device__ inline float pair_cell_err(const JoinArgs &A, float c8m, float mi, float ii, float mj, float ij) {
+    return 0x1p-24f * (1.f + 0x1p-20f) * fmaf(A.amax, ii + ij, fmaf(4.f, fabsf(mi) * ii + fabsf(mj) * ij, c8m));
+}
+
+// (1 + rel_dtw) (r + e sqrt(2m))^2 for a pair with cell-difference error bound e (s2m =
+// sqrt(2m)).  By Minkowski's inequality over the at most 2m cells of a path and the FP32
+// accumulation of a path sum (at most 2m u relative, inside rel_dtw, which also covers the
+// rounding of r):
+//  * r = sqrt(C): a pair whose exact cost is at most C has an FP32 DTW of at most this (the
+//    FP32 DTW is at most its FP32 sum along the exact optimal path);
+//  * r = sqrt(FP32 DTW cost): the pair's exact cost is at most this (the exact cost is at most
+//    the exact sum along the path the FP32 DTW took).
+__device__ inline float widen_cost(float r, float e, float s2m, const JoinArgs &A) {
+    const float q = fmaf(e, s2m, r);
+    return (1.f + A.rel_dtw) * (q * q);
+}
+
+// DTW threshold of one pair: a pair whose exact cost is at most T2 (sT2 = sqrt(T2)) stays at or
+// below it; the series-wide dtw_ub is used where smaller.
+__device__ inline float pair_dtw_ub(const Thr &t, float sT2, float e, float s2m, const JoinArgs &A) {
+    return fminf(widen_cost(sT2, e, s2m, A), t.dtw_ub);
 }
 
 // Records a pair whose FP32 cost is within the error margin of the best-so-far (re-scored in
-// FP64 after the join) and lowers the best-so-far.  The push counter is 64-bit, so it keeps
-// counting past cand_cap without wrapping: the host sees how many slots a re-run needs.
-__device__ inline void push_candidate(const JoinArgs &A, int i, int j, float cost) {
+// FP64 after the join) and lowers the best-so-far to key_cost, a bound on the pair's exact cost
+// (widen_cost of its FP32 cost).  The push counter is 64-bit, so it keeps counting past
+// cand_cap without wrapping: the host sees how many slots a re-run needs.
+__device__ inline void push_candidate(const JoinArgs &A, int i, int j, float cost, float key_cost) {
     const unsigned long long slot = atomicAdd(A.n_cands, 1ull);
     if (slot < (unsigned long long)A.cand_cap) A.cands[slot] = make_int4(i, j, __float_as_int(cost), 0);
-    atomicMin(A.bsf, pack_bsf(cost, (uint32_t)i));
+    atomicMin(A.bsf, pack_bsf(key_cost, (uint32_t)i));
 }
 
 __device__ inline float sq(float x) { return x * x; }
@@ -534,7 +549,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
         const JoinArgs &A = *Ap;
         const CascadeLDS<PW> L(smem, m);
         const int w = A.w;
-        const float c6m = 6.f * sqrtf((float)m), s2m = sqrtf(2.f * (float)m);
+        const float c8m = 8.f * sqrtf((float)m), s2m = sqrtf(2.f * (float)m);
         const int seg_len = w <= 63 ? w + 1 : 64;
         const int per_wave = 64 / seg_len;
         const int seg = lane / seg_len;
@@ -569,9 +584,9 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 ij = L.cis[lj];
                 nmj = -L.cmu[lj] * ij;
             }
-            // the pair's own DTW threshold (the same value on every lane of its segment)
-            const float ub = seg < nseg ? pair_dtw_ub(th, sqrtf(th.T2), A, c6m, s2m, L.rmu[li], ii, L.cmu[lj], ij)
-                                        : th.dtw_ub;
+            // the pair's own error bound and DTW threshold (the same on every lane of its segment)
+            const float perr = pair_cell_err(A, c8m, L.rmu[li], ii, L.cmu[lj], ij);
+            const float ub = seg < nseg ? pair_dtw_ub(th, sqrtf(th.T2), perr, s2m, A) : th.dtw_ub;
             using ZT = std::conditional_t<FZ, StagedZ, StagedZPlain>;
             ZT z;
             if constexpr (FZ)
@@ -727,7 +742,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
             count(C_DTW_AB, __popcll(__builtin_amdgcn_ballot_w64(ab)));
             if (lead && !ab) {
                 const int gi = (int)(cb + li), gj = (int)(cb + d0 + lj);
-                push_candidate(A, gi, gj, res);
+                push_candidate(A, gi, gj, res, widen_cost(sqrtf(res), perr, s2m, A));
             }
         }
         q3n = 0;
@@ -748,7 +763,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
         // together at most 1 + 3.1 m u, u = 2^-24), so a pair is dropped only when its FP32 DTW
         // would exceed the pair's DTW threshold (pair_dtw_ub)
         const float zfac = 1.f + 4.f * (float)m * 0x1p-24f;
-        const float c6m = 6.f * sm, s2m = sqrtf(2.f * (float)m);
+        const float c8m = 8.f * sm, s2m = sqrtf(2.f * (float)m);
         int taken = 0;     // queue entries handed out (uniform)
         bool act = false;  // this lane holds a pair
         int li = 0, lj = 0, k = 0;
@@ -782,7 +797,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                     rA = isi;
                     rB = -isj;
                     bA = pair_bias(mci, isi, mcj, isj);
-                    thrA = thrB = pair_dtw_ub(th, sT2, A, c6m, s2m, mci, isi, mcj, isj) * zfac;
+                    thrA = thrB = pair_dtw_ub(th, sT2, pair_cell_err(A, c8m, mci, isi, mcj, isj), s2m, A) * zfac;
                 } else {
                     // raw envelope of window i mapped into the units of window j, and the reverse
                     rA = isi / isj;  // sigma_j / sigma_i
@@ -1826,7 +1841,11 @@ __device__ __forceinline__ void seed_dtw_block(const JoinArgs &A, const int2 *pa
     const PlainZ<float> z{zs, zs + m};
     int t_stop = 0;
     const float res = dtw_band<float, QP>(m, A.w, 64, 1, th.dtw_ub, z, t_stop);
-    if (res <= th.dtw_ub && lane == 0) push_candidate(A, i, j, res);
+    if (res <= th.dtw_ub && lane == 0) {
+        const float sm = sqrtf((float)m);
+        const float e = pair_cell_err(A, 8.f * sm, mi, ii, mj, ij);
+        push_candidate(A, i, j, res, widen_cost(sqrtf(res), e, sqrtf(2.f * (float)m), A));
+    }
 }
 
 template <int QP>
