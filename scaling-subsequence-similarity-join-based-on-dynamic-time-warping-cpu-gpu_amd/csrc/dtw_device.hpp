@@ -184,9 +184,14 @@ __device__ __forceinline__ T dtw_band_fused(int m, int w, int seg_len, int nseg,
     // row term of pair k (shared by its even and odd cell) and a cell's difference
     auto rowz = [&](int k, T xr) __attribute__((always_inline)) { return (T)fma(xr, (T)z.ii, cbias[k]); };
     auto diff = [&](T zr, T xc) __attribute__((always_inline)) { return (T)fma(xc, (T)z.nij, zr); };
+    // neighbour values across lanes (DPP wave shifts): the edge lane of each shift (lane 0 from
+    // the left, lane 63 from the right) has no source and keeps the register's old value, which
+    // is +inf from here on, so the shifts need no +inf reloaded each step
+    T lin = INF, rin = INF;
     auto odd_step = [&](int r[QP], T zr[QP]) __attribute__((always_inline)) {
         // odd offsets: up = next even (t-1), left = own even (t-1), diag = own odd (t-2)
-        const T right_in = from_right<T>(INF, E[0]);
+        rin = from_right<T>(rin, E[0]);
+        const T right_in = rin;
 #pragma unroll
         for (int k = 0; k < QP; ++k) {
             const T up = k + 1 < QP ? E[k + 1] : right_in;
@@ -233,7 +238,8 @@ __device__ __forceinline__ T dtw_band_fused(int m, int w, int seg_len, int nseg,
 #pragma unroll
         for (int k = 0; k < QP; ++k) rz[k] = rowz(k, zr[k]);
         // even offsets at t: up = own odd (t-1), left = previous odd (t-1), diag = own even (t-2)
-        const T left_in = from_left<T>(INF, O[QP - 1]);
+        lin = from_left<T>(lin, O[QP - 1]);
+        const T left_in = lin;
 #pragma unroll
         for (int k = 0; k < QP; ++k) {
             const T left = k > 0 ? O[k - 1] : left_in;
@@ -242,7 +248,8 @@ __device__ __forceinline__ T dtw_band_fused(int m, int w, int seg_len, int nseg,
         }
         if (odd) {
             // odd offsets: up = next even (t), left = own even (t), diag = own odd (t-1)
-            const T right_in = from_right<T>(INF, E[0]);
+            rin = from_right<T>(rin, E[0]);
+            const T right_in = rin;
 #pragma unroll
             for (int k = 0; k < QP; ++k) {
                 const T up = k + 1 < QP ? E[k + 1] : right_in;
