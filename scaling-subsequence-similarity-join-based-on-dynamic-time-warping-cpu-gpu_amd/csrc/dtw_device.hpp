@@ -133,8 +133,9 @@ inline int dtw_qp_for(int w) {
 // c = mu_j / sigma_j - mu_i / sigma_i, two FMAs per pair of cells sharing a row and no
 // separate normalisation of the columns.  The row term of an offset pair wholly past the band
 // (both members out, or a lane outside every segment) takes c = +inf: its differences are then
-// -inf and its cells +inf without a select; only the odd member of the pair whose even member
-// is the band's last offset w needs one.  Same contract and results as dtw_band otherwise.
+// +inf and its cells +inf without a select; the odd member of the pair whose even member is
+// the band's last offset w takes a NaN column coefficient instead (nijo).  Same contract and
+// results as dtw_band otherwise.
 template <typename T, int QP, typename Z, typename SUF>
 __device__ __forceinline__ T dtw_band_fused(int m, int w, int seg_len, int nseg, T ub, const Z &z, int &t_stop,
                                             SUF suf) {
@@ -184,6 +185,13 @@ __device__ __forceinline__ T dtw_band_fused(int m, int w, int seg_len, int nseg,
     // row term of pair k (shared by its even and odd cell) and a cell's difference
     auto rowz = [&](int k, T xr) __attribute__((always_inline)) { return (T)fma(xr, (T)z.ii, cbias[k]); };
     auto diff = [&](T zr, T xc) __attribute__((always_inline)) { return (T)fma(xc, (T)z.nij, zr); };
+    // column coefficient of each odd cell: z.nij, or NaN for an odd member past the band.  Its
+    // difference and cell are then NaN, which every min3 here ignores as it would +inf (IEEE
+    // minNum), and which no check sees (a check takes min3 from +inf); so no select per cell.
+    T nijo[QP];
+#pragma unroll
+    for (int k = 0; k < QP; ++k) nijo[k] = inO[k] ? (T)z.nij : (T)__builtin_nan("");
+    auto diffo = [&](T zr, T xc, int k) __attribute__((always_inline)) { return (T)fma(xc, nijo[k], zr); };
     // neighbour values across lanes (DPP wave shifts): the edge lane of each shift (lane 0 from
     // the left, lane 63 from the right) has no source and keeps the register's old value, which
     // is +inf from here on, so the shifts need no +inf reloaded each step
@@ -195,9 +203,8 @@ __device__ __forceinline__ T dtw_band_fused(int m, int w, int seg_len, int nseg,
 #pragma unroll
         for (int k = 0; k < QP; ++k) {
             const T up = k + 1 < QP ? E[k + 1] : right_in;
-            const T d = diff(rowz(k, zr[k]), colv(r[k] + e_ix[k] + 1));
-            const T v = fma(d, d, min3(O[k], up, E[k]));
-            O[k] = inO[k] ? v : INF;
+            const T d = diffo(rowz(k, zr[k]), colv(r[k] + e_ix[k] + 1), k);
+            O[k] = fma(d, d, min3(O[k], up, E[k]));
         }
     };
     bool alive = valid;
@@ -253,9 +260,8 @@ __device__ __forceinline__ T dtw_band_fused(int m, int w, int seg_len, int nseg,
 #pragma unroll
             for (int k = 0; k < QP; ++k) {
                 const T up = k + 1 < QP ? E[k + 1] : right_in;
-                const T d = diff(rz[k], zco[k]);
-                const T v = fma(d, d, min3(O[k], up, E[k]));
-                O[k] = inO[k] ? v : INF;
+                const T d = diffo(rz[k], zco[k], k);
+                O[k] = fma(d, d, min3(O[k], up, E[k]));
             }
         }
 #pragma unroll
