@@ -4,6 +4,7 @@ Everything is built in-tree with hipcc (no JIT cache):
     <package>/lib/libdtwj.so   shared library exporting include/dtwj.h
     <package>/bin/gpu_dtw      drop-in for the reference's GPU_dtw <data> <time_out> <m> <r>
     <package>/bin/copy_calib   plain 1 GiB copy kernel for calibrating HBM traffic counters
+    <package>/bin/fp16_env_check   the packed FP16 envelope decode against plain arithmetic
 
 Usage: python build.py [-j N]
 """
@@ -30,7 +31,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", o
 LIB_SOURCES = ["dtwj_api.cpp", "series_gen.cpp", "loader.cpp", "prep.hip", "join.hip", "dtw.hip", "proposal.hip",
                "noalloc.hip", "mpx.hip", "mask.hip"]
 CLI_SOURCES = ["gpu_dtw_cli.cpp"]
-TOOL_SOURCES = ["copy_calib.hip"]   # standalone executables (traffic-counter calibration)
+TOOL_SOURCES = ["copy_calib.hip", "fp16_env_check.hip"]   # standalone executables (traffic-counter calibration,
+                                                         # check of the packed-width decode)
 
 
 def _compile(src, force=False):

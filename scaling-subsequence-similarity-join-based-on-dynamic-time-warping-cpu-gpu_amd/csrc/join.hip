@@ -42,6 +42,7 @@
 #include <type_traits>
 
 #include "dtw_device.hpp"
+#include "fp16_env.hpp"
 #include "../../include/dtwj.h"
 #include "internal.hpp"
 
@@ -303,10 +304,7 @@ struct CascadeLDS {
         }
     }
     // widened envelope (upper, lower) at a sample x with packed widths wb
-    __device__ static void env(float x, uint32_t wb, float &up, float &lo) {
-        up = x + (float)__builtin_bit_cast(_Float16, (uint16_t)(wb & 0xffffu));
-        lo = x - (float)__builtin_bit_cast(_Float16, (uint16_t)(wb >> 16));
-    }
+    __device__ static void env(float x, uint32_t wb, float &up, float &lo) { fp16_env_decode(x, wb, up, lo); }
     // envelope of the row window at position k (series value x = sai[k]), of the column window
     __device__ void env_i(int k, float x, float &up, float &lo) const {
         if constexpr (PW)
