@@ -526,6 +526,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     int64_t cb = 0;  // first row of the RC rows staged for the cascade
     int m = 0;
     Thr th{};
+    unsigned long long thk = ~0ull;  // best-so-far word th was computed from in the cascade
     float *jarr = smem;                // NJARR x JW2 column tables of the sweep
     float *rarr = jarr + NJARR * JW2;  // NJARR x RC2 row tables of the sweep
     // cascade staging (valid only while draining), laid out per series from its window length
@@ -540,6 +541,23 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     // the kernels with several offset pairs per lane, and in the one-pair-per-lane kernels for
     // batches whose bands reach 16 (SW), where the DTW has a suffix bound
     constexpr bool FZ = Q >= 2 || SW;
+    // In the cascade of the kernels with the fused arithmetic (FZ), th is recomputed (three
+    // square roots) only when the best-so-far word differs from thk, the word it was last
+    // computed from there; a wave moving to another series invalidates it (no packed word has
+    // an all-ones cost field).  The sweep's chunk staging recomputes th from the current word
+    // without updating thk: words only fall, so when a later word equals thk, th is from that
+    // word or a lower one, both valid.  (The other kernels recompute it each time: their code
+    // measured faster that way.)
+    auto set_th = [&](unsigned long long key) __attribute__((always_inline)) {
+        if constexpr (FZ) {
+            if (key != thk) {
+                th = threshold_of(key, *Ap);
+                thk = key;
+            }
+        } else {
+            th = threshold_of(key, *Ap);
+        }
+    };
 
     // ---- stage 4: DTW of every queued pair.  Bands up to 63 run several pairs per wave
     // (segments of w + 1 lanes); wider bands one pair per wave with QP offset pairs per lane.
@@ -568,7 +586,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
         for (int e0 = 0; e0 < q3n; e0 += per_wave) {
             const int nseg = (q3n - e0) < per_wave ? (q3n - e0) : per_wave;
             if (e0 > 0) {
-                th = threshold_of(key, A);
+                set_th(key);
                 key = load_bsf(A);
             }
             int li = 0, lj = 0;
@@ -1139,7 +1157,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
             }
             count(C_BALL, (unsigned long long)total);
             __syncthreads();
-            th = threshold_of(key, A);
+            set_th(key);
             for (int p = 0;; p += 64) {
                 const bool act = p + lane < total;
                 int pi = 0, pj = 0;
@@ -1240,6 +1258,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
             cur = s;
             Ap = series + s;
             m = Ap->m;
+            thk = ~0ull;
         }
         const JoinArgs &A = *Ap;
         if (!A.valid) continue;
