@@ -1173,32 +1173,56 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                     const float ae0 = ai[m - 1], ae1 = ai[m - 2], ae2 = ai[m - 3];
                     const float be0 = aj[m - 1], be1 = aj[m - 2], be2 = aj[m - 3];
                     __builtin_amdgcn_sched_barrier(0);
-                    // x: window i (columns), y: window j (rows)
-                    // z-values as one FMA each, x / sigma - mean / sigma (within eps_z)
-                    const float nmi = -mi * ii, nmj = -mj * ij;
-                    const float x0 = fmaf(a0, ii, nmi), x1 = fmaf(a1, ii, nmi), x2 = fmaf(a2, ii, nmi);
-                    const float y0 = fmaf(b0, ij, nmj), y1 = fmaf(b1, ij, nmj), y2 = fmaf(b2, ij, nmj);
-                    const float xe0 = fmaf(ae0, ii, nmi), xe1 = fmaf(ae1, ii, nmi), xe2 = fmaf(ae2, ii, nmi);
-                    const float ye0 = fmaf(be0, ij, nmj), ye1 = fmaf(be1, ij, nmj), ye2 = fmaf(be2, ij, nmj);
                     // Any warping path from (0,0) first leaves {0,1}^2 at (2,0),(2,1),(2,2),(1,2) or
                     // (0,2), entered from an adjacent cell of {0,1}^2.  Same at the far corner.
-                    const float c00 = sq(y0 - x0), c10 = sq(y1 - x0), c01 = sq(y0 - x1), c11 = sq(y1 - x1);
-                    const float c20 = sq(y2 - x0), c21 = sq(y2 - x1), c22 = sq(y2 - x2), c12 = sq(y1 - x2),
-                                c02 = sq(y0 - x2);
-                    float bl = c10 + c20;
-                    bl = fminf(bl, fminf(c10, c11) + c21);
-                    bl = fminf(bl, c11 + c22);
-                    bl = fminf(bl, fminf(c01, c11) + c12);
-                    bl = fminf(bl, c01 + c02);
-                    const float e00 = sq(ye0 - xe0), e10 = sq(ye1 - xe0), e01 = sq(ye0 - xe1), e11 = sq(ye1 - xe1);
-                    const float e20 = sq(ye2 - xe0), e21 = sq(ye2 - xe1), e22 = sq(ye2 - xe2), e12 = sq(ye1 - xe2),
-                                e02 = sq(ye0 - xe2);
-                    float br = e10 + e20;
-                    br = fminf(br, fminf(e10, e11) + e21);
-                    br = fminf(br, e11 + e22);
-                    br = fminf(br, fminf(e01, e11) + e12);
-                    br = fminf(br, e01 + e02);
-                    const float kim = (c00 + bl) + (e00 + br);
+                    auto corner = [](float c00, float c10, float c01, float c11, float c20, float c21, float c22,
+                                     float c12, float c02) __attribute__((always_inline)) {
+                        float b = c10 + c20;
+                        b = fminf(b, fminf(c10, c11) + c21);
+                        b = fminf(b, c11 + c22);
+                        b = fminf(b, fminf(c01, c11) + c12);
+                        b = fminf(b, c01 + c02);
+                        return c00 + b;
+                    };
+                    // x: window i (columns), y: window j (rows)
+                    float kim;
+                    if constexpr (FZ) {
+                        // the DTW's own fused difference fma(y, -1/sigma_j, fma(x, 1/sigma_i, c))
+                        // (within the 2 eps_z of two rounded z-values; the row term formed once
+                        // per sample of window i)
+                        const float cz = pair_bias(mi, ii, mj, ij), nij = -ij;
+                        const float x0 = fmaf(a0, ii, cz), x1 = fmaf(a1, ii, cz), x2 = fmaf(a2, ii, cz);
+                        const float xe0 = fmaf(ae0, ii, cz), xe1 = fmaf(ae1, ii, cz), xe2 = fmaf(ae2, ii, cz);
+                        auto d2 = [&](float y, float xz) __attribute__((always_inline)) { return sq(fmaf(y, nij, xz)); };
+                        kim = corner(d2(b0, x0), d2(b1, x0), d2(b0, x1), d2(b1, x1), d2(b2, x0), d2(b2, x1), d2(b2, x2),
+                                     d2(b1, x2), d2(b0, x2)) +
+                              corner(d2(be0, xe0), d2(be1, xe0), d2(be0, xe1), d2(be1, xe1), d2(be2, xe0), d2(be2, xe1),
+                                     d2(be2, xe2), d2(be1, xe2), d2(be0, xe2));
+                    } else {
+                        // z-values as one FMA each, x / sigma - mean / sigma (within eps_z)
+                        const float nmi = -mi * ii, nmj = -mj * ij;
+                        const float x0 = fmaf(a0, ii, nmi), x1 = fmaf(a1, ii, nmi), x2 = fmaf(a2, ii, nmi);
+                        const float y0 = fmaf(b0, ij, nmj), y1 = fmaf(b1, ij, nmj), y2 = fmaf(b2, ij, nmj);
+                        const float xe0 = fmaf(ae0, ii, nmi), xe1 = fmaf(ae1, ii, nmi), xe2 = fmaf(ae2, ii, nmi);
+                        const float ye0 = fmaf(be0, ij, nmj), ye1 = fmaf(be1, ij, nmj), ye2 = fmaf(be2, ij, nmj);
+                        const float c00 = sq(y0 - x0), c10 = sq(y1 - x0), c01 = sq(y0 - x1), c11 = sq(y1 - x1);
+                        const float c20 = sq(y2 - x0), c21 = sq(y2 - x1), c22 = sq(y2 - x2), c12 = sq(y1 - x2),
+                                    c02 = sq(y0 - x2);
+                        float bl = c10 + c20;
+                        bl = fminf(bl, fminf(c10, c11) + c21);
+                        bl = fminf(bl, c11 + c22);
+                        bl = fminf(bl, fminf(c01, c11) + c12);
+                        bl = fminf(bl, c01 + c02);
+                        const float e00 = sq(ye0 - xe0), e10 = sq(ye1 - xe0), e01 = sq(ye0 - xe1), e11 = sq(ye1 - xe1);
+                        const float e20 = sq(ye2 - xe0), e21 = sq(ye2 - xe1), e22 = sq(ye2 - xe2), e12 = sq(ye1 - xe2),
+                                    e02 = sq(ye0 - xe2);
+                        float br = e10 + e20;
+                        br = fminf(br, fminf(e10, e11) + e21);
+                        br = fminf(br, e11 + e22);
+                        br = fminf(br, fminf(e01, e11) + e12);
+                        br = fminf(br, e01 + e02);
+                        kim = (c00 + bl) + (e00 + br);
+                    }
                     alive = kim <= th.kim;
                 }
                 const unsigned long long msk = __builtin_amdgcn_ballot_w64(alive);
