@@ -537,9 +537,10 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     // kernels with several offset pairs per lane: the LB_Keogh totals of each DTW queue entry
     // (columns of window j against the envelope of window i, rows of i against that of j)
     float *q3ta = reinterpret_cast<float *>(qp + QPCAP), *q3tb = q3ta + Q3CAP;
-    // the cascade's DTW and LB_Keogh form cell differences from raw values (see StagedZ): in
-    // the kernels with several offset pairs per lane, and in the one-pair-per-lane kernels for
-    // batches whose bands reach 16 (SW), where the DTW has a suffix bound
+    // the cascade's DTW, LB_Keogh and LB_Kim form cell differences from raw values (see
+    // StagedZ): in the kernels with several offset pairs per lane, and in the one-pair-per-lane
+    // kernels of launches whose widest band is 4 or more (SW; bands of 16 and more there also
+    // take the DTW suffix bound from the LB_Keogh totals)
     constexpr bool FZ = Q >= 2 || SW;
     // In the cascade of the kernels with the fused arithmetic (FZ), th is recomputed (three
     // square roots) only when the best-so-far word differs from thk, the word it was last
@@ -702,8 +703,8 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 }
             }
             if constexpr (FZ) {
-                // suffix bound from the LB_Keogh totals (the margins hold up to m = 8192); the
-                // one-pair-per-lane kernels without SW run narrow bands without it
+                // suffix bound from the LB_Keogh totals (the margins hold up to m = 8192) for
+                // bands of 16 and more; narrower ones run without it
                 if (Q >= 2 && w > 63 && m <= 8192) {
                     TotSuffix<PW, false> ts{&L, li, lj, m, (w + 1) >> 1, ii, -ij, z.c, q3ta[e0], q3tb[e0], -1, 0.f, 0.f};
                     res = dtw_band<float, Q>(m, w, 64, 1, ub, z, t_stop, ts);
@@ -1939,9 +1940,10 @@ bool join_packed(int Q, int32_t m, bool tot = false) {
     if (Q != 1) return false;
     return 160 * 1024 / (join_lds_bytes_pw(m, false, tot) + 8 * kWcnt) < 8;
 }
-// The one-pair-per-lane join with the suffix bound from LB_Keogh totals: launches whose widest
-// band has a suffix bound (w >= 16) and fits one offset pair per lane.
-bool join_sw(int Q, int32_t max_w) { return Q == 1 && max_w >= 16; }
+// The one-pair-per-lane join with the fused arithmetic (and, for bands of 16 and more, the
+// suffix bound from LB_Keogh totals): launches whose widest band is 4 to 63.  The narrowest
+// (w <= 3, sweep-bound) keep the separately normalised form, which measured faster there.
+bool join_sw(int Q, int32_t max_w) { return Q == 1 && max_w >= 4; }
 }  // namespace
 
 size_t join_lds_bytes(int32_t m, int32_t w) {
