@@ -1938,6 +1938,7 @@ size_t join_lds_bytes_pw(int32_t m, bool pw, bool tot = false) {
 bool join_packed(int Q, int32_t m, bool tot = false) {
     if (packed_envelope(Q)) return true;
     if (Q != 1) return false;
+    if (tot) return true;  // the fused-arithmetic one-pair kernels: packed staging at every m
     return 160 * 1024 / (join_lds_bytes_pw(m, false, tot) + 8 * kWcnt) < 8;
 }
 // The one-pair-per-lane join with the fused arithmetic (and, for bands of 16 and more, the
