@@ -1976,10 +1976,7 @@ int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t ns
     switch (Q) {
         case 1:
             if (sw) {
-                if (pw)
-                    DTWJ_LAUNCH_JOIN(1, true, true);
-                else
-                    DTWJ_LAUNCH_JOIN(1, false, true);
+                DTWJ_LAUNCH_JOIN(1, true, true);  // join_packed() holds for these launches
             } else {
                 if (pw)
                     DTWJ_LAUNCH_JOIN(1, true, false);
