@@ -1146,15 +1146,32 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
             const uint32_t mask = ~e & 15u;
             const int li = (int)(e >> 10) - row0;
             const int lb = li + KD * (int)((e >> 4) & 63u);
-            // pairs listed diagonal-bit-major: the pairs of bit q follow those of bits < q, in
-            // lane order (one ballot per bit instead of a cross-lane prefix sum)
             int total = 0;
+            if constexpr (FZ) {
+                // pairs listed entry-major: an entry's pairs (consecutive columns of one row)
+                // follow those of the lanes below it, so that neighbouring list slots read
+                // neighbouring windows in the stages behind (fewer LDS bank conflicts).  The
+                // offsets are the prefix sums of the per-lane counts (0..4), bit by bit.
+                const uint32_t cnt = (uint32_t)__popc(mask);
+                const unsigned long long c0 = __builtin_amdgcn_ballot_w64(cnt & 1u);
+                const unsigned long long c1 = __builtin_amdgcn_ballot_w64(cnt & 2u);
+                const unsigned long long c2 = __builtin_amdgcn_ballot_w64(cnt & 4u);
+                int o = (int)mbcnt64(c0) + 2 * (int)mbcnt64(c1) + 4 * (int)mbcnt64(c2);
+                total = __popcll(c0) + 2 * __popcll(c1) + 4 * __popcll(c2);
 #pragma unroll
-            for (int q = 0; q < KD; ++q) {
-                const bool has = (mask >> q) & 1u;
-                const unsigned long long bq = __builtin_amdgcn_ballot_w64(has);
-                if (has) plist[total + (int)mbcnt64(bq)] = pack_pair(li, lb + q);
-                total += __popcll(bq);
+                for (int q = 0; q < KD; ++q) {
+                    if ((mask >> q) & 1u) plist[o++] = pack_pair(li, lb + q);
+                }
+            } else {
+                // pairs listed diagonal-bit-major: the pairs of bit q follow those of bits < q,
+                // in lane order (one ballot per bit instead of a cross-lane prefix sum)
+#pragma unroll
+                for (int q = 0; q < KD; ++q) {
+                    const bool has = (mask >> q) & 1u;
+                    const unsigned long long bq = __builtin_amdgcn_ballot_w64(has);
+                    if (has) plist[total + (int)mbcnt64(bq)] = pack_pair(li, lb + q);
+                    total += __popcll(bq);
+                }
             }
             count(C_BALL, (unsigned long long)total);
             __syncthreads();
