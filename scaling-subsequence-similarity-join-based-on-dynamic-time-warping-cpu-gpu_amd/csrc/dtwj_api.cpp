@@ -519,8 +519,7 @@ struct Proposal {
     float *d_norm = nullptr; // feature mean, scale
     int64_t *d_vi = nullptr, *d_vj = nullptr;
     double *d_vcost = nullptr;
-    uint64_t *d_k0 = nullptr, *d_k1 = nullptr;  // candidate keys before / after the sort
-    void *d_sort_tmp = nullptr;
+    uint64_t *d_keys = nullptr;  // candidate keys, sorted in place (pair_sort_len entries)
     std::vector<int2> cands;  // candidate pairs in (i, j) order (host)
 };
 
@@ -533,8 +532,7 @@ int propose(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w,
     const size_t rows16 = (ncand + 15) / 16 * 16;
     const int np = mlp_param_count(), pad = mlp_feature_pad();
     const int nv = std::max(n_verify, 1) * 2;
-    const size_t sort_bytes = pair_sort_tmp_bytes((int64_t)ncand);
-    if (ncand > 0 && sort_bytes == 0) return fail(DTWJ_EDEVICE, "radix sort set-up failed");
+    const int64_t sort_len = pair_sort_len((int64_t)ncand);
     auto carve = [&](Arena &ar) {
         take_job(ar, J);
         Pr.d_cand = ar.take<int2>(ncand + 1);
@@ -548,9 +546,7 @@ int propose(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w,
         Pr.d_vi = ar.take<int64_t>(nv);
         Pr.d_vj = ar.take<int64_t>(nv);
         Pr.d_vcost = ar.take<double>(nv);
-        Pr.d_k0 = ar.take<uint64_t>(ncand + 1);
-        Pr.d_k1 = ar.take<uint64_t>(ncand + 1);
-        Pr.d_sort_tmp = ar.take<char>(sort_bytes + 1);
+        Pr.d_keys = ar.take<uint64_t>((size_t)sort_len);
     };
     Arena dry;
     carve(dry);
@@ -583,9 +579,9 @@ int propose(dtwj_context *c, const double *d_a, int64_t n, int32_t m, int32_t w,
     if (int rc = launch_seed_lanes(S, Pr.d_cand, s)) return rc;
     // the lanes' pairs in (i, j) order, sorted on the device; duplicates are dropped (none
     // arise: every (item, lane) of the scan owns its own rows and diagonals)
-    if (int rc = launch_pair_sort(Pr.d_cand, (int64_t)ncand, Pr.d_k0, Pr.d_k1, Pr.d_sort_tmp, sort_bytes, s)) return rc;
+    if (int rc = launch_pair_sort(Pr.d_cand, (int64_t)ncand, Pr.d_keys, sort_len, s)) return rc;
     std::vector<uint64_t> keys(ncand);
-    DTWJ_HIP(hipMemcpyAsync(keys.data(), Pr.d_k1, sizeof(uint64_t) * ncand, hipMemcpyDeviceToHost, s));
+    DTWJ_HIP(hipMemcpyAsync(keys.data(), Pr.d_keys, sizeof(uint64_t) * ncand, hipMemcpyDeviceToHost, s));
     DTWJ_HIP(hipStreamSynchronize(s));
     Pr.cands.clear();
     Pr.cands.reserve(ncand);

@@ -135,11 +135,10 @@ int mlp_feature_pad();
 void mlp_to_params(const dtwj_mlp &M, float *P);
 void params_to_mlp(const float *P, dtwj_mlp &M);
 int launch_prop_features(const JoinArgs &A, const int2 *cand, int count, float *F, hipStream_t s);
-// Candidate pairs sorted by (i, j) on the device (radix sort of 64-bit keys; entries without a
-// pair sort last as ~0); tmp holds pair_sort_tmp_bytes(count) bytes.
-size_t pair_sort_tmp_bytes(int64_t count);
-int launch_pair_sort(const int2 *cand, int64_t count, uint64_t *key_in, uint64_t *key_out, void *tmp, size_t tmp_bytes,
-                     hipStream_t s);
+// Candidate pairs sorted by (i, j) on the device (bitonic network over 64-bit keys; entries
+// without a pair sort last as ~0): key holds len = pair_sort_len(count) keys, a power of two.
+int64_t pair_sort_len(int64_t count);
+int launch_pair_sort(const int2 *cand, int64_t count, uint64_t *key, int64_t len, hipStream_t s);
 int launch_prop_pack(const float *F, int count, int rows, const float *mean, const float *scale, uint16_t *X,
                      hipStream_t s);
 int launch_mlp_forward(const uint16_t *X, int count, const uint16_t *W1b, const float *P, float *out, hipStream_t s);

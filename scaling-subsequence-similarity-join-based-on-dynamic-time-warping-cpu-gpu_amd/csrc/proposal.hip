@@ -10,8 +10,6 @@
 // Forward: M = 16 samples, K = 48 padded features (3 steps), N = 64 hidden (4 tiles).
 // Weight gradient dW1 = X^T dH: M = 48 features (3 tiles), K = 16 samples, N = 64 hidden; the
 // hidden gradient dH comes out of the forward in the D layout, which is exactly the B layout.
-#include <hipcub/hipcub.hpp>
-
 #include "common.hpp"
 #include "../../include/dtwj.h"
 #include "internal.hpp"
@@ -341,29 +339,96 @@ void params_to_mlp(const float *P, dtwj_mlp &M) {
 }
 
 // Candidate pairs as 64-bit keys (i << 32 | j, both non-negative and below 2^31), entries
-// without a pair as the largest key, so that an ascending sort orders pairs by (i, j) and puts
-// the empty entries last.
-__global__ __launch_bounds__(256) void k_pair_keys(const int2 *__restrict__ cand, int64_t count, uint64_t *key) {
+// without a pair, and the padding up to the sorted length, as the largest key, so that an
+// ascending sort orders pairs by (i, j) and puts the empty entries last.
+__global__ __launch_bounds__(256) void k_pair_keys(const int2 *__restrict__ cand, int64_t count, int64_t len,
+                                                   uint64_t *key) {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= count) return;
-    const int2 p = cand[t];
-    key[t] = p.x >= 0 ? ((uint64_t)(uint32_t)p.x << 32) | (uint32_t)p.y : ~0ull;
+    if (t >= len) return;
+    uint64_t k = ~0ull;
+    if (t < count) {
+        const int2 p = cand[t];
+        if (p.x >= 0) k = ((uint64_t)(uint32_t)p.x << 32) | (uint32_t)p.y;
+    }
+    key[t] = k;
 }
 
-size_t pair_sort_tmp_bytes(int64_t count) {
-    size_t bytes = 0;
-    if (hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, (const uint64_t *)nullptr, (uint64_t *)nullptr, (int)count, 0,
-                                          64) != hipSuccess)
-        return 0;
-    return bytes;
+// Bitonic sorting network over len = 2^p keys.  Stage k (the length of the runs being merged,
+// 2 .. len) takes steps j = k / 2 .. 1; step j compares key i with key i + j for every i whose
+// bit j is clear, and orders the two ascending where bit k of i is clear, descending where it is
+// set.  Thread t of a step owns the pair whose lower index is i = 2 t - (t mod j), so every key
+// is touched once per step.  Steps with j < SB / 2 stay inside one block of SB keys and run in
+// LDS, all of them in one launch; the wider steps of a stage run one launch each over HBM.
+constexpr int SB = 2048;  // keys per block of the LDS steps (1024 threads, one pair each)
+
+__device__ __forceinline__ void cmp_swap(uint64_t &a, uint64_t &b, bool up) {
+    if ((a > b) == up) {
+        const uint64_t t = a;
+        a = b;
+        b = t;
+    }
 }
 
-int launch_pair_sort(const int2 *cand, int64_t count, uint64_t *key_in, uint64_t *key_out, void *tmp, size_t tmp_bytes,
-                     hipStream_t s) {
+// Stages k_lo .. k_hi (powers of two), each from step min(k, SB) / 2 down to 1, over the block's
+// SB keys; the direction of a pair follows the global index of its lower key.
+__global__ __launch_bounds__(1024) void k_bitonic_lds(uint64_t *key, int64_t k_lo, int64_t k_hi) {
+    __shared__ uint64_t v[SB];
+    const int t = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * SB;
+    v[t] = key[base + t];
+    v[t + SB / 2] = key[base + t + SB / 2];
+    __syncthreads();
+    for (int64_t k = k_lo; k <= k_hi; k <<= 1) {
+        for (int j = (int)(k < SB ? k : SB) / 2; j > 0; j >>= 1) {
+            const int i = 2 * t - (t & (j - 1));
+            cmp_swap(v[i], v[i + j], ((base + i) & k) == 0);
+            __syncthreads();
+        }
+    }
+    key[base + t] = v[t];
+    key[base + t + SB / 2] = v[t + SB / 2];
+}
+
+// One step j >= SB / 2 of stage k over all len keys (len / 2 threads).
+__global__ __launch_bounds__(256) void k_bitonic_step(uint64_t *key, int64_t half, int64_t k, int64_t j) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= half) return;
+    const int64_t i = 2 * t - (t & (j - 1));
+    uint64_t a = key[i], b = key[i + j];
+    const bool up = (i & k) == 0;
+    if ((a > b) == up) {
+        key[i] = b;
+        key[i + j] = a;
+    }
+}
+
+int64_t pair_sort_len(int64_t count) {
+    int64_t len = SB;
+    while (len < count) len <<= 1;
+    return len;
+}
+
+int launch_pair_sort(const int2 *cand, int64_t count, uint64_t *key, int64_t len, hipStream_t s) {
     if (count <= 0) return 0;
-    hipLaunchKernelGGL(k_pair_keys, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, cand, count, key_in);
+    if (len < count || len < SB || (len & (len - 1)) != 0) {
+        set_error("pair sort: the key buffer length must be a power of two, at least 2048 and the count");
+        return DTWJ_EINVAL;
+    }
+    hipLaunchKernelGGL(k_pair_keys, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, cand, count, len, key);
     DTWJ_HIP(hipGetLastError());
-    DTWJ_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, (const uint64_t *)key_in, key_out, (int)count, 0, 64, s));
+    const unsigned blocks = (unsigned)(len / SB);
+    // stages up to SB entirely in LDS
+    hipLaunchKernelGGL(k_bitonic_lds, dim3(blocks), dim3(SB / 2), 0, s, key, (int64_t)2, (int64_t)SB);
+    DTWJ_HIP(hipGetLastError());
+    for (int64_t k = 2 * (int64_t)SB; k <= len; k <<= 1) {
+        for (int64_t j = k / 2; j >= SB; j >>= 1) {
+            hipLaunchKernelGGL(k_bitonic_step, dim3((unsigned)((len / 2 + 255) / 256)), dim3(256), 0, s, key, len / 2,
+                               k, j);
+            DTWJ_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_bitonic_lds, dim3(blocks), dim3(SB / 2), 0, s, key, k, k);
+        DTWJ_HIP(hipGetLastError());
+    }
     return 0;
 }
 

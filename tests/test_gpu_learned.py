@@ -65,6 +65,19 @@ def test_candidate_features_match_numpy(gpu):
         assert np.expm1(got[37]) * m <= cost * (1 + 1e-4) + 1e-6
 
 
+@pytest.mark.parametrize("n,n_items", [(20000, 32), (20000, 100), (400000, 4096)])
+def test_candidates_come_sorted_and_distinct(gpu, n, n_items):
+    """The device sort (a bitonic network over the padded power-of-two key buffer: 2048, 8192 and
+    262,144 keys here, the last with stages merged through HBM) returns the candidates in
+    strictly increasing (i, j) order; the features above pin the pairs themselves."""
+    a, _, _ = gpu.generate(n, 11, n_plants=1, plant_len=128)
+    pi, pj, _ = gpu.proposal_candidates(a, 128, 6, n_items=n_items)
+    assert 0.1 * 64 * n_items < len(pi) <= 64 * n_items
+    key = pi.astype(np.int64) * (1 << 32) + pj
+    assert np.all(np.diff(key) > 0)
+    assert np.all(pj - pi >= 128) and np.all(pj <= a.size - 128)
+
+
 def _random_model(gpu, rng, feats):
     model = gpu.mlp_init(feats, seed=int(rng.integers(1 << 30)))
     np.ctypeslib.as_array(model.b1)[:] = rng.standard_normal(64).astype(np.float32) * 0.2
