@@ -63,8 +63,7 @@ constexpr int JW1 = RC1 + DG + 8;
 static_assert(3 * JW1 + 3 * RC1 <= kSweepFloats, "one-sided tables fit the two-sided area");
 static_assert(RC1 % RC == 0 && RC2 % RC == 0, "the cascade drains a chunk in RC-row parts");
 constexpr int PLCAP = 64 * KD;         // expanded pairs of one 64-entry queue batch
-constexpr int Q2CAP = 256;             // LB_Keogh queue (packed pairs)
-constexpr int Q2RUN = Q2CAP - 64;      // run LB_Keogh once this many pairs are queued
+constexpr int Q2CAP = 256;             // LB_Keogh queue (packed pairs); it runs from Q2CAP - 64 on
 constexpr int Q3CAP = 128;             // DTW queue (packed pairs)
 constexpr int QPCAP = 128;             // segment-mean queue (packed pairs): < 64 left + one LB_Kim batch
 constexpr int QLIM = kJoinQueueCap - 8 * 64;  // drain before an 8-row group could overflow
@@ -78,13 +77,22 @@ static_assert(kWcnt >= C_NCOUNTERS && kWcnt % 2 == 0, "stage counters");
 // and envelope of the chunk's row windows and column windows (upper and lower envelope in FP32,
 // or, with pw, one word of packed FP16 widths), and the centred mean and 1/sigma of those
 // windows.
-__host__ __device__ constexpr int cascade_floats(int m, bool pw) {
-    return (pw ? 2 : 3) * (RC + m) + (pw ? 2 : 3) * (RC + DG + m) + 2 * RC + 2 * (RC + DG);
+__host__ __device__ constexpr int cascade_floats(int m, bool pw, bool gs = false) {
+    return (pw ? 2 : 3) * (RC + m) + (pw ? 2 : 3) * (RC + DG + m) + 2 * RC + (gs ? 0 : 2 * (RC + DG));
 }
 
-__host__ __device__ constexpr int union_floats(int m, bool pw) {
-    return cascade_floats(m, pw) > kSweepFloats ? cascade_floats(m, pw) : kSweepFloats;
+__host__ __device__ constexpr int union_floats(int m, bool pw, bool gs = false) {
+    return cascade_floats(m, pw, gs) > kSweepFloats ? cascade_floats(m, pw, gs) : kSweepFloats;
 }
+
+// gs: the join reads the mean and 1/sigma of the cascade's column windows from global memory
+// instead of staging them in LDS, its DTW queue holds 64 pairs and its LB_Keogh queue 128 (it
+// runs from 64 queued pairs on).  Launches of two offset pairs per lane take that form where
+// it fits more waves per CU (their staging of long windows sets the count: at m = 1024,
+// 22.3 KiB per wave, 7 waves instead of 6); the DTW runs one pair per wave there, so a deeper
+// queue buys nothing.
+__host__ __device__ constexpr int dtw_queue_cap(bool gs) { return gs ? 64 : Q3CAP; }
+__host__ __device__ constexpr int keogh_queue_cap(bool gs) { return gs ? 128 : Q2CAP; }
 
 // Join kernels whose cascade stages packed envelope widths: those of bands of 129..255 offsets
 // (two offset pairs per lane), which come with long windows and run two waves per SIMD.
@@ -283,14 +291,20 @@ struct CascadeLDS {
     float *sai, *saj, *rmu, *ris, *cmu, *cis;
     float *sui = nullptr, *sli = nullptr, *suj = nullptr, *slj = nullptr;  // without PW
     uint32_t *swi = nullptr, *swj = nullptr;                              // with PW
-    __device__ CascadeLDS(float *base, int m) {
+    // gs: the column statistics are not staged (cmu, cis unused)
+    __device__ CascadeLDS(float *base, int m, bool gs = false) {
         asm volatile("" : "+s"(m));
         const int CI = RC + m, CJ = RC + DG + m;
         rmu = base;
         ris = rmu + RC;
-        cmu = ris + RC;
-        cis = cmu + (RC + DG);
-        sai = cis + (RC + DG);
+        if (gs) {
+            cmu = cis = nullptr;
+            sai = ris + RC;
+        } else {
+            cmu = ris + RC;
+            cis = cmu + (RC + DG);
+            sai = cis + (RC + DG);
+        }
         if constexpr (PW) {
             saj = sai + CI;
             swi = reinterpret_cast<uint32_t *>(saj + CJ);
@@ -325,6 +339,8 @@ struct CascadeLDS {
 };
 static_assert((511 + 3) / 2 + 1 <= 2 * RC + 2 * (RC + DG),
               "DTW reads before a row window stay in the statistics for the widest band, w = 511");
+static_assert((127 + 3) / 2 + 1 <= 2 * RC,
+              "without staged column statistics (bands up to w = 127), in the row statistics");
 
 // Suffix bound of a DTW from the pair's LB_Keogh totals, which LB_Keogh formed term by term with
 // the DTW's own arithmetic (fz_term): the sum over the rows (columns) from R on is the total minus
@@ -481,7 +497,7 @@ __device__ inline int pair_col(uint32_t p) { return (int)(p >> 16); }
 // device work queue is the range [0, item_base[nseries]) claimed through work_counter), then
 // the claim-order multiplier;
 // uni: floats of the LDS union area, sized for the longest window of the launch.
-template <int Q, bool PW, bool SW>
+template <int Q, bool PW, bool SW, bool GS = false>
 __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base,
                                           int32_t nseries, int32_t uni, unsigned long long *work_counter, uint32_t *queue) {
     extern __shared__ __align__(16) float smem[];  // 16-byte aligned: the sweep reads float4 rows
@@ -532,11 +548,29 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     // cascade staging (valid only while draining), laid out per series from its window length
     uint32_t *plist = reinterpret_cast<uint32_t *>(smem + uni);  // expanded pairs of a batch
     uint32_t *q2 = plist + PLCAP;                                // LB_Keogh queue
-    uint32_t *q3 = q2 + Q2CAP;                                   // DTW queue
-    uint32_t *qp = q3 + Q3CAP;                                   // segment-mean queue
+    constexpr int Q2C = keogh_queue_cap(GS), Q3C = dtw_queue_cap(GS);
+    uint32_t *q3 = q2 + Q2C;                                     // DTW queue
+    uint32_t *qp = q3 + Q3C;                                     // segment-mean queue
     // kernels with several offset pairs per lane: the LB_Keogh totals of each DTW queue entry
     // (columns of window j against the envelope of window i, rows of i against that of j)
-    float *q3ta = reinterpret_cast<float *>(qp + QPCAP), *q3tb = q3ta + Q3CAP;
+    float *q3ta = reinterpret_cast<float *>(qp + QPCAP), *q3tb = q3ta + Q3C;
+    // GS: centred mean and 1/sigma of the part's column windows read from global memory (the
+    // WinRow records hold 1/sigma at float 2 of 8), window index clamped to the part's last
+    // window in the series
+    global_ptr<float> gcmu = nullptr, gcis = nullptr;
+    int gclim = 0;
+    auto col_mu = [&](const CascadeLDS<PW> &L, int k) __attribute__((always_inline)) -> float {
+        if constexpr (GS)
+            return gcmu[k < gclim ? k : gclim];
+        else
+            return L.cmu[k];
+    };
+    auto col_is = [&](const CascadeLDS<PW> &L, int k) __attribute__((always_inline)) -> float {
+        if constexpr (GS)
+            return gcis[8 * (k < gclim ? k : gclim)];
+        else
+            return L.cis[k];
+    };
     // the cascade's DTW, LB_Keogh and LB_Kim form cell differences from raw values (see
     // StagedZ): in the kernels with several offset pairs per lane, and in the one-pair-per-lane
     // kernels of launches whose widest band is 4 or more (SW; bands of 16 and more there also
@@ -564,7 +598,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     // (segments of w + 1 lanes); wider bands one pair per wave with QP offset pairs per lane.
     auto process_q3 = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
-        const CascadeLDS<PW> L(smem, m);
+        const CascadeLDS<PW> L(smem, m, GS);
         const int w = A.w;
         const float c8m = 8.f * sqrtf((float)m), s2m = sqrtf(2.f * (float)m);
         const int seg_len = w <= 63 ? w + 1 : 64;
@@ -598,16 +632,17 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 lj = pair_col(pr);
                 ii = L.ris[li];
                 nmi = -L.rmu[li] * ii;
-                ij = L.cis[lj];
-                nmj = -L.cmu[lj] * ij;
+                ij = col_is(L, lj);
+                nmj = -col_mu(L, lj) * ij;
             }
             // the pair's own error bound and DTW threshold (the same on every lane of its segment)
-            const float perr = pair_cell_err(A, c8m, L.rmu[li], ii, L.cmu[lj], ij);
+            const float mcj = col_mu(L, lj);
+            const float perr = pair_cell_err(A, c8m, L.rmu[li], ii, mcj, ij);
             const float ub = seg < nseg ? pair_dtw_ub(th, sqrtf(th.T2), perr, s2m, A) : th.dtw_ub;
             using ZT = std::conditional_t<FZ, StagedZ, StagedZPlain>;
             ZT z;
             if constexpr (FZ)
-                z = StagedZ{L.sai + li, L.saj + lj, ii, -ij, seg < nseg ? pair_bias(L.rmu[li], ii, L.cmu[lj], ij) : 0.f};
+                z = StagedZ{L.sai + li, L.saj + lj, ii, -ij, seg < nseg ? pair_bias(L.rmu[li], ii, mcj, ij) : 0.f};
             else
                 z = StagedZPlain{L.sai + li, L.saj + lj, ii, nmi, ij, nmj};
             int t_stop = 0;
@@ -633,13 +668,13 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                     sj = pair_col(pr);
                     sii = L.ris[si];
                     snmi = -L.rmu[si] * sii;
-                    sij = L.cis[sj];
-                    snmj = -L.cmu[sj] * sij;
+                    sij = col_is(L, sj);
+                    snmj = -col_mu(L, sj) * sij;
                 }
                 const StagedZPlain zs{L.sai + si, L.saj + sj, sii, snmi, sij, snmj};
                 float sc = 0.f;  // the fused form's pair bias, as the DTW forms it
                 if constexpr (FZ) {
-                    if (sseg < nseg) sc = pair_bias(L.rmu[si], sii, L.cmu[sj], sij);
+                    if (sseg < nseg) sc = pair_bias(L.rmu[si], sii, col_mu(L, sj), sij);
                 }
                 float cA = 0.f, cB = 0.f;
 #pragma unroll
@@ -724,9 +759,9 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                         ts.si = pair_row(pr);
                         ts.sj = pair_col(pr);
                         ts.ii = L.ris[ts.si];
-                        const float gij = L.cis[ts.sj];
+                        const float gij = col_is(L, ts.sj);
                         ts.nij = -gij;
-                        ts.c = pair_bias(L.rmu[ts.si], ts.ii, L.cmu[ts.sj], gij);
+                        ts.c = pair_bias(L.rmu[ts.si], ts.ii, col_mu(L, ts.sj), gij);
                     }
                     ts.rbase = seg * slen;
                     ts.totC = seg < nseg ? q3ta[e0 + seg] : 0.f;
@@ -769,7 +804,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     // the next pair as soon as its pair is abandoned or complete.  Consumes the whole queue.
     auto process_q2 = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
-        const CascadeLDS<PW> L(smem, m);
+        const CascadeLDS<PW> L(smem, m, GS);
         __syncthreads();  // q2 entries were written by other lanes
         // th: the drain batch's thresholds (at most one batch old)
         const float sm = sqrtf((float)m);
@@ -807,8 +842,8 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 const uint32_t pr = q2[idx];
                 li = pair_row(pr);
                 lj = pair_col(pr);
-                const float isi = L.ris[li], isj = L.cis[lj];
-                const float mci = L.rmu[li], mcj = L.cmu[lj];
+                const float isi = L.ris[li], isj = col_is(L, lj);
+                const float mci = L.rmu[li], mcj = col_mu(L, lj);
                 if constexpr (FZ) {
                     // rA = 1/sigma_i, rB = -1/sigma_j, bA = the pair's bias (StagedZ)
                     rA = isi;
@@ -978,7 +1013,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 q3n += __popcll(dk);
                 count(C_KEOGH, __popcll(dk));
                 if (done) act = false;
-                if (q3n > Q3CAP - 64) break;
+                if (q3n > Q3C - 64) break;
             }
         }
         if (q3n > 0) {
@@ -1023,7 +1058,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     // LB_Keogh and DTW; everything is flushed before the next part is staged.
     auto drain = [&]() __attribute__((always_inline)) {
         const JoinArgs &A = *Ap;
-        const CascadeLDS<PW> L(smem, m);
+        const CascadeLDS<PW> L(smem, m, GS);
         const int64_t n = A.n, nw = A.nw;
         const int CI = RC + m, CJ = RC + DG + m;
         __syncthreads();  // queue entries were stored by other lanes; column tables retire
@@ -1126,7 +1161,17 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 stage3(L.saj, L.suj, L.slj, cb + d0, CJ);
             }
             stage_stats(L.rmu, L.ris, cb, RC);
-            stage_stats(L.cmu, L.cis, cb + d0, RC + DG);
+            if constexpr (GS) {
+                // the part's first column window, cb + d0, lies in the series whenever one of
+                // its pairs does; the clamps keep every read inside the series regardless
+                const int64_t cg = cb + d0 < nw - 1 ? cb + d0 : nw - 1;
+                gcmu = (global_ptr<float>)(A.mu_c + cg);
+                gcis = (global_ptr<float>)A.win + 8 * cg + WISIG;
+                const int64_t lim = nw - 1 - cg;
+                gclim = lim < 0 ? 0 : (lim > RC + DG - 1 ? RC + DG - 1 : (int)lim);
+            } else {
+                stage_stats(L.cmu, L.cis, cb + d0, RC + DG);
+            }
         }
         __syncthreads();
         const int row0 = (int)(cb - i_beg);  // part start relative to the item start
@@ -1186,7 +1231,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                     pj = pair_col(pr);
                     const float *ai = L.sai + pi, *aj = L.saj + pj;
                     // the 16 LDS reads of the pair go out together (one wait), then the arithmetic
-                    const float mi = L.rmu[pi], ii = L.ris[pi], mj = L.cmu[pj], ij = L.cis[pj];
+                    const float mi = L.rmu[pi], ii = L.ris[pi], mj = col_mu(L, pj), ij = col_is(L, pj);
                     const float a0 = ai[0], a1 = ai[1], a2 = ai[2], b0 = aj[0], b1 = aj[1], b2 = aj[2];
                     const float ae0 = ai[m - 1], ae1 = ai[m - 2], ae2 = ai[m - 3];
                     const float be0 = aj[m - 1], be1 = aj[m - 2], be2 = aj[m - 3];
@@ -1257,7 +1302,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 }
                 // queued pairs refer to this part's staging: all of them run before the next part
                 if (paa && (qpn >= 64 || (last && qpn > 0))) process_qp();
-                if (q2n >= Q2RUN || (last && q2n > 0)) process_q2();
+                if (q2n >= Q2C - 64 || (last && q2n > 0)) process_q2();
                 if (p + 64 >= total) break;
             }
             __syncthreads();  // plist is rewritten by the next batch
@@ -1724,11 +1769,11 @@ __global__ __launch_bounds__(64) void k_join(const JoinArgs *__restrict__ series
 // The same join held to the register budget of two waves per SIMD (a few spills to scratch):
 // used for bands of two offset pairs per lane, whose packed-envelope staging lets two waves
 // per SIMD fit the LDS up to m = 1024.
-template <int Q>
+template <int Q, bool GS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_join_occ2(
     const JoinArgs *__restrict__ series, const int64_t *__restrict__ item_base, int32_t nseries, int32_t uni,
     unsigned long long *work_counter, uint32_t *queue) {
-    join_body<Q, true, false>(series, item_base, nseries, uni, work_counter, queue);
+    join_body<Q, true, false, GS>(series, item_base, nseries, uni, work_counter, queue);
 }
 
 // ---- set-up of every series of a join: the FP32 error model from the preprocessing's value
@@ -1945,9 +1990,24 @@ int launch_paa_pairs(const float *paa, int32_t nseg, int32_t len, const int64_t 
 
 namespace {
 // tot: the kernel carries LB_Keogh totals with its DTW queue (several offset pairs per lane)
-size_t join_lds_bytes_pw(int32_t m, bool pw, bool tot = false) {
-    return sizeof(float) * (size_t)union_floats(m, pw) + sizeof(uint32_t) * (PLCAP + Q2CAP + Q3CAP + QPCAP) +
-           (tot ? sizeof(float) * 2 * Q3CAP : 0);
+size_t join_lds_bytes_pw(int32_t m, bool pw, bool tot = false, bool gs = false) {
+    const int q3 = dtw_queue_cap(gs);
+    return sizeof(float) * (size_t)union_floats(m, pw, gs) +
+           sizeof(uint32_t) * (PLCAP + keogh_queue_cap(gs) + q3 + QPCAP) + (tot ? sizeof(float) * 2 * q3 : 0);
+}
+// Waves per CU that LDS of the given dynamic size allows (160 KiB, allocated in 512-byte
+// blocks, with the kernel's static stage counters), at most the 8 that the registers allow.
+int lds_waves_per_cu(size_t bytes) {
+    const size_t b = (bytes + 8 * kWcnt + 511) / 512 * 512;
+    const int n = (int)(160 * 1024 / b);
+    return n < 8 ? n : 8;
+}
+// The gs form for launches of two offset pairs per lane whose LDS then holds more waves per CU
+// (measured at m = 600, where both hold 8: 2.3 % slower).
+bool join_gs(int Q, int32_t m) {
+    if (Q != 2) return false;
+    return lds_waves_per_cu(join_lds_bytes_pw(m, true, true, true)) >
+           lds_waves_per_cu(join_lds_bytes_pw(m, true, true, false));
 }
 // Packed FP16 envelope staging: always for two offset pairs per lane (k_join_occ2), and for one
 // pair per lane when FP32 staging of windows of length m would leave fewer than the 8 waves per
@@ -1967,7 +2027,7 @@ bool join_sw(int Q, int32_t max_w) { return Q == 1 && max_w >= 4; }
 size_t join_lds_bytes(int32_t m, int32_t w) {
     const int Q = dtw_qp_for(w);
     const bool tot = Q >= 2 || join_sw(Q, w);
-    return join_lds_bytes_pw(m, join_packed(Q, m, tot), tot);
+    return join_lds_bytes_pw(m, join_packed(Q, m, tot), tot, join_gs(Q, m));
 }
 
 int launch_join_setup(JoinArgs *d_series, int32_t nseries, unsigned long long *d_work, hipStream_t s) {
@@ -1982,7 +2042,8 @@ int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t ns
     const bool sw = join_sw(Q, max_w);
     const bool pw = join_packed(Q, max_m, Q >= 2 || sw);
     const size_t lds = join_lds_bytes(max_m, max_w);
-    const int32_t uni = union_floats(max_m, pw);
+    const bool gs = join_gs(Q, max_m);
+    const int32_t uni = union_floats(max_m, pw, gs);
     if (lds + 8 * kWcnt > 160 * 1024) {  // + the kernel's static stage counters
         set_error("window too long for the join's LDS staging");
         return DTWJ_EINVAL;
@@ -2002,8 +2063,12 @@ int launch_join(const JoinArgs *d_series, const int64_t *d_item_base, int32_t ns
             }
             break;
         case 2:  // two waves per SIMD (packed envelope staging)
-            hipLaunchKernelGGL(k_join_occ2<2>, dim3(grid), dim3(64), lds, s, d_series, d_item_base, nseries, uni,
-                               d_work, d_queue);
+            if (gs)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_join_occ2<2, true>), dim3(grid), dim3(64), lds, s, d_series,
+                                   d_item_base, nseries, uni, d_work, d_queue);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_join_occ2<2, false>), dim3(grid), dim3(64), lds, s, d_series,
+                                   d_item_base, nseries, uni, d_work, d_queue);
             break;
         case 4: DTWJ_LAUNCH_JOIN(4, false, false); break;
         case 8: DTWJ_LAUNCH_JOIN(8, false, false); break;

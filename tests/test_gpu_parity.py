@@ -116,6 +116,21 @@ def test_top1_wide_bands_match_oracle(gpu, oracle, n, m, w, seed):
         assert r.pairs_swept == r.pairs_total
 
 
+@pytest.mark.parametrize("n,m,w,seed", [(3000, 256, 70, 14), (4000, 800, 80, 15), (3500, 1024, 120, 16)])
+def test_top1_two_offset_pair_bands_match_oracle(gpu, oracle, n, m, w, seed):
+    """Bands of 129..255 offsets run two offset pairs per lane.  Where that fits more waves per
+    CU (m = 800, 1024 here) the kernel reads the column windows' mean and 1/sigma from global
+    memory and runs shorter cascade queues; at m = 256 it stages them in LDS.  Both forms, in
+    every sweep mode and with small tiles (several items and drains per wave)."""
+    a = rw(n, seed)
+    o = oracle.motif_discovery(a, m, w)
+    for opts in ({}, {"sweep": 1, "rows_per_item": 512, "waves_per_cu": 1}, {"sweep": 2}):
+        r = gpu.top1(a, m, w, **opts)
+        assert (r.i, r.j) == (o.i, o.j), (opts, r.i, r.j, o.i, o.j)
+        assert abs(r.dist - o.bsf) <= 1e-9 * o.bsf
+        assert r.pairs_swept == r.pairs_total
+
+
 def test_single_pair_and_extreme_bands(gpu, oracle):
     a = rw(256, 21)
     r = gpu.top1(a, 128, 5)                     # n = 2m: exactly one admissible pair
