@@ -56,6 +56,22 @@ def test_batch_with_several_band_classes_matches_oracle(gpu, oracle):
             assert abs(res[t].dist - o.bsf) <= 1e-9 * o.bsf
 
 
+def test_batch_of_two_offset_pair_bands_with_long_windows_matches_oracle(gpu, oracle):
+    """Widest band 205 offsets (two offset pairs per lane) and longest window 1024: the batch
+    runs the form that reads column-window statistics from global memory (each series' own) and
+    sizes LDS by m = 1024, with shorter members and narrower bands beside it."""
+    mix = [(3500, 1024, 102, 140), (3000, 256, 13, 141), (4000, 800, 80, 142), (2500, 512, 70, 143)]
+    series = [np.cumsum(np.random.default_rng(seed).standard_normal(n)) for n, _, _, seed in mix]
+    ms, ws = [m for _, m, _, _ in mix], [w for _, _, w, _ in mix]
+    refs = [oracle.motif_discovery(a, m, w) for a, m, w in zip(series, ms, ws)]
+    for opts in ({}, {"graph": -1}):
+        res = gpu.top1_batch(series, ms, ws, **opts)
+        for t, (o, m, w) in enumerate(zip(refs, ms, ws)):
+            assert not isinstance(res[t], Exception), (t, res[t])
+            assert (res[t].i, res[t].j) == (o.i, o.j), (opts, t, m, w)
+            assert abs(res[t].dist - o.bsf) <= 1e-9 * o.bsf
+
+
 def test_graph_and_direct_launch_agree_and_repeat(gpu):
     series, ms, ws = make_batch(gpu, MIX[:6])
     ctx = gpu.Context()
