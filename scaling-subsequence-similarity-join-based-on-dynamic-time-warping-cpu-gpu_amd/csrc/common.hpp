@@ -134,6 +134,7 @@ struct JoinArgs {
     const float *lc;       // n, L - c0
     const uint32_t *wc;    // n, envelope widths (U - a, a - L) as FP16 rounded up: lo, hi half
     const float *mu_c;     // nw, mu - c0
+    const float *isg;      // nw, 1 / sigma (the WinRow value, as a flat array)
     const float *blkmax;   // 3 x nblk: max sigma, max sv, max 1/sigma over 256-window blocks
     const float *gstat;    // preprocessing statistics (value range, see Prep::gstat)
     const int32_t *flags;  // preprocessing flags (non-finite input, constant window)

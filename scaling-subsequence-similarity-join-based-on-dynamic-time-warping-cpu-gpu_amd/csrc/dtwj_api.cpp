@@ -272,6 +272,7 @@ size_t job_bytes(const Job &J) {
     add(4 * n * 3);       // ac uc lc
     add(4 * n);           // wc
     add(4 * nw);          // mu_c
+    add(4 * nw);          // isg
     add(sizeof(WinRow) * nw);
     add(4 * 3 * nblk);
     add(4 * 8);           // gstat
@@ -308,6 +309,7 @@ void take_job(Arena &ar, Job &J) {
     P.lc = ar.take<float>(n);
     P.wc = ar.take<uint32_t>(n);
     P.mu_c = ar.take<float>(nw);
+    P.isg = ar.take<float>(nw);
     P.win = ar.take<WinRow>(nw);
     P.nblk = (int32_t)((nw + 255) / 256);
     P.blkmax = ar.take<float>(3 * (size_t)P.nblk);
@@ -337,6 +339,7 @@ void take_job(Arena &ar, Job &J) {
     A.lc = P.lc;
     A.wc = P.wc;
     A.mu_c = P.mu_c;
+    A.isg = P.isg;
     A.blkmax = P.blkmax;
     A.gstat = P.gstat;
     A.flags = P.flags;
@@ -1654,6 +1657,7 @@ int dtwj_dtw_pairs(const double *series, int64_t n, int32_t m, int32_t w, const 
         A.win = P.win;
         A.ac = P.ac;
         A.mu_c = P.mu_c;
+        A.isg = P.isg;
         A.n = n;
         A.nw = nw;
         A.m = m;

@@ -20,6 +20,7 @@ struct Prep {
     float *ac = nullptr, *uc = nullptr, *lc = nullptr;  // n
     uint32_t *wc = nullptr;                             // n: FP16 widths uc - ac (lo), ac - lc (hi)
     float *mu_c = nullptr;                              // nw
+    float *isg = nullptr;                               // nw: 1 / sigma (as in WinRow)
     WinRow *win = nullptr;                              // nw
     float *blkmax = nullptr;                            // 3 * nblk
     int32_t nblk = 0;

@@ -554,9 +554,9 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     // kernels with several offset pairs per lane: the LB_Keogh totals of each DTW queue entry
     // (columns of window j against the envelope of window i, rows of i against that of j)
     float *q3ta = reinterpret_cast<float *>(qp + QPCAP), *q3tb = q3ta + Q3C;
-    // GS: centred mean and 1/sigma of the part's column windows read from global memory (the
-    // WinRow records hold 1/sigma at float 2 of 8), window index clamped to the part's last
-    // window in the series
+    // GS: centred mean and 1/sigma of the part's column windows read from global memory (flat
+    // arrays, so that neighbouring lanes read neighbouring words), window index clamped to the
+    // part's last window in the series
     global_ptr<float> gcmu = nullptr, gcis = nullptr;
     int gclim = 0;
     auto col_mu = [&](const CascadeLDS<PW> &L, int k) __attribute__((always_inline)) -> float {
@@ -567,7 +567,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
     };
     auto col_is = [&](const CascadeLDS<PW> &L, int k) __attribute__((always_inline)) -> float {
         if constexpr (GS)
-            return gcis[8 * (k < gclim ? k : gclim)];
+            return gcis[k < gclim ? k : gclim];
         else
             return L.cis[k];
     };
@@ -1166,7 +1166,7 @@ __device__ __forceinline__ void join_body(const JoinArgs *__restrict__ series, c
                 // its pairs does; the clamps keep every read inside the series regardless
                 const int64_t cg = cb + d0 < nw - 1 ? cb + d0 : nw - 1;
                 gcmu = (global_ptr<float>)(A.mu_c + cg);
-                gcis = (global_ptr<float>)A.win + 8 * cg + WISIG;
+                gcis = (global_ptr<float>)(A.isg + cg);
                 const int64_t lim = nw - 1 - cg;
                 gclim = lim < 0 ? 0 : (lim > RC + DG - 1 ? RC + DG - 1 : (int)lim);
             } else {

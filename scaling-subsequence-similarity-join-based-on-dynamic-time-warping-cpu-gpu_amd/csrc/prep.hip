@@ -294,7 +294,7 @@ __device__ inline void atomic_min_pos(float *p, float x) { atomicMin((unsigned i
 __global__ __launch_bounds__(256) void k_window_stats(const double *__restrict__ a, const double *__restrict__ v,
                                                       const dd_t *__restrict__ P, const double *c0p, int64_t n,
                                                       int m, double *mu, double *sig, double *muv, float *mu_c,
-                                                      WinRow *win, float *blkmax, int nblk, float *gstat,
+                                                      float *isg, WinRow *win, float *blkmax, int nblk, float *gstat,
                                                       int32_t *flags) {
     __shared__ float r0[256], r1[256], r2[256], r3[256], r4[256];
     const double c0 = *c0p;
@@ -320,6 +320,7 @@ __global__ __launch_bounds__(256) void k_window_stats(const double *__restrict__
         }
         double is = s.sig > 0.0 ? 1.0 / s.sig : 0.0;
         row.isig = (float)is;
+        isg[i] = row.isig;
         row.mN = (float)((double)m + s.N);
         row.R = (float)s.R;
         row.sig = (float)s.sig;
@@ -444,7 +445,7 @@ int prep_series(Prep &P, hipStream_t s) {
     hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(kScanThreads), 0, s, P.a, P.U, P.L, P.c0, n, nb, P.bsum,
                        P.P);
     hipLaunchKernelGGL(k_window_stats, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, P.a, P.v, P.P, P.c0, n, m,
-                       P.mu, P.sig, P.muv, P.mu_c, P.win, P.blkmax, P.nblk, P.gstat, P.flags);
+                       P.mu, P.sig, P.muv, P.mu_c, P.isg, P.win, P.blkmax, P.nblk, P.gstat, P.flags);
     hipLaunchKernelGGL(k_positions, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P.a, P.U, P.L, P.c0, n, P.ac,
                        P.uc, P.lc, P.wc, P.gstat);
     if (P.paa) {
