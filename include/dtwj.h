@@ -291,6 +291,16 @@ int dtwj_envelope(const double *series, int64_t n, int32_t w, double *lower, dou
 int dtwj_dtw_pairs(const double *series, int64_t n, int32_t m, int32_t w, const int64_t *pi,
                    const int64_t *pj, int64_t npairs, int32_t prec, double *cost);
 
+/* LB_Keogh of window j against the radius-w envelope of window i (all m positions) plus the
+ * reference's LB_Petitjean increments on positions [3, m - 3): window j projected into window
+ * i's envelope, the radius-w envelope of that projection, and what window i pays outside it
+ * beyond window j's own envelope.  FP64 on z-normalised windows, no abandoning; m <= 4096.
+ * Replaces lbPetitjean_new (CPU_version1/LB_PETITJEAN.cpp:9-123) as the reference calls it in
+ * its CPU join (CPU_version1/new_dtw_motifGUI.cpp:514-520, q = window j, x = window i); the
+ * join's own cascade does not run it (a building block, like the masked ball bound). */
+int dtwj_lb_petitjean_pairs(const double *series, int64_t n, int32_t m, int32_t w, const int64_t *pi,
+                            const int64_t *pj, int64_t npairs, double *lb);
+
 /* Segment-mean (PAA) lower bound of LB_Keogh that the join applies between LB_Kim and
  * LB_Keogh, for an explicit pair list: each window cut into S segments of len = floor(m / S)
  * samples (S = 32 for m >= 512, else 16), len * sum over segments of max(Z - U, L - Z, 0)^2 of

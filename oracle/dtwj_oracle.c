@@ -1680,6 +1680,28 @@ int orc_bruteforce_top1(const double *a, int64_t n, int32_t m, int32_t w, int64_
     return 0;
 }
 
+/* LB_Keogh of window j against the radius-w envelope of window i over all m positions (no
+ * abandoning), plus the LB_Petitjean increments of lb_petitjean above (q = window j, x = window
+ * i: the orientation of the join's second LB_KK direction, CPU_version1/new_dtw_motifGUI.cpp:
+ * 514-520), both windows z-normalised by orc_znorm, per-window envelopes by orc_lemire. */
+double orc_lb_petitjean_pair(const double *a, int64_t n, int32_t m, int32_t w, int64_t i, int64_t j) {
+    double *buf = (double *)xcalloc((size_t)m * 9, sizeof(double));
+    double *zi = buf, *zj = buf + m, *ui = buf + 2 * m, *li = buf + 3 * m, *uj = buf + 4 * m, *lj = buf + 5 * m;
+    double *p = buf + 6 * m, *up = buf + 7 * m, *lp = buf + 8 * m;
+    double lbk = -1.0, cb_unused = 0.0;
+    (void)cb_unused;
+    if (orc_znorm(a, n, i, m, zi) == 0 && orc_znorm(a, n, j, m, zj) == 0) {
+        orc_lemire(zi, m, w, li, ui);
+        orc_lemire(zj, m, w, lj, uj);
+        lbk = orc_lb_keogh_sq(zi, zj, m, w);
+        double *cb = (double *)xcalloc((size_t)m, sizeof(double));
+        lb_petitjean(p, up, lp, zj, uj, lj, zi, ui, li, w, m, ORC_INF, &lbk, cb);
+        free(cb);
+    }
+    free(buf);
+    return lbk;
+}
+
 double orc_lb_keogh_sq(const double *zi, const double *zj, int32_t m, int32_t w) {
     double s = 0.0;
     for (int k = 0; k < m; k++) {

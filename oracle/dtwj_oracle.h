@@ -97,6 +97,9 @@ int orc_bruteforce_top1(const double *a, int64_t n, int32_t m, int32_t w, int64_
 /* textbook two-sided-envelope LB_Keogh of z_j against the radius-w envelope of z_i
    clipped at the window ends (squared) */
 double orc_lb_keogh_sq(const double *zi, const double *zj, int32_t m, int32_t w);
+/* LB_Keogh of window j against window i's envelope plus the LB_Petitjean increments
+ * (CPU_version1/LB_PETITJEAN.cpp:9-123), no abandoning; -1 for a constant window. */
+double orc_lb_petitjean_pair(const double *a, int64_t n, int32_t m, int32_t w, int64_t i, int64_t j);
 
 #ifdef __cplusplus
 }

@@ -57,6 +57,9 @@ def lib():
         L.orc_bruteforce_top1.argtypes = [dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, lp, lp, dp]
         L.orc_lb_keogh_sq.argtypes = [dp, dp, ctypes.c_int32, ctypes.c_int32]
         L.orc_lb_keogh_sq.restype = ctypes.c_double
+        L.orc_lb_petitjean_pair.argtypes = [dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                                            ctypes.c_int64]
+        L.orc_lb_petitjean_pair.restype = ctypes.c_double
         u8p = ctypes.POINTER(ctypes.c_uint8)
         L.orc_global_mask.argtypes = [dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, dp, dp, u8p, dp, dp, ip]
         L.orc_masked_ball_diag.argtypes = [dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, dp, dp,
@@ -144,6 +147,11 @@ def bruteforce_top1(a, m, w):
     if lib().orc_bruteforce_top1(_d(a), a.size, m, w, ctypes.byref(bi), ctypes.byref(bj), ctypes.byref(d)):
         raise ValueError("bruteforce failed")
     return bi.value, bj.value, d.value
+
+
+def lb_petitjean_pair(a, m, w, i, j):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return lib().orc_lb_petitjean_pair(_d(a), a.size, m, w, int(i), int(j))
 
 
 def lb_keogh_sq(zi, zj, w):

@@ -112,7 +112,7 @@ EXPORTED = [
     "dtwj_gen_spec_default", "dtwj_generate", "dtwj_top1_batch",
     "dtwj_context_top1_batch", "dtwj_proposal_candidates", "dtwj_mlp_forward", "dtwj_mlp_gradient",
     "dtwj_mlp_train", "dtwj_top1_learned", "dtwj_plan", "dtwj_top1_device_noalloc", "dtwj_matrix_profile",
-    "dtwj_paa_pairs", "dtwj_global_mask", "dtwj_masked_ball_pairs",
+    "dtwj_paa_pairs", "dtwj_lb_petitjean_pairs", "dtwj_global_mask", "dtwj_masked_ball_pairs",
 ]
 
 _lib = None
@@ -150,6 +150,8 @@ def lib():
     L.dtwj_dtw_pairs.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, i64_p, i64_p, ctypes.c_int64,
                                  ctypes.c_int32, d_p]
     L.dtwj_paa_pairs.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, i64_p, i64_p, ctypes.c_int64, d_p]
+    L.dtwj_lb_petitjean_pairs.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, i64_p, i64_p,
+                                          ctypes.c_int64, d_p]
     L.dtwj_global_mask.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, P(ctypes.c_uint8), d_p, d_p,
                                    i32_p]
     L.dtwj_masked_ball_pairs.argtypes = [d_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, i64_p, i64_p,
@@ -507,6 +509,19 @@ def dtw_pairs(series, m, w, pi, pj, prec=32):
     i64 = ctypes.POINTER(ctypes.c_int64)
     _check(lib().dtwj_dtw_pairs(_dptr(a), a.size, int(m), int(w), pi.ctypes.data_as(i64), pj.ctypes.data_as(i64),
                                 pi.size, int(prec), _dptr(out)))
+    return out
+
+
+def lb_petitjean_pairs(series, m, w, pi, pj):
+    """LB_Keogh of window pj[k] against window pi[k]'s envelope plus the LB_Petitjean
+    increments, FP64 on the GPU, for each window pair (pi[k], pj[k])."""
+    a = _as_series(series)
+    pi = np.ascontiguousarray(np.asarray(pi, dtype=np.int64))
+    pj = np.ascontiguousarray(np.asarray(pj, dtype=np.int64))
+    out = np.empty(pi.size)
+    i64 = ctypes.POINTER(ctypes.c_int64)
+    _check(lib().dtwj_lb_petitjean_pairs(_dptr(a), a.size, int(m), int(w), pi.ctypes.data_as(i64),
+                                         pj.ctypes.data_as(i64), pi.size, _dptr(out)))
     return out
 
 

@@ -196,4 +196,83 @@ int launch_dtw_pairs(const JoinArgs &A, const double *sig64, const int64_t *pi, 
     return launch_pairs_t<float>(A, sig64, pi, pj, npairs, cost, s);
 }
 
+// LB_Keogh of window j against the radius-w envelope of window i (all m positions) plus the
+// LB_Petitjean increments (CPU_version1/LB_PETITJEAN.cpp:9-123; q = window j, x = window i):
+// project q into x's envelope on positions [3, m - 3), take the radius-w envelope of the
+// projection, and add, per position of [3, m - 3), what x pays outside it beyond what q's own
+// envelope already accounts for.  FP64, one block of 256 threads per pair, z-windows in LDS;
+// every envelope value is a direct max / min over its clipped radius-w range.
+__global__ __launch_bounds__(256) void k_lb_petitjean(const double *__restrict__ a, const double *__restrict__ mu,
+                                                      const double *__restrict__ sig, int m, int w,
+                                                      const int64_t *__restrict__ pi, const int64_t *__restrict__ pj,
+                                                      double *__restrict__ out) {
+    extern __shared__ double sh[];
+    double *zi = sh, *zj = sh + m, *pr = sh + 2 * m;
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    const int64_t i = pi[blockIdx.x], j = pj[blockIdx.x];
+    const double mi = mu[i], si = sig[i], mj = mu[j], sj = sig[j];
+    for (int k = t; k < m; k += 256) {
+        zi[k] = (a[i + k] - mi) / si;
+        zj[k] = (a[j + k] - mj) / sj;
+    }
+    __syncthreads();
+    auto env = [&](const double *z, int k, double &u, double &l) {
+        const int lo = k - w > 0 ? k - w : 0, hi = k + w < m - 1 ? k + w : m - 1;
+        u = l = z[lo];
+        for (int r = lo + 1; r <= hi; ++r) {
+            u = fmax(u, z[r]);
+            l = fmin(l, z[r]);
+        }
+    };
+    auto sq = [](double x) { return x * x; };
+    double acc = 0.0;
+    for (int k = t; k < m; k += 256) {
+        double u, l;
+        env(zi, k, u, l);
+        const double q = zj[k];
+        acc += q > u ? sq(q - u) : (q < l ? sq(q - l) : 0.0);
+        double p = q;
+        if (k >= 3 && k < m - 3) p = q > u ? u : (q < l ? l : q);
+        pr[k] = p;
+    }
+    __syncthreads();
+    for (int k = t + 3; k < m - 3; k += 256) {
+        double up, lp, uq, lq;
+        env(pr, k, up, lp);
+        env(zj, k, uq, lq);
+        const double x = zi[k];
+        double d = 0.0;
+        if (x > up && up > uq)
+            d = sq(x - uq) - sq(up - uq);
+        else if (x < lp && lp < lq)
+            d = sq(x - lq) - sq(lp - lq);
+        else if (x > up)
+            d = sq(x - up);
+        else if (x < lp)
+            d = sq(x - lp);
+        acc += d;
+    }
+    red[t] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    if (t == 0) out[blockIdx.x] = red[0];
+}
+
+int launch_lb_petitjean(const double *a, const double *mu, const double *sig, int32_t m, int32_t w, const int64_t *pi,
+                        const int64_t *pj, int64_t npairs, double *out, hipStream_t s) {
+    if (npairs <= 0) return 0;
+    if (npairs > 0x7fffffffLL || m > 4096) {
+        set_error("LB_Petitjean pairs: at most 2^31 - 1 pairs and m <= 4096");
+        return DTWJ_EINVAL;
+    }
+    hipLaunchKernelGGL(k_lb_petitjean, dim3((unsigned)npairs), dim3(256), sizeof(double) * 3 * (size_t)m, s, a, mu, sig,
+                       m, w, pi, pj, out);
+    DTWJ_HIP(hipGetLastError());
+    return 0;
+}
+
 }  // namespace dtwj

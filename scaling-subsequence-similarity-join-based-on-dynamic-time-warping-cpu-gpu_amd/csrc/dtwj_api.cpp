@@ -1676,6 +1676,44 @@ int dtwj_dtw_pairs(const double *series, int64_t n, int32_t m, int32_t w, const 
     return rc;
 }
 
+int dtwj_lb_petitjean_pairs(const double *series, int64_t n, int32_t m, int32_t w, const int64_t *pi,
+                            const int64_t *pj, int64_t npairs, double *lb) {
+    if (int rc = validate(series, n, m, w)) return rc;
+    if (m > 4096) return fail(DTWJ_EINVAL, "LB_Petitjean pairs need m <= 4096");
+    if (!pi || !pj || !lb || npairs < 0) return fail(DTWJ_EINVAL, "bad pair arguments");
+    const int64_t nw = n - m + 1;
+    for (int64_t p = 0; p < npairs; ++p)
+        if (pi[p] < 0 || pj[p] < 0 || pi[p] >= nw || pj[p] >= nw) return fail(DTWJ_EINVAL, "pair index out of range");
+    if (npairs == 0) return DTWJ_OK;
+    dtwj_context *c = nullptr;
+    Prep P;
+    const double *d_a = nullptr;
+    int rc = building_block_prep(series, n, m, std::min(w, m - 1), &c, P, &d_a);
+    int64_t *d_pi = nullptr, *d_pj = nullptr;
+    double *d_lb = nullptr;
+    if (rc == DTWJ_OK) {
+        if (hipMalloc(&d_pi, sizeof(int64_t) * npairs) != hipSuccess || hipMalloc(&d_pj, sizeof(int64_t) * npairs) != hipSuccess ||
+            hipMalloc(&d_lb, sizeof(double) * npairs) != hipSuccess)
+            rc = fail(DTWJ_ENOMEM, "device allocation failed");
+    }
+    if (rc == DTWJ_OK) {
+        hipError_t e = hipMemcpy(d_pi, pi, sizeof(int64_t) * npairs, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_pj, pj, sizeof(int64_t) * npairs, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy", __FILE__, __LINE__);
+    }
+    if (rc == DTWJ_OK) rc = launch_lb_petitjean(d_a, P.mu, P.sig, m, std::min(w, m - 1), d_pi, d_pj, npairs, d_lb, c->stream);
+    if (rc == DTWJ_OK) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(lb, d_lb, sizeof(double) * npairs, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "LB_Petitjean pairs", __FILE__, __LINE__);
+    }
+    if (d_pi) (void)hipFree(d_pi);
+    if (d_pj) (void)hipFree(d_pj);
+    if (d_lb) (void)hipFree(d_lb);
+    dtwj_context_destroy(c);
+    return rc;
+}
+
 int dtwj_paa_pairs(const double *series, int64_t n, int32_t m, int32_t w, const int64_t *pi, const int64_t *pj,
                    int64_t npairs, double *lb) {
     if (int rc = validate(series, n, m, w)) return rc;

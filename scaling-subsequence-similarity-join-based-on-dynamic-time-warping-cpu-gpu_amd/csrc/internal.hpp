@@ -130,6 +130,10 @@ int launch_masked_ball(const MaskBuffers &B, const int64_t *pi, const int64_t *p
 size_t noalloc_scratch_bytes();
 int noalloc_scratch(int device, void **ptr);
 
+// LB_Keogh plus the LB_Petitjean increments of explicit pairs (dtw.hip), FP64.
+int launch_lb_petitjean(const double *a, const double *mu, const double *sig, int32_t m, int32_t w, const int64_t *pi,
+                        const int64_t *pj, int64_t npairs, double *out, hipStream_t s);
+
 // Learned proposal (proposal.hip).
 int mlp_param_count();
 int mlp_feature_pad();
